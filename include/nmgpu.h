@@ -1,0 +1,153 @@
+/*
+ * nmgpu — MI355X (gfx950) filtered-Lanczos eigensolver for the generalized
+ * problem A~ x = lambda B~ x of planetary normal modes.
+ *
+ * A C-ABI shaped after the pEVSL Fortran-90 interface that NormalModes calls
+ * (src/mod_pevsl.f90, src/mod_matvec.f90).  Each entry names the reference
+ * call it replaces.  Conventions:
+ *   - every function returns 0 on success or a negative NMG_E* code, and
+ *     nmg_last_error() gives the message of the last failure on the calling
+ *     thread; no entry aborts the process;
+ *   - handles are opaque 64-bit integers (the reference keeps pEVSL handles
+ *     in integer*8, src/mod_cg_datatype.f90:92-96);
+ *   - CSR input is 0-based (rowdist(1) = 0, col-1; src/mod_matvec.f90:59);
+ *   - array arguments are HOST pointers unless the name ends in _dev;
+ *   - one HIP stream per solver handle; calls on one handle are serialised by
+ *     the caller (pEVSL is not re-entrant either).
+ */
+#ifndef NMGPU_H
+#define NMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t nmg_handle;
+
+enum {
+  NMG_OK = 0,
+  NMG_EINVAL = -1,   /* bad argument */
+  NMG_EHIP = -2,     /* HIP runtime / library failure */
+  NMG_ENOMEM = -3,   /* device allocation failed */
+  NMG_ESTATE = -4,   /* call out of order (e.g. no operator set) */
+  NMG_ENUM = -5,     /* numerical breakdown (non-SPD B, NaN, no convergence) */
+  NMG_ENOTIMPL = -6  /* requested variant not available */
+};
+
+/* precision of the Chebyshev filter (nmg_set_precision) */
+enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1 };
+
+const char *nmg_last_error(void);
+int nmg_version(int *major, int *minor);
+int nmg_device_info(char *name, int name_len, int *cu_count, double *hbm_gbytes);
+
+/* --- solver handle ------------------------------------------------------ */
+/* pEVSL_Start_F90(comm, h)      src/mod_pevsl.f90:54; src/mod_matvec.f90:75,152 */
+int nmg_start(int comm, nmg_handle *h);
+/* PEVSL_FINISH_F90(h)           src/mod_pevsl.f90:220; src/mod_matvec.f90:95 */
+int nmg_finish(nmg_handle h);
+/* PEVSL_SETPROBSIZES_F90(h, N, n_loc, nfirst)   src/mod_pevsl.f90:57 */
+int nmg_setprobsizes(nmg_handle h, int64_t n_glob, int64_t n_loc, int64_t nfirst);
+
+/* --- matrices ----------------------------------------------------------- */
+/* PEVSL_PARCSRCREATE_F90(nrow_g, ncol_g, row_starts, col_starts, ia, ja, a,
+ * comm, mat)   src/mod_matvec.f90:69-71, 117-119, 146-148, 196-198, 218-220,
+ * 242-244.  One process owns all rows (row_starts = col_starts = {0, n}).
+ * ia: int32[nrow+1] 0-based offsets, ja: int32[nnz] 0-based columns, a: f64.
+ * The matrix is copied to HBM once; the layout (3x3 block CSR, scalar mass
+ * x I3, or CSR) is chosen from its structure. */
+int nmg_parcsr_create(int nrow_g, int ncol_g, const int *row_starts, const int *col_starts,
+                      const int *ia, const int *ja, const double *a, int comm, nmg_handle *mat);
+/* the same with int64 row offsets (nnz beyond 2^31) */
+int nmg_csr_create(int64_t nrow, int64_t ncol, const int64_t *rowptr, const int32_t *col,
+                   const double *val, nmg_handle *mat);
+int nmg_csr_free(nmg_handle mat);
+/* layout chosen at creation: 0 CSR, 1 3x3-block CSR, 2 scalar mass (x) I3 */
+int nmg_csr_info(nmg_handle mat, int64_t *nrow, int64_t *ncol, int64_t *nnz, int *layout);
+/* pevsl_parcsrmatvec_f90(x, y, mat)  src/mod_matvec.f90:453,471,495,507,509,515 */
+int nmg_parcsr_matvec(const double *x, double *y, nmg_handle mat);
+/* Bdiagscaling / Apdiagscaling  src/mod_matvec.f90:252-342, 345-441:
+ * d_i = 1/sqrt(sign * M_ii), M_ij <- sign * d_i M_ij d_j, done in HBM.
+ * sign = +1 for B, -1 for Ap.  d: host f64[nrow] out (may be NULL). */
+int nmg_csr_diagscale(nmg_handle mat, double sign, double *d);
+
+/* --- operators of the eigenproblem ------------------------------------- */
+/* PEVSL_SETAMV_F90(h, sparseAV, mymatvec): w = D A D v  src/mod_pevsl.f90:77,
+ * src/mod_matvec.f90:445-458.  A is used through a scaled copy D A D. */
+int nmg_set_amv_dad(nmg_handle h, nmg_handle matA, const double *d);
+/* PEVSL_SETAMV_F90(h, sparseBV|sparseApV, ...) with a plain matrix:
+ * w = M v  (used for bounds of B~, src/mod_matvec.f90:80, and of Ap~ :157) */
+int nmg_set_amv(nmg_handle h, nmg_handle mat);
+/* PEVSL_SETBMV_F90(h, sparseBV, mymatvec)  src/mod_pevsl.f90:69 */
+int nmg_set_bmv(nmg_handle h, nmg_handle matB);
+/* PEVSL_SETBSOL_CHEBITER_F90(h, type, chebB)  src/mod_pevsl.f90:72-73 */
+int nmg_set_bsol_chebiter(nmg_handle h, int type, nmg_handle cheb);
+/* PEVSL_SET_GENEIG_F90(h)  src/mod_pevsl.f90:82 */
+int nmg_set_geneig(nmg_handle h);
+/* Filter precision (no reference counterpart: pEVSL is FP64 only). */
+int nmg_set_precision(nmg_handle h, int mode);
+/* Lanczos start vector seed (pEVSL draws a random vinit). */
+int nmg_set_seed(nmg_handle h, uint64_t seed);
+
+/* --- Chebyshev iteration for B~^{-1} ----------------------------------- */
+/* pEVSL_SETUP_CHEBITER_F90(lmin, lmax, deg, mat, cheb)
+ * src/mod_matvec.f90:88-93, 167-174.  deg = number of products with mat. */
+int nmg_setup_chebiter(double lmin, double lmax, int deg, nmg_handle mat, nmg_handle *cheb);
+/* pevsl_chebiter_f90(type, b, x, cheb)  src/mod_matvec.f90:480, 512 */
+int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
+/* pEVSL_chebiterstatsprint_f90(cheb)  src/mod_pevsl.f90:132-135 */
+int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
+int nmg_chebiter_free(nmg_handle cheb);
+
+/* --- eigensolver ------------------------------------------------------- */
+/* pEVSL_LANBOUNDS_F90(h, mlan, maxit, tol, lmin, lmax)
+ * src/mod_pevsl.f90:84; src/mod_matvec.f90:85, 162 */
+int nmg_lanbounds(nmg_handle h, int mlan, int maxit, double tol, double *lmin, double *lmax);
+/* pEVSL_FINDPOL_F90(xintv, thr_int, thr_ext, pol)  src/mod_pevsl.f90:115 */
+int nmg_findpol(const double *xintv, double thr_int, double thr_ext, nmg_handle *pol);
+/* polynomial parameters: degree, damping type, centre/half-width of the
+ * spectrum map, delta centre gam, acceptance bar; mu: f64[deg+1] or NULL */
+int nmg_pol_info(nmg_handle pol, int *deg, double *cc, double *dd, double *gam, double *bar,
+                 double *mu);
+/* pEVSL_FREEPOL_F90(pol)  src/mod_pevsl.f90:215 */
+int nmg_freepol(nmg_handle pol);
+/* pEVSL_CHEBLANNR_F90(h, xintv, maxit, tol, pol)  src/mod_pevsl.f90:122 */
+int nmg_cheblannr(nmg_handle h, const double *xintv, int maxit, double tol, nmg_handle pol);
+/* pEVSL_GET_NEV_F90(h, nev)  src/mod_pevsl.f90:124 */
+int nmg_get_nev(nmg_handle h, int *nev);
+/* pEVSL_COPY_RESULT_F90(h, val, vec, n_loc)  src/mod_pevsl.f90:130:
+ * eigenvalues ascending, vec column j at j*n_loc (scaled coordinates x~,
+ * B~-normalised).  vec may be NULL. */
+int nmg_copy_result(nmg_handle h, double *val, double *vec, int64_t n_loc);
+/* residual check of src/mod_pevsl.f90:137-162 on the device-resident result:
+ * res_ref[j] = sqrt(sum_i r_i^2 / N)/|lam_j| (reference definition) and
+ * res2[j] = ||r||_2/|lam_j| with x~ B~-normalised, r = A~x~ - lam B~x~. */
+int nmg_residuals(nmg_handle h, double *res_ref, double *res2);
+
+/* Tuning/diagnostics of the last nmg_cheblannr / nmg_lanbounds call:
+ * Lanczos steps, filter-operator applications, candidate count, seconds. */
+int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *ncand,
+                    double *seconds);
+/* Convergence controls of nmg_cheblannr (defaults: ntest 30, cycle 20,
+ * rtol 1e-9 on the filtered Ritz residual). */
+int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
+
+/* --- device-vector access (for tests and benchmarks) ------------------- */
+/* One application of the filter operator of the handle:
+ * y = p((A B^-1 - cc)/dd) z (z-space of ChebLanNr) with pol; host vectors. */
+int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double *z, double *y);
+/* Operators of the handle on host vectors: which = 0 A, 1 B, 2 B-solve. */
+int nmg_apply(nmg_handle h, int which, const double *x, double *y);
+/* Kernel timing probe: runs `reps` products of mat on device-resident
+ * vectors and returns the mean kernel time in seconds (HIP events on the
+ * library's own stream) and the algorithmic bytes of one product. */
+int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
+/* Same for one fused Chebyshev B-solve step (one product + vector update). */
+int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NMGPU_H */

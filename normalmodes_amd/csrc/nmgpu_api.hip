@@ -1,0 +1,1178 @@
+// C-ABI of the nmgpu library and the device-resident driver of the
+// filtered-Lanczos path (LanBounds, FindPol, ChebLanNr, results, residuals).
+//
+// The call sequence mirrors what NormalModes does through pEVSL:
+//   setupmatvec  (src/mod_matvec.f90:24-250):   nmg_parcsr_create, nmg_csr_diagscale,
+//                nmg_start/nmg_set_amv/nmg_lanbounds on B~, nmg_setup_chebiter
+//   pnm_apply_pevsl (src/mod_pevsl.f90:54-162): nmg_start, nmg_setprobsizes,
+//                nmg_set_bmv, nmg_set_bsol_chebiter, nmg_set_amv_dad, nmg_set_geneig,
+//                nmg_lanbounds, nmg_findpol, nmg_cheblannr, nmg_get_nev,
+//                nmg_copy_result, nmg_residuals, nmg_freepol, nmg_finish
+// Everything between upload and copy-out stays in HBM; the only host work per
+// Lanczos step is the periodic tridiagonal eigenvalue check.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <unordered_set>
+#include <vector>
+
+#include "nmgpu_internal.h"
+
+namespace nmg {
+
+static thread_local std::string g_err;
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_err = msg;
+}
+
+// ---------------------------------------------------------------------------
+// handle registry
+enum Kind { K_SOLVER = 1, K_MAT = 2, K_CHEB = 3, K_POL = 4 };
+static std::mutex g_mu;
+static std::unordered_set<const void*> g_live[5];
+
+template <class T>
+static nmg_handle reg(T* p, int kind) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_live[kind].insert(p);
+  return (nmg_handle)(intptr_t)p;
+}
+template <class T>
+static T* get(nmg_handle h, int kind) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const void* p = (const void*)(intptr_t)h;
+  if (!p || !g_live[kind].count(p)) throw Error{NMG_EINVAL, "invalid handle"};
+  return (T*)p;
+}
+static void unreg(const void* p, int kind) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_live[kind].erase(p);
+}
+
+template <class F>
+static int guarded(F&& f) {
+  // library calls (rocBLAS/rocSOLVER set-up) can leave a sticky HIP error
+  // behind; start every entry from a clean state so launch checks are exact
+  (void)hipGetLastError();
+  try {
+    f();
+    return NMG_OK;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(NMG_ENOMEM, "host allocation failed");
+    return NMG_ENOMEM;
+  } catch (...) {
+    set_error(NMG_EINVAL, "unknown failure");
+    return NMG_EINVAL;
+  }
+}
+
+template <class T>
+static T* dmalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error{NMG_ENOMEM, "hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed"};
+  }
+  return (T*)p;
+}
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static hipStream_t default_stream() {
+  static hipStream_t s = nullptr;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+  });
+  NMG_CHECK(s != nullptr, NMG_EHIP, "cannot create HIP stream");
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// matrices
+void DevMat::release() {
+  dfree(rowptr);
+  dfree(col);
+  dfree(val);
+  dfree(val32);
+  dfree(browptr);
+  dfree(bcol);
+  dfree(bval);
+  dfree(bval32);
+}
+DevMat::~DevMat() { release(); }
+
+double DevMat::algorithmic_bytes(bool fp32) const {
+  const double sv = fp32 ? 4.0 : 8.0;
+  if (layout == L_CSR)
+    return (double)nnz * (sv + 4.0) + (double)(nrow + 1) * 8.0 + (double)ncol * 8.0 + (double)nrow * 8.0;
+  const double vals = layout == L_BSR3 ? 9.0 : 1.0;
+  return (double)nbnz * (vals * sv + 4.0) + (double)(nb + 1) * 8.0 + (double)ncol * 8.0 +
+         (double)nrow * 8.0;
+}
+
+// Layout detection on the host copy.
+static int detect_layout(int64_t nrow, int64_t ncol, const int64_t* rp, const int32_t* ci,
+                         const double* va) {
+  if (nrow % 3 != 0 || ncol % 3 != 0 || nrow == 0) return L_CSR;
+  const int64_t nb = nrow / 3;
+  bool bsr = true, mass = true;
+  for (int64_t v = 0; v < nb && (bsr || mass); ++v) {
+    const int64_t r0 = rp[3 * v], l0 = rp[3 * v + 1] - r0;
+    for (int p = 1; p < 3; ++p)
+      if (rp[3 * v + p + 1] - rp[3 * v + p] != l0) bsr = mass = false;
+    if (!bsr && !mass) break;
+    if (bsr) {
+      if (l0 % 3 != 0) bsr = false;
+      for (int p = 0; p < 3 && bsr; ++p) {
+        const int64_t rs = rp[3 * v + p];
+        for (int64_t j = 0; j < l0 && bsr; j += 3) {
+          const int32_t c0 = ci[rs + j];
+          if (c0 % 3 != 0 || ci[rs + j + 1] != c0 + 1 || ci[rs + j + 2] != c0 + 2) bsr = false;
+          if (ci[rs + j] != ci[r0 + j]) bsr = false;
+          if (j > 0 && ci[rs + j] <= ci[rs + j - 1]) bsr = false;
+        }
+      }
+    }
+    if (mass) {
+      for (int p = 0; p < 3 && mass; ++p) {
+        const int64_t rs = rp[3 * v + p];
+        for (int64_t j = 0; j < l0 && mass; ++j) {
+          if ((ci[rs + j] % 3) != p) mass = false;
+          else if (ci[rs + j] / 3 != ci[r0 + j] / 3) mass = false;
+          else if (va[rs + j] != va[r0 + j]) mass = false;
+        }
+      }
+    }
+  }
+  if (bsr) return L_BSR3;
+  if (mass) return L_MASS3;
+  return L_CSR;
+}
+
+DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rp, const int32_t* ci,
+                    const double* va, bool allow_blocked) {
+  NMG_CHECK(nrow > 0 && ncol > 0, NMG_EINVAL, "empty matrix");
+  NMG_CHECK(rp[0] == 0, NMG_EINVAL, "row offsets must start at 0");
+  for (int64_t i = 0; i < nrow; ++i)
+    NMG_CHECK(rp[i + 1] >= rp[i], NMG_EINVAL, "row offsets must be non-decreasing");
+  const int64_t nnz = rp[nrow];
+  for (int64_t k = 0; k < nnz; ++k)
+    NMG_CHECK(ci[k] >= 0 && ci[k] < ncol, NMG_EINVAL, "column index out of range");
+  std::unique_ptr<DevMat> M(new DevMat());
+  M->nrow = nrow;
+  M->ncol = ncol;
+  M->nnz = nnz;
+  M->layout = allow_blocked ? detect_layout(nrow, ncol, rp, ci, va) : L_CSR;
+  if (M->layout == L_CSR) {
+    M->rowptr = dmalloc<int64_t>(nrow + 1);
+    M->col = dmalloc<int32_t>(nnz);
+    M->val = dmalloc<double>(nnz);
+    NMG_HIP(hipMemcpy(M->rowptr, rp, sizeof(int64_t) * (nrow + 1), hipMemcpyHostToDevice));
+    NMG_HIP(hipMemcpy(M->col, ci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    NMG_HIP(hipMemcpy(M->val, va, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    return M.release();
+  }
+  const int64_t nb = nrow / 3;
+  const bool bsr = M->layout == L_BSR3;
+  std::vector<int64_t> brp(nb + 1, 0);
+  for (int64_t v = 0; v < nb; ++v) {
+    const int64_t l0 = rp[3 * v + 1] - rp[3 * v];
+    brp[v + 1] = brp[v] + (bsr ? l0 / 3 : l0);
+  }
+  const int64_t nbnz = brp[nb];
+  const int planes = bsr ? 9 : 1;
+  std::vector<int32_t> bc(nbnz);
+  std::vector<double> bv((size_t)planes * nbnz);
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < nb; ++v) {
+    for (int64_t j = 0; j < brp[v + 1] - brp[v]; ++j) {
+      const int64_t k = brp[v] + j;
+      if (bsr) {
+        bc[k] = ci[rp[3 * v] + 3 * j] / 3;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) bv[(size_t)(3 * r + c) * nbnz + k] = va[rp[3 * v + r] + 3 * j + c];
+      } else {
+        bc[k] = ci[rp[3 * v] + j] / 3;
+        bv[k] = va[rp[3 * v] + j];
+      }
+    }
+  }
+  M->nb = nb;
+  M->nbnz = nbnz;
+  M->browptr = dmalloc<int64_t>(nb + 1);
+  M->bcol = dmalloc<int32_t>(nbnz);
+  M->bval = dmalloc<double>((size_t)planes * nbnz);
+  NMG_HIP(hipMemcpy(M->browptr, brp.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(M->bcol, bc.data(), sizeof(int32_t) * nbnz, hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(M->bval, bv.data(), sizeof(double) * planes * nbnz, hipMemcpyHostToDevice));
+  return M.release();
+}
+
+DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
+  std::unique_ptr<DevMat> M(new DevMat());
+  M->nrow = S.nrow;
+  M->ncol = S.ncol;
+  M->nnz = S.nnz;
+  M->layout = S.layout;
+  M->nb = S.nb;
+  M->nbnz = S.nbnz;
+  auto cp = [&](auto* src, auto*& dst, size_t count) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
+    if (!src) return;
+    dst = dmalloc<T>(count);
+    NMG_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToDevice, s));
+  };
+  if (S.layout == L_CSR) {
+    cp(S.rowptr, M->rowptr, S.nrow + 1);
+    cp(S.col, M->col, S.nnz);
+    cp(S.val, M->val, S.nnz);
+  } else {
+    const size_t planes = S.layout == L_BSR3 ? 9 : 1;
+    cp(S.browptr, M->browptr, S.nb + 1);
+    cp(S.bcol, M->bcol, S.nbnz);
+    cp(S.bval, M->bval, planes * S.nbnz);
+  }
+  return M.release();
+}
+
+// ---------------------------------------------------------------------------
+struct Cheb {
+  DevMat* mat = nullptr;
+  double lmin = 0, lmax = 0;
+  int deg = 0;
+  int64_t ncalls = 0, nmatvec = 0;
+  double seconds = 0;
+};
+
+struct DVec {
+  double* p = nullptr;
+  int64_t n = 0;
+  void ensure(int64_t m) {
+    if (m <= n) return;
+    dfree(p);
+    p = dmalloc<double>(m);
+    n = m;
+  }
+  ~DVec() { dfree(p); }
+};
+
+struct Solver {
+  int device = 0;
+  hipStream_t s = nullptr;
+  rocblas_handle blas = nullptr;
+  int64_t n = 0;
+  DevMat* amat = nullptr;            // plain operator (nmg_set_amv)
+  std::unique_ptr<DevMat> adad;      // owned D A D (nmg_set_amv_dad)
+  DevMat* bmat = nullptr;
+  Cheb* bsol = nullptr;
+  bool geneig = false;
+  int precision = NMG_FP64;
+  uint64_t seed = 7;
+  int ntest = 30, cycle = 20;
+  double rtol = 1e-9;
+  // work vectors
+  DVec w[8];
+  DVec red;  // reductions
+  DVec ab;   // alpha / beta device scalars
+  double* V = nullptr;
+  double* Z = nullptr;
+  int cap = 0;
+  // results
+  int nev = 0;
+  std::vector<double> lam;
+  double* X = nullptr;
+  // stats
+  int lan_steps = 0;
+  int64_t filter_applies = 0;
+  int ncand = 0;
+  double seconds = 0;
+
+  ~Solver() {
+    free_basis();
+    dfree(X);
+    if (blas) rocblas_destroy_handle(blas);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void free_basis() {
+    dfree(V);
+    if (Z && Z != V) dfree(Z);
+    Z = nullptr;
+    cap = 0;
+  }
+  const DevMat& A() const {
+    if (adad) return *adad;
+    NMG_CHECK(amat != nullptr, NMG_ESTATE, "no A operator set");
+    return *amat;
+  }
+  bool fp32() const { return precision == NMG_FP32_FILTER; }
+};
+
+// grow the Lanczos basis to hold `need` columns (copying the first `keep`)
+static void ensure_basis(Solver& S, int need, int keep, int maxcap) {
+  if (need <= S.cap) return;
+  int nc = std::max(need, std::min(maxcap, std::max(64, S.cap * 2)));
+  double* V = dmalloc<double>((size_t)S.n * nc);
+  double* Z = S.geneig ? dmalloc<double>((size_t)S.n * nc) : V;
+  if (keep > 0) {
+    NMG_HIP(hipMemcpyAsync(V, S.V, sizeof(double) * S.n * keep, hipMemcpyDeviceToDevice, S.s));
+    if (S.geneig)
+      NMG_HIP(hipMemcpyAsync(Z, S.Z, sizeof(double) * S.n * keep, hipMemcpyDeviceToDevice, S.s));
+    NMG_HIP(hipStreamSynchronize(S.s));
+  }
+  S.free_basis();
+  S.V = V;
+  S.Z = Z;
+  S.cap = nc;
+}
+
+// x ~ B~^{-1} b by the Chebyshev semi-iteration (x_0 = 0, deg products):
+//   x_1 = b/th,  x_{k+1} = x_{k-1} + w_{k+1}(x_k - x_{k-1} + (b - B x_k)/th),
+//   w_1 = 2, w_{k+1} = 1/(1 - w_k/(4 s^2)), th = (lmax+lmin)/2, s = th/de.
+// The result lands in `out`; `tmp` is a second buffer.
+static void cheb_solve(const Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
+                       bool fp32) {
+  const int64_t n = C.mat->nrow;
+  const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
+  double* xk = (C.deg % 2 == 0) ? out : tmp;
+  double* xm = (C.deg % 2 == 0) ? tmp : out;
+  k_scale_copy(n, 1.0 / th, b, xk, s);
+  k_fill(n, 0.0, xm, s);
+  double w = 2.0;
+  for (int k = 1; k <= C.deg; ++k) {
+    w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    std::swap(xk, xm);
+  }
+}
+
+static void apply_B(Solver& S, const double* x, double* y) {
+  if (S.bmat) k_spmv(*S.bmat, x, y, S.s);
+  else k_copy(S.n, x, y, S.s);
+}
+
+static void bsolve(Solver& S, const double* b, double* out, double* tmp) {
+  if (!S.geneig) {
+    if (out != b) k_copy(S.n, b, out, S.s);
+    return;
+  }
+  NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
+  cheb_solve(*S.bsol, b, out, tmp, S.s, S.fp32());
+  S.bsol->ncalls++;
+  S.bsol->nmatvec += S.bsol->deg;
+}
+
+// y = p((A B^-1 - cc)/dd) z  (z-space; p((A - cc)/dd) z for a standard problem)
+static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double* y) {
+  const int64_t n = S.n;
+  for (int i = 0; i < 4; ++i) S.w[i].ensure(n);
+  double* vk = S.w[0].p;
+  double* vkm = S.w[1].p;
+  double* b1 = S.w[2].p;
+  double* b2 = S.w[3].p;
+  k_copy(n, z, vk, S.s);
+  k_scale_copy(n, P.mu[0], vk, y, S.s);
+  const DevMat& A = S.A();
+  for (int k = 1; k <= P.deg; ++k) {
+    const double t = (k == 1 ? 1.0 : 2.0) / P.dd;
+    const double* in = vk;
+    if (S.geneig) {
+      bsolve(S, vk, b1, b2);
+      in = b1;
+    }
+    k_filter_step(A, in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, S.fp32());
+    std::swap(vk, vkm);
+  }
+  S.filter_applies++;
+}
+
+static void setup_work(Solver& S) {
+  for (int i = 0; i < 8; ++i) S.w[i].ensure(S.n);
+  S.red.ensure((int64_t)std::max(dot_work_size(), (size_t)16));
+}
+
+// one Lanczos step: Z(:,k+1) <- op(.) then three-term, CGS2, B-solve, normalise
+// op: 0 = A v_k (bounds), 1 = filter of z_k
+static void lanczos_step(Solver& S, int k, int op, const PolyFilter* P, double* hwork) {
+  const int64_t n = S.n;
+  double* v = S.V + (size_t)k * n;
+  double* z = S.Z + (size_t)k * n;
+  double* vn = S.V + (size_t)(k + 1) * n;
+  double* zn = S.Z + (size_t)(k + 1) * n;
+  double* alpha = S.ab.p + k;
+  double* beta = S.ab.p + S.ab.n / 2 + k;
+  if (op == 0) k_spmv(S.A(), v, zn, S.s);
+  else filter_apply(S, *P, z, zn);
+  if (k > 0) k_axpy_dev(n, beta - 1, S.Z + (size_t)(k - 1) * n, zn, S.s);
+  k_dot(n, v, zn, alpha, false, S.red.p, S.s);
+  k_axpy_dev(n, alpha, z, zn, S.s);
+  // classical Gram-Schmidt, twice, against (Z, V)(:, 0:k)
+  double* h = hwork;
+  double* gw = hwork + (k + 2);
+  for (int pass = 0; pass < 2; ++pass) {
+    k_gemv_t(n, k + 1, S.V, n, zn, h, gw, S.s);
+    k_gemv_n_sub(n, k + 1, S.Z, n, h, zn, S.s);
+  }
+  if (S.geneig) {
+    bsolve(S, zn, vn, S.w[4].p);
+    k_dot(n, vn, zn, beta, true, S.red.p, S.s);
+    k_invscale_dev(n, beta, vn, zn, S.s);
+  } else {
+    k_dot(n, zn, zn, beta, true, S.red.p, S.s);
+    k_invscale_dev(n, beta, zn, nullptr, S.s);
+  }
+}
+
+static void lanczos_start(Solver& S) {
+  const int64_t n = S.n;
+  std::vector<double> v0(n);
+  start_vector(n, S.seed, v0.data());
+  NMG_HIP(hipMemcpyAsync(S.V, v0.data(), sizeof(double) * n, hipMemcpyHostToDevice, S.s));
+  double* nrm = S.ab.p + S.ab.n - 1;
+  if (S.geneig) {
+    apply_B(S, S.V, S.Z);
+    k_dot(n, S.V, S.Z, nrm, true, S.red.p, S.s);
+    k_invscale_dev(n, nrm, S.V, S.Z, S.s);
+  } else {
+    k_dot(n, S.V, S.V, nrm, true, S.red.p, S.s);
+    k_invscale_dev(n, nrm, S.V, nullptr, S.s);
+  }
+  NMG_HIP(hipStreamSynchronize(S.s));
+}
+
+static void fetch_tridiag(Solver& S, int kdim, std::vector<double>& al, std::vector<double>& be) {
+  al.resize(kdim);
+  be.resize(kdim);
+  NMG_HIP(hipMemcpyAsync(al.data(), S.ab.p, sizeof(double) * kdim, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipMemcpyAsync(be.data(), S.ab.p + S.ab.n / 2, sizeof(double) * kdim, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  for (int i = 0; i < kdim; ++i)
+    NMG_CHECK(std::isfinite(al[i]) && std::isfinite(be[i]), NMG_ENUM,
+              "Lanczos breakdown: non-finite recurrence coefficient");
+}
+
+// rocBLAS / rocSOLVER calls: check the status and clear any HIP error state
+// the library left behind, so later kernel-launch checks see only their own
+#define NMG_LIB(call, what)                                                            \
+  do {                                                                                 \
+    rocblas_status st_ = (call);                                                       \
+    (void)hipGetLastError();                                                           \
+    NMG_CHECK(st_ == rocblas_status_success, NMG_EHIP,                                 \
+              std::string(what) + " failed: " + rocblas_status_to_string(st_));        \
+  } while (0)
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+static void do_lanbounds(Solver& S, int mlan, int maxit, double tol, double* lmin, double* lmax) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  const double t0 = now();
+  int m = std::min<int64_t>(std::min(mlan, maxit), S.n);
+  NMG_CHECK(m >= 2, NMG_EINVAL, "mlan/maxit too small");
+  setup_work(S);
+  S.ab.ensure(2 * (int64_t)(m + 2));
+  S.free_basis();
+  ensure_basis(S, std::min(m + 1, 64), 0, m + 1);
+  std::vector<double> hbuf;
+  DVec hw;
+  lanczos_start(S);
+  std::vector<double> al, be, w, sv;
+  int k = 0;
+  bool done = false;
+  for (k = 0; k < m && !done; ++k) {
+    ensure_basis(S, k + 2, k + 1, m + 1);
+    hw.ensure((int64_t)gemv_work_size(S.n, k + 2) + k + 4);
+    lanczos_step(S, k, 0, nullptr, hw.p);
+    const int kd = k + 1;
+    if (kd % 10 == 0 || kd == m) {
+      fetch_tridiag(S, kd, al, be);
+      if (tridiag_eigvals(kd, al.data(), be.data(), w) != NMG_OK)
+        throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
+      tridiag_eigvec(kd, al.data(), be.data(), w[0], sv);
+      const double rmin = std::fabs(be[kd - 1] * sv[kd - 1]);
+      tridiag_eigvec(kd, al.data(), be.data(), w[kd - 1], sv);
+      const double rmax = std::fabs(be[kd - 1] * sv[kd - 1]);
+      *lmin = w[0] - rmin;
+      *lmax = w[kd - 1] + rmax;
+      const double spread = std::fabs(w[kd - 1]) + std::fabs(w[0]);
+      if (be[kd - 1] == 0.0 || (rmin <= tol * spread && rmax <= tol * spread)) done = true;
+    }
+  }
+  S.lan_steps = k;
+  S.free_basis();
+  S.seconds = now() - t0;
+}
+
+static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, const PolyFilter& P) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  NMG_CHECK(xintv[1] > xintv[0], NMG_EINVAL, "empty interval");
+  const double t0 = now();
+  const int64_t n = S.n;
+  maxit = (int)std::min<int64_t>(maxit, n);
+  setup_work(S);
+  S.ab.ensure(2 * (int64_t)(maxit + 2));
+  S.free_basis();
+  ensure_basis(S, std::min(maxit + 1, 64), 0, maxit + 1);
+  S.filter_applies = 0;
+  DVec hw;
+  lanczos_start(S);
+  std::vector<double> al, be, w, sv;
+  double tr0 = 0.0;
+  int kdim = 0;
+  for (int k = 0; k < maxit; ++k) {
+    ensure_basis(S, k + 2, k + 1, maxit + 1);
+    hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
+    lanczos_step(S, k, 1, &P, hw.p);
+    kdim = k + 1;
+    if (kdim < S.ntest || ((kdim - S.ntest) % S.cycle) != 0) continue;
+    fetch_tridiag(S, kdim, al, be);
+    if (be[kdim - 1] == 0.0) break;
+    if (tridiag_eigvals(kdim, al.data(), be.data(), w) != NMG_OK)
+      throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
+    double tr1 = 0.0;
+    int cnt = 0;
+    for (int i = 0; i < kdim; ++i)
+      if (w[i] >= P.bar) {
+        tr1 += w[i];
+        cnt++;
+      }
+    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
+    tr0 = tr1;
+    if (!conv) continue;
+    double rmax = 0.0;
+    for (int i = 0; i < kdim; ++i)
+      if (w[i] >= P.bar) {
+        tridiag_eigvec(kdim, al.data(), be.data(), w[i], sv);
+        rmax = std::max(rmax, std::fabs(be[kdim - 1] * sv[kdim - 1]));
+      }
+    if (rmax <= S.rtol) break;
+  }
+  S.lan_steps = kdim;
+  fetch_tridiag(S, kdim, al, be);
+
+  // ---- extraction: eigenvectors of T on the device (rocSOLVER divide & conquer)
+  DVec dD, dE, dC;
+  dD.ensure(kdim);
+  dE.ensure(kdim);
+  dC.ensure((int64_t)kdim * kdim);
+  NMG_HIP(hipMemcpyAsync(dD.p, al.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
+  NMG_HIP(hipMemcpyAsync(dE.p, be.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
+  rocblas_int* dinfo = dmalloc<rocblas_int>(1);
+  std::unique_ptr<rocblas_int, void (*)(rocblas_int*)> info_guard(dinfo, [](rocblas_int* p) { (void)hipFree(p); });
+  NMG_LIB(rocsolver_dstedc(S.blas, rocblas_evect_tridiagonal, kdim, dD.p, dE.p, dC.p, kdim, dinfo), "rocsolver_dstedc failed");
+  std::vector<double> theta(kdim);
+  rocblas_int info = 0;
+  NMG_HIP(hipMemcpyAsync(theta.data(), dD.p, sizeof(double) * kdim, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  NMG_CHECK(info == 0, NMG_ENUM, "tridiagonal eigensolver failed");
+  std::vector<int> sel;
+  for (int i = 0; i < kdim; ++i)
+    if (theta[i] >= P.bar) sel.push_back(i);
+  const int nc = (int)sel.size();
+  S.ncand = nc;
+  S.nev = 0;
+  S.lam.clear();
+  dfree(S.X);
+  if (nc > 0) {
+    DVec Ssel, Y, AY, BY, GA, GB, mu, ee;
+    Ssel.ensure((int64_t)kdim * nc);
+    for (int j = 0; j < nc; ++j)
+      NMG_HIP(hipMemcpyAsync(Ssel.p + (size_t)j * kdim, dC.p + (size_t)sel[j] * kdim,
+                             sizeof(double) * kdim, hipMemcpyDeviceToDevice, S.s));
+    Y.ensure(n * nc);
+    const double one = 1.0, zero = 0.0;
+    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, nc,
+                            kdim, &one, S.V, (rocblas_int)n, Ssel.p, kdim, &zero, Y.p,
+                            (rocblas_int)n), "rocblas_dgemm (Ritz vectors) failed");
+    // Rayleigh-Ritz of (A~, B~) on span(Y)
+    AY.ensure(n * nc);
+    BY.ensure(n * nc);
+    for (int j = 0; j < nc; ++j) {
+      k_spmv(S.A(), Y.p + (size_t)j * n, AY.p + (size_t)j * n, S.s);
+      apply_B(S, Y.p + (size_t)j * n, BY.p + (size_t)j * n);
+    }
+    GA.ensure((int64_t)nc * nc);
+    GB.ensure((int64_t)nc * nc);
+    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, nc, nc,
+                            (rocblas_int)n, &one, Y.p, (rocblas_int)n, AY.p, (rocblas_int)n, &zero, GA.p,
+                            nc), "rocblas_dgemm (Y^T A Y) failed");
+    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, nc, nc,
+                            (rocblas_int)n, &one, Y.p, (rocblas_int)n, BY.p, (rocblas_int)n, &zero, GB.p,
+                            nc), "rocblas_dgemm (Y^T B Y) failed");
+    // symmetrise (the products agree to rounding)
+    {
+      std::vector<double> ga((size_t)nc * nc), gb((size_t)nc * nc);
+      NMG_HIP(hipMemcpyAsync(ga.data(), GA.p, sizeof(double) * nc * nc, hipMemcpyDeviceToHost, S.s));
+      NMG_HIP(hipMemcpyAsync(gb.data(), GB.p, sizeof(double) * nc * nc, hipMemcpyDeviceToHost, S.s));
+      NMG_HIP(hipStreamSynchronize(S.s));
+      for (int i = 0; i < nc; ++i)
+        for (int j = i + 1; j < nc; ++j) {
+          double a = 0.5 * (ga[(size_t)i * nc + j] + ga[(size_t)j * nc + i]);
+          double b = 0.5 * (gb[(size_t)i * nc + j] + gb[(size_t)j * nc + i]);
+          ga[(size_t)i * nc + j] = ga[(size_t)j * nc + i] = a;
+          gb[(size_t)i * nc + j] = gb[(size_t)j * nc + i] = b;
+        }
+      NMG_HIP(hipMemcpyAsync(GA.p, ga.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice, S.s));
+      NMG_HIP(hipMemcpyAsync(GB.p, gb.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice, S.s));
+    }
+    mu.ensure(nc);
+    ee.ensure(nc);
+    NMG_LIB(rocsolver_dsygvd(S.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, nc,
+                               GA.p, nc, GB.p, nc, mu.p, ee.p, dinfo), "rocsolver_dsygvd failed");
+    std::vector<double> rmu(nc);
+    NMG_HIP(hipMemcpyAsync(rmu.data(), mu.p, sizeof(double) * nc, hipMemcpyDeviceToHost, S.s));
+    NMG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, S.s));
+    NMG_HIP(hipStreamSynchronize(S.s));
+    NMG_CHECK(info == 0, NMG_ENUM, "Rayleigh-Ritz eigensolver failed (B~ Gram matrix not SPD)");
+    std::vector<int> keep;
+    for (int j = 0; j < nc; ++j)
+      if (rmu[j] >= xintv[0] && rmu[j] <= xintv[1]) keep.push_back(j);
+    const int nev = (int)keep.size();
+    if (nev > 0) {
+      DVec Wsel;
+      Wsel.ensure((int64_t)nc * nev);
+      for (int j = 0; j < nev; ++j)
+        NMG_HIP(hipMemcpyAsync(Wsel.p + (size_t)j * nc, GA.p + (size_t)keep[j] * nc, sizeof(double) * nc,
+                               hipMemcpyDeviceToDevice, S.s));
+      S.X = dmalloc<double>((size_t)n * nev);
+      NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, nev,
+                              nc, &one, Y.p, (rocblas_int)n, Wsel.p, nc, &zero, S.X,
+                              (rocblas_int)n), "rocblas_dgemm (eigenvectors) failed");
+      for (int j = 0; j < nev; ++j) S.lam.push_back(rmu[keep[j]]);
+      S.nev = nev;
+    }
+    NMG_HIP(hipStreamSynchronize(S.s));
+  }
+  S.free_basis();
+  S.seconds = now() - t0;
+}
+
+static void do_residuals(Solver& S, double* res_ref, double* res2) {
+  const int64_t n = S.n;
+  setup_work(S);
+  DVec out;
+  out.ensure(2 * std::max(S.nev, 1));
+  for (int j = 0; j < S.nev; ++j) {
+    const double* x = S.X + (size_t)j * n;
+    k_spmv(S.A(), x, S.w[5].p, S.s);
+    apply_B(S, x, S.w[6].p);
+    k_resid(n, S.w[5].p, S.w[6].p, x, S.lam[j], out.p + 2 * j, S.red.p, S.s);
+  }
+  std::vector<double> h(2 * std::max(S.nev, 1));
+  NMG_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 2 * std::max(S.nev, 1), hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  for (int j = 0; j < S.nev; ++j) {
+    const double al = std::fabs(S.lam[j]) > 0 ? std::fabs(S.lam[j]) : 1.0;
+    if (res_ref) res_ref[j] = std::sqrt(h[2 * j] / (double)n) / al;
+    if (res2) res2[j] = std::sqrt(h[2 * j] / h[2 * j + 1]) / al;
+  }
+}
+
+}  // namespace nmg
+
+using namespace nmg;
+
+// ===========================================================================
+extern "C" {
+
+const char* nmg_last_error(void) { return g_err.c_str(); }
+
+int nmg_version(int* major, int* minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return NMG_OK;
+}
+
+int nmg_device_info(char* name, int name_len, int* cu_count, double* hbm_gbytes) {
+  return guarded([&] {
+    int dev = 0;
+    NMG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    NMG_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) {
+      std::snprintf(name, name_len, "%s", p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_gbytes) *hbm_gbytes = (double)p.totalGlobalMem / 1e9;
+  });
+}
+
+int nmg_start(int comm, nmg_handle* h) {
+  (void)comm;
+  return guarded([&] {
+    NMG_CHECK(h != nullptr, NMG_EINVAL, "null handle pointer");
+    std::unique_ptr<Solver> S(new Solver());
+    NMG_HIP(hipGetDevice(&S->device));
+    NMG_HIP(hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking));
+    NMG_LIB(rocblas_create_handle(&S->blas), "rocblas handle");
+    NMG_LIB(rocblas_set_stream(S->blas, S->s), "rocblas stream");
+    (void)hipGetLastError();
+    *h = reg(S.release(), K_SOLVER);
+  });
+}
+
+int nmg_finish(nmg_handle h) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    (void)hipStreamSynchronize(S->s);
+    unreg(S, K_SOLVER);
+    delete S;
+  });
+}
+
+int nmg_setprobsizes(nmg_handle h, int64_t n_glob, int64_t n_loc, int64_t nfirst) {
+  (void)nfirst;
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(n_glob > 0 && n_loc == n_glob, NMG_EINVAL,
+              "one device holds all rows: n_loc must equal n_glob and be positive");
+    S->n = n_loc;
+  });
+}
+
+int nmg_csr_create(int64_t nrow, int64_t ncol, const int64_t* rowptr, const int32_t* col,
+                   const double* val, nmg_handle* mat) {
+  return guarded([&] {
+    NMG_CHECK(rowptr && col && val && mat, NMG_EINVAL, "null argument");
+    DevMat* M = make_devmat(nrow, ncol, rowptr, col, val, true);
+    *mat = reg(M, K_MAT);
+  });
+}
+
+int nmg_parcsr_create(int nrow_g, int ncol_g, const int* row_starts, const int* col_starts,
+                      const int* ia, const int* ja, const double* a, int comm, nmg_handle* mat) {
+  (void)comm;
+  return guarded([&] {
+    NMG_CHECK(ia && ja && a && mat, NMG_EINVAL, "null argument");
+    if (row_starts)
+      NMG_CHECK(row_starts[0] == 0 && row_starts[1] == nrow_g, NMG_EINVAL,
+                "row_starts must describe one owner of all rows");
+    if (col_starts)
+      NMG_CHECK(col_starts[0] == 0 && col_starts[1] == ncol_g, NMG_EINVAL,
+                "col_starts must describe one owner of all columns");
+    std::vector<int64_t> rp(nrow_g + 1);
+    for (int i = 0; i <= nrow_g; ++i) rp[i] = ia[i];
+    DevMat* M = make_devmat(nrow_g, ncol_g, rp.data(), (const int32_t*)ja, a, true);
+    *mat = reg(M, K_MAT);
+  });
+}
+
+int nmg_csr_free(nmg_handle mat) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    unreg(M, K_MAT);
+    delete M;
+  });
+}
+
+int nmg_csr_info(nmg_handle mat, int64_t* nrow, int64_t* ncol, int64_t* nnz, int* layout) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    if (nrow) *nrow = M->nrow;
+    if (ncol) *ncol = M->ncol;
+    if (nnz) *nnz = M->nnz;
+    if (layout) *layout = M->layout;
+  });
+}
+
+int nmg_parcsr_matvec(const double* x, double* y, nmg_handle mat) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(x && y, NMG_EINVAL, "null vector");
+    hipStream_t s = default_stream();
+    DVec dx, dy;
+    dx.ensure(M->ncol);
+    dy.ensure(M->nrow);
+    NMG_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * M->ncol, hipMemcpyHostToDevice, s));
+    k_spmv(*M, dx.p, dy.p, s);
+    NMG_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * M->nrow, hipMemcpyDeviceToHost, s));
+    NMG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nmg_csr_diagscale(nmg_handle mat, double sign, double* d) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(M->nrow == M->ncol, NMG_EINVAL, "diagonal scaling needs a square matrix");
+    NMG_CHECK(sign == 1.0 || sign == -1.0, NMG_EINVAL, "sign must be +1 or -1");
+    hipStream_t s = default_stream();
+    const int64_t n = M->nrow;
+    std::vector<double> hd(n);
+    DVec dd;
+    dd.ensure(n);
+    diag_devmat(*M, dd.p, s);
+    NMG_HIP(hipMemcpyAsync(hd.data(), dd.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    NMG_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) {
+      const double v = sign * hd[i];
+      NMG_CHECK(v > 0.0 && std::isfinite(v), NMG_ENUM,
+                "diagonal entry " + std::to_string(i) + " is not positive after the sign: cannot scale");
+      hd[i] = 1.0 / std::sqrt(v);
+    }
+    if (M->layout == L_MASS3)
+      for (int64_t v = 0; v < n / 3; ++v)
+        NMG_CHECK(hd[3 * v] == hd[3 * v + 1] && hd[3 * v] == hd[3 * v + 2], NMG_EINVAL,
+                  "mass layout with component-dependent diagonal");
+    NMG_HIP(hipMemcpyAsync(dd.p, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    scale_devmat(*M, dd.p, dd.p, sign, s);
+    NMG_HIP(hipStreamSynchronize(s));
+    if (d) std::memcpy(d, hd.data(), sizeof(double) * n);
+  });
+}
+
+int nmg_set_amv_dad(nmg_handle h, nmg_handle matA, const double* d) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    DevMat* M = get<DevMat>(matA, K_MAT);
+    NMG_CHECK(S->n > 0 && M->nrow == S->n && M->ncol == S->n, NMG_EINVAL, "A size mismatch");
+    NMG_CHECK(d != nullptr, NMG_EINVAL, "null scaling vector");
+    for (int64_t i = 0; i < S->n; ++i)
+      NMG_CHECK(std::isfinite(d[i]) && d[i] > 0.0, NMG_EINVAL, "scaling vector must be positive");
+    std::unique_ptr<DevMat> C(clone_devmat(*M, S->s));
+    DVec dd;
+    dd.ensure(S->n);
+    NMG_HIP(hipMemcpyAsync(dd.p, d, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
+    scale_devmat(*C, dd.p, dd.p, 1.0, S->s);
+    NMG_HIP(hipStreamSynchronize(S->s));
+    S->adad = std::move(C);
+    S->amat = nullptr;
+  });
+}
+
+int nmg_set_amv(nmg_handle h, nmg_handle mat) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(S->n > 0 && M->nrow == S->n && M->ncol == S->n, NMG_EINVAL, "operator size mismatch");
+    S->amat = M;
+    S->adad.reset();
+  });
+}
+
+int nmg_set_bmv(nmg_handle h, nmg_handle matB) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    DevMat* M = get<DevMat>(matB, K_MAT);
+    NMG_CHECK(S->n > 0 && M->nrow == S->n && M->ncol == S->n, NMG_EINVAL, "B size mismatch");
+    S->bmat = M;
+  });
+}
+
+int nmg_set_bsol_chebiter(nmg_handle h, int type, nmg_handle cheb) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(type == 1 || type == 2, NMG_EINVAL, "chebiter type must be 1 or 2");
+    NMG_CHECK(S->n > 0 && C->mat->nrow == S->n, NMG_EINVAL, "B-solve size mismatch");
+    S->bsol = C;
+  });
+}
+
+int nmg_set_geneig(nmg_handle h) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(S->bmat && S->bsol, NMG_ESTATE, "set B and its solve before the generalized flag");
+    S->geneig = true;
+  });
+}
+
+int nmg_set_precision(nmg_handle h, int mode) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER, NMG_EINVAL, "unknown precision");
+    S->precision = mode;
+    if (mode == NMG_FP32_FILTER) {
+      if (S->adad) make_fp32(*S->adad, S->s);
+      else if (S->amat) make_fp32(*S->amat, S->s);
+      if (S->bsol) make_fp32(*S->bsol->mat, S->s);
+      NMG_HIP(hipStreamSynchronize(S->s));
+    }
+  });
+}
+
+int nmg_set_seed(nmg_handle h, uint64_t seed) {
+  return guarded([&] { get<Solver>(h, K_SOLVER)->seed = seed; });
+}
+
+int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(ntest >= 2 && cycle >= 1 && rtol > 0.0, NMG_EINVAL, "bad Lanczos controls");
+    S->ntest = ntest;
+    S->cycle = cycle;
+    S->rtol = rtol;
+  });
+}
+
+int nmg_setup_chebiter(double lmin, double lmax, int deg, nmg_handle mat, nmg_handle* cheb) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(cheb != nullptr, NMG_EINVAL, "null handle pointer");
+    NMG_CHECK(lmin > 0.0 && lmax > lmin && std::isfinite(lmax), NMG_EINVAL,
+              "chebiter needs 0 < lmin < lmax (an SPD matrix)");
+    NMG_CHECK(deg >= 1 && deg <= 100000, NMG_EINVAL, "chebiter degree must be >= 1");
+    NMG_CHECK(M->nrow == M->ncol, NMG_EINVAL, "chebiter needs a square matrix");
+    Cheb* C = new Cheb();
+    C->mat = M;
+    C->lmin = lmin;
+    C->lmax = lmax;
+    C->deg = deg;
+    *cheb = reg(C, K_CHEB);
+  });
+}
+
+int nmg_chebiter(int type, const double* b, double* x, nmg_handle cheb) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(type == 1 || type == 2, NMG_EINVAL, "chebiter type must be 1 or 2");
+    NMG_CHECK(b && x, NMG_EINVAL, "null vector");
+    hipStream_t s = default_stream();
+    const int64_t n = C->mat->nrow;
+    DVec db, d1, d2;
+    db.ensure(n);
+    d1.ensure(n);
+    d2.ensure(n);
+    NMG_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const double t0 = now();
+    cheb_solve(*C, db.p, d1.p, d2.p, s, false);
+    NMG_HIP(hipMemcpyAsync(x, d1.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    NMG_HIP(hipStreamSynchronize(s));
+    C->seconds += now() - t0;
+    C->ncalls++;
+    C->nmatvec += C->deg;
+  });
+}
+
+int nmg_chebiter_stats(nmg_handle cheb, int64_t* ncalls, int64_t* nmatvec, double* seconds) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    if (ncalls) *ncalls = C->ncalls;
+    if (nmatvec) *nmatvec = C->nmatvec;
+    if (seconds) *seconds = C->seconds;
+  });
+}
+
+int nmg_chebiter_free(nmg_handle cheb) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    unreg(C, K_CHEB);
+    delete C;
+  });
+}
+
+int nmg_lanbounds(nmg_handle h, int mlan, int maxit, double tol, double* lmin, double* lmax) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(lmin && lmax, NMG_EINVAL, "null output");
+    NMG_CHECK(tol > 0.0, NMG_EINVAL, "tol must be positive");
+    do_lanbounds(*S, mlan, maxit, tol, lmin, lmax);
+  });
+}
+
+int nmg_findpol(const double* xintv, double thr_int, double thr_ext, nmg_handle* pol) {
+  return guarded([&] {
+    NMG_CHECK(xintv && pol, NMG_EINVAL, "null argument");
+    NMG_CHECK(thr_int > 0 && thr_int < 1 && thr_ext > 0 && thr_ext < 1, NMG_EINVAL,
+              "thresholds must lie in (0, 1)");
+    std::unique_ptr<PolyFilter> P(new PolyFilter());
+    int rc = find_pol(xintv, thr_int, thr_ext, 2, 10000, 2, *P);
+    NMG_CHECK(rc != NMG_EINVAL, NMG_EINVAL, "interval [a,b] does not intersect [lmin,lmax]");
+    NMG_CHECK(rc == NMG_OK, NMG_ENUM, "no filter of degree <= 10000 meets the threshold");
+    *pol = reg(P.release(), K_POL);
+  });
+}
+
+int nmg_pol_info(nmg_handle pol, int* deg, double* cc, double* dd, double* gam, double* bar, double* mu) {
+  return guarded([&] {
+    PolyFilter* P = get<PolyFilter>(pol, K_POL);
+    if (deg) *deg = P->deg;
+    if (cc) *cc = P->cc;
+    if (dd) *dd = P->dd;
+    if (gam) *gam = P->gam;
+    if (bar) *bar = P->bar;
+    if (mu) std::memcpy(mu, P->mu.data(), sizeof(double) * (P->deg + 1));
+  });
+}
+
+int nmg_freepol(nmg_handle pol) {
+  return guarded([&] {
+    PolyFilter* P = get<PolyFilter>(pol, K_POL);
+    unreg(P, K_POL);
+    delete P;
+  });
+}
+
+int nmg_cheblannr(nmg_handle h, const double* xintv, int maxit, double tol, nmg_handle pol) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    PolyFilter* P = get<PolyFilter>(pol, K_POL);
+    NMG_CHECK(xintv != nullptr, NMG_EINVAL, "null interval");
+    NMG_CHECK(maxit >= 2 && tol > 0.0, NMG_EINVAL, "bad maxit/tol");
+    (void)S->A();
+    do_cheblannr(*S, xintv, maxit, tol, *P);
+  });
+}
+
+int nmg_get_nev(nmg_handle h, int* nev) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(nev != nullptr, NMG_EINVAL, "null output");
+    *nev = S->nev;
+  });
+}
+
+int nmg_copy_result(nmg_handle h, double* val, double* vec, int64_t n_loc) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(n_loc == S->n, NMG_EINVAL, "n_loc does not match the problem size");
+    if (val) std::memcpy(val, S->lam.data(), sizeof(double) * S->nev);
+    if (vec && S->nev > 0) {
+      NMG_HIP(hipMemcpyAsync(vec, S->X, sizeof(double) * S->n * S->nev, hipMemcpyDeviceToHost, S->s));
+      NMG_HIP(hipStreamSynchronize(S->s));
+    }
+  });
+}
+
+int nmg_residuals(nmg_handle h, double* res_ref, double* res2) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    do_residuals(*S, res_ref, res2);
+  });
+}
+
+int nmg_solve_stats(nmg_handle h, int* lan_steps, int64_t* filter_applies, int* ncand, double* seconds) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    if (lan_steps) *lan_steps = S->lan_steps;
+    if (filter_applies) *filter_applies = S->filter_applies;
+    if (ncand) *ncand = S->ncand;
+    if (seconds) *seconds = S->seconds;
+  });
+}
+
+int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double* z, double* y) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    PolyFilter* P = get<PolyFilter>(pol, K_POL);
+    NMG_CHECK(z && y, NMG_EINVAL, "null vector");
+    setup_work(*S);
+    DVec dz, dy;
+    dz.ensure(S->n);
+    dy.ensure(S->n);
+    NMG_HIP(hipMemcpyAsync(dz.p, z, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
+    filter_apply(*S, *P, dz.p, dy.p);
+    NMG_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_apply(nmg_handle h, int which, const double* x, double* y) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(x && y, NMG_EINVAL, "null vector");
+    NMG_CHECK(which >= 0 && which <= 2, NMG_EINVAL, "which must be 0 (A), 1 (B) or 2 (B-solve)");
+    setup_work(*S);
+    DVec dx, dy;
+    dx.ensure(S->n);
+    dy.ensure(S->n);
+    NMG_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
+    if (which == 0) k_spmv(S->A(), dx.p, dy.p, S->s, S->fp32());
+    else if (which == 1) apply_B(*S, dx.p, dy.p);
+    else {
+      NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
+      bool g = S->geneig;
+      S->geneig = true;
+      bsolve(*S, dx.p, dy.p, S->w[7].p);
+      S->geneig = g;
+    }
+    NMG_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double* seconds, double* bytes) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
+    hipStream_t s = default_stream();
+    if (fp32 && !(M->val32 || M->bval32)) make_fp32(*M, s);
+    DVec dx, dy;
+    dx.ensure(M->ncol);
+    dy.ensure(M->nrow);
+    k_fill(M->ncol, 1.0, dx.p, s);
+    k_spmv(*M, dx.p, dy.p, s, fp32 != 0);  // warm-up
+    hipEvent_t e0, e1;
+    NMG_HIP(hipEventCreate(&e0));
+    NMG_HIP(hipEventCreate(&e1));
+    NMG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) k_spmv(*M, dx.p, dy.p, s, fp32 != 0);
+    NMG_HIP(hipEventRecord(e1, s));
+    NMG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = ms * 1e-3 / reps;
+    if (bytes) *bytes = M->algorithmic_bytes(fp32 != 0);
+  });
+}
+
+int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double* seconds, double* bytes) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
+    DevMat* M = C->mat;
+    hipStream_t s = default_stream();
+    if (fp32 && !(M->val32 || M->bval32)) make_fp32(*M, s);
+    const int64_t n = M->nrow;
+    DVec b, xk, xm;
+    b.ensure(n);
+    xk.ensure(n);
+    xm.ensure(n);
+    k_fill(n, 1.0, b.p, s);
+    k_fill(n, 0.5, xk.p, s);
+    k_fill(n, 0.0, xm.p, s);
+    const double th = 0.5 * (C->lmax + C->lmin);
+    k_cheb_step(*M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
+    hipEvent_t e0, e1;
+    NMG_HIP(hipEventCreate(&e0));
+    NMG_HIP(hipEventCreate(&e1));
+    NMG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+      k_cheb_step(*M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
+      std::swap(xk.p, xm.p);
+    }
+    NMG_HIP(hipEventRecord(e1, s));
+    NMG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = ms * 1e-3 / reps;
+    // product (matrix + x + y) with y replaced by the update: + b read + xm read
+    if (bytes) *bytes = M->algorithmic_bytes(fp32 != 0) + 2.0 * 8.0 * (double)n;
+  });
+}
+
+}  // extern "C"

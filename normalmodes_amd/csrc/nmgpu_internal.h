@@ -1,0 +1,119 @@
+// Internal declarations shared by the nmgpu HIP library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nmgpu.h"
+
+namespace nmg {
+
+// ---------------------------------------------------------------------------
+// errors
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(int code, const std::string& msg);
+#define NMG_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::nmg::Error{NMG_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+#define NMG_CHECK(cond, code, msg)                    \
+  do {                                                \
+    if (!(cond)) throw ::nmg::Error{(code), (msg)};   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// matrices in HBM
+enum Layout { L_CSR = 0, L_BSR3 = 1, L_MASS3 = 2 };
+
+struct DevMat {
+  int64_t nrow = 0, ncol = 0, nnz = 0;  // scalar (DoF-level) sizes
+  int layout = L_CSR;
+  // CSR (any matrix): rowptr[nrow+1], col[nnz], val[nnz]
+  int64_t* rowptr = nullptr;
+  int32_t* col = nullptr;
+  double* val = nullptr;
+  float* val32 = nullptr;
+  // 3x3-block rows (BSR3) and scalar-mass-times-I3 (MASS3) share the node
+  // pattern: nb node rows, nbnz node entries, browptr[nb+1], bcol[nbnz]
+  int64_t nb = 0, nbnz = 0;
+  int64_t* browptr = nullptr;
+  int32_t* bcol = nullptr;
+  // BSR3: 9 value planes bval[p*nbnz + k], p = 3*r + c;  MASS3: 1 plane
+  double* bval = nullptr;
+  float* bval32 = nullptr;
+  ~DevMat();
+  void release();
+  double algorithmic_bytes(bool fp32) const;  // one product: matrix + x + y
+};
+
+// creates a DevMat from host CSR (0-based); picks the layout
+DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rowptr, const int32_t* col,
+                    const double* val, bool allow_blocked = true);
+// deep copy on the device
+DevMat* clone_devmat(const DevMat& M, hipStream_t s);
+// M_ij <- M_ij * dl[i] * dr[j] (device vectors, DoF level)
+void scale_devmat(DevMat& M, const double* dl, const double* dr, double sign, hipStream_t s);
+// diag(M) (DoF level) into d (device)
+void diag_devmat(const DevMat& M, double* d, hipStream_t s);
+// refresh float copies of the values
+void make_fp32(DevMat& M, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// kernel launchers (nmgpu_kernels.hip); all asynchronous on stream s
+void k_spmv(const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32 = false);
+// xm <- xm + w (xk - xm + (b - M xk) * ith)
+void k_cheb_step(const DevMat& M, const double* b, const double* xk, double* xm, double w,
+                 double ith, hipStream_t s, bool fp32 = false);
+// vkm <- t (A x - cc vk) - (first ? 0 : vkm);  y += mu * vkm
+void k_filter_step(const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
+                   double t, double cc, double mu, bool first, hipStream_t s, bool fp32 = false);
+void k_fill(int64_t n, double v, double* y, hipStream_t s);
+void k_copy(int64_t n, const double* x, double* y, hipStream_t s);
+void k_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);  // y = a x
+void k_mul(int64_t n, const double* d, double* x, hipStream_t s);                  // x *= d
+// y -= (*a) x  (a device scalar), optionally y -= (*b) z
+void k_axpy_dev(int64_t n, const double* a, const double* x, double* y, hipStream_t s);
+// x *= 1 / *beta ; z *= 1 / *beta (z may be null)
+void k_invscale_dev(int64_t n, const double* beta, double* x, double* z, hipStream_t s);
+// deterministic dot into *out (device); sqrt_out: *out = sqrt(dot)
+void k_dot(int64_t n, const double* x, const double* y, double* out, bool sqrt_out, double* work,
+           hipStream_t s);
+// h[0:k] = V(:,0:k)^T z ; V column-major, ld
+void k_gemv_t(int64_t n, int k, const double* V, int64_t ld, const double* z, double* h,
+              double* work, hipStream_t s);
+// z -= Z(:,0:k) h
+void k_gemv_n_sub(int64_t n, int k, const double* Z, int64_t ld, const double* h, double* z,
+                  hipStream_t s);
+// residual pieces for one vector: out[0] = sum r^2, out[1] = x^T bx, r = ax - lam bx
+void k_resid(int64_t n, const double* ax, const double* bx, const double* x, double lam,
+             double* out, double* work, hipStream_t s);
+size_t dot_work_size();
+size_t gemv_work_size(int64_t n, int k);
+
+// ---------------------------------------------------------------------------
+// host math (nmgpu_host.cpp)
+struct PolyFilter {
+  int deg = 0;
+  int type = 0;  // 0 interior, 1 left end, 2 right end
+  double cc = 0, dd = 1, gam = 0, bar = 0, aa = 0, bb = 0;
+  std::vector<double> mu;
+};
+int find_pol(const double* xintv, double thr_int, double thr_ext, int min_deg, int max_deg,
+             int damping, PolyFilter& pol);
+// eigenvalues of a symmetric tridiagonal (ascending), implicit QL
+int tridiag_eigvals(int m, const double* d, const double* e, std::vector<double>& w);
+// eigenvector of the tridiagonal for an (isolated) eigenvalue: last component
+// and full vector by inverse iteration
+void tridiag_eigvec(int m, const double* d, const double* e, double theta, std::vector<double>& s);
+// Lanczos start vector (splitmix64 of seed and index, uniform [-1,1))
+void start_vector(int64_t n, uint64_t seed, double* v);
+
+}  // namespace nmg

@@ -1,0 +1,530 @@
+// gfx950 kernels of the filtered-Lanczos path.
+//
+// Sparse products run G lanes of a 64-lane wavefront per (node) row: the
+// lanes stride through the row's entries, read value planes with unit stride
+// (coalesced), gather x, and reduce with cross-lane shuffles.  Three storage
+// layouts are used (chosen from the matrix structure at upload):
+//   CSR    generic scalar rows;
+//   BSR3   3x3 blocks per node pair (the stiffness A of the elastic
+//          operator, src/mod_cg_create_matrix.f90:1376-1387): one column
+//          index per 9 values, 9 value planes;
+//   MASS3  scalar mass matrix times I3 (B, src/mod_cg_create_matrix.f90:
+//          1417-1435): one scalar entry serves the three components.
+// Each product carries an epilogue functor so that the vector updates of the
+// Chebyshev B-solve and of the polynomial filter are fused into the same pass.
+#include "nmgpu_internal.h"
+
+namespace nmg {
+
+constexpr int kBlock = 256;
+constexpr int kDotBlocks = 512;
+
+// --------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = group_sum<64>(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// --------------------------------------------------------------------------
+// epilogues
+struct EpiStore {
+  double* y;
+  __device__ __forceinline__ void operator()(int64_t i, double v) const { y[i] = v; }
+};
+struct EpiCheb {
+  const double* b;
+  const double* xk;
+  double* xm;
+  double w, ith;
+  __device__ __forceinline__ void operator()(int64_t i, double v) const {
+    const double xki = xk[i], xmi = xm[i];
+    xm[i] = xmi + w * (xki - xmi + (b[i] - v) * ith);
+  }
+};
+struct EpiFilter {
+  const double* vk;
+  double* vkm;
+  double* y;
+  double t, cc, mu;
+  int first;
+  __device__ __forceinline__ void operator()(int64_t i, double v) const {
+    double vn = t * (v - cc * vk[i]);
+    if (!first) vn -= vkm[i];
+    vkm[i] = vn;
+    y[i] += mu * vn;
+  }
+};
+
+// --------------------------------------------------------------------------
+// row kernels
+template <int G, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void csr_rows(int64_t nrow, const int64_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci,
+                                                   const VT* __restrict__ va,
+                                                   const double* __restrict__ x, Epi epi) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = t / G;
+  const int lane = threadIdx.x & (G - 1);
+  double s = 0.0;
+  if (row < nrow) {
+    const int64_t e = rp[row + 1];
+    for (int64_t k = rp[row] + lane; k < e; k += G) s += (double)va[k] * x[ci[k]];
+  }
+  s = group_sum<G>(s);
+  if (row < nrow && lane == 0) epi(row, s);
+}
+
+template <int G, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
+                                                    const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci,
+                                                    const VT* __restrict__ bv,
+                                                    const double* __restrict__ x, Epi epi) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = t / G;
+  const int lane = threadIdx.x & (G - 1);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  if (row < nb) {
+    const int64_t e = rp[row + 1];
+    for (int64_t k = rp[row] + lane; k < e; k += G) {
+      const int64_t c3 = 3 * (int64_t)ci[k];
+      const double x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
+      y0 += (double)bv[k] * x0 + (double)bv[nbnz + k] * x1 + (double)bv[2 * nbnz + k] * x2;
+      y1 += (double)bv[3 * nbnz + k] * x0 + (double)bv[4 * nbnz + k] * x1 +
+            (double)bv[5 * nbnz + k] * x2;
+      y2 += (double)bv[6 * nbnz + k] * x0 + (double)bv[7 * nbnz + k] * x1 +
+            (double)bv[8 * nbnz + k] * x2;
+    }
+  }
+  y0 = group_sum<G>(y0);
+  y1 = group_sum<G>(y1);
+  y2 = group_sum<G>(y2);
+  if (row < nb && lane < 3) epi(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+}
+
+template <int G, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ci,
+                                                     const VT* __restrict__ mv,
+                                                     const double* __restrict__ x, Epi epi) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = t / G;
+  const int lane = threadIdx.x & (G - 1);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  if (row < nb) {
+    const int64_t e = rp[row + 1];
+    for (int64_t k = rp[row] + lane; k < e; k += G) {
+      const int64_t c3 = 3 * (int64_t)ci[k];
+      const double m = (double)mv[k];
+      y0 += m * x[c3];
+      y1 += m * x[c3 + 1];
+      y2 += m * x[c3 + 2];
+    }
+  }
+  y0 = group_sum<G>(y0);
+  y1 = group_sum<G>(y1);
+  y2 = group_sum<G>(y2);
+  if (row < nb && lane < 3) epi(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+}
+
+static int pick_group(double avg) {
+  if (avg > 40.0) return 64;
+  if (avg > 20.0) return 32;
+  if (avg > 10.0) return 16;
+  return 8;
+}
+
+template <int G, class VT, class Epi>
+static void launch_G(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  if (M.layout == L_CSR) {
+    const int64_t thr = M.nrow * G;
+    const int64_t grid = (thr + kBlock - 1) / kBlock;
+    if (grid > 0)
+      hipLaunchKernelGGL((csr_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
+                         M.rowptr, M.col, v, x, epi);
+  } else {
+    const int64_t thr = M.nb * G;
+    const int64_t grid = (thr + kBlock - 1) / kBlock;
+    if (grid <= 0) return;
+    if (M.layout == L_BSR3)
+      hipLaunchKernelGGL((bsr3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+                         M.nbnz, M.browptr, M.bcol, v, x, epi);
+    else
+      hipLaunchKernelGGL((mass3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+                         M.browptr, M.bcol, v, x, epi);
+  }
+}
+
+template <class VT, class Epi>
+static void launch_rows(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  double avg = M.layout == L_CSR ? (double)M.nnz / (double)(M.nrow ? M.nrow : 1)
+                                 : (double)M.nbnz / (double)(M.nb ? M.nb : 1);
+  switch (pick_group(avg)) {
+    case 64: launch_G<64>(M, v, x, epi, s); break;
+    case 32: launch_G<32>(M, v, x, epi, s); break;
+    case 16: launch_G<16>(M, v, x, epi, s); break;
+    default: launch_G<8>(M, v, x, epi, s); break;
+  }
+  NMG_HIP(hipGetLastError());
+}
+
+template <class Epi>
+static void dispatch(const DevMat& M, const double* x, Epi epi, hipStream_t s, bool fp32) {
+  if (M.layout == L_CSR) {
+    if (fp32 && M.val32) launch_rows(M, M.val32, x, epi, s);
+    else launch_rows(M, M.val, x, epi, s);
+  } else {
+    if (fp32 && M.bval32) launch_rows(M, M.bval32, x, epi, s);
+    else launch_rows(M, M.bval, x, epi, s);
+  }
+}
+
+void k_spmv(const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32) {
+  dispatch(M, x, EpiStore{y}, s, fp32);
+}
+
+void k_cheb_step(const DevMat& M, const double* b, const double* xk, double* xm, double w,
+                 double ith, hipStream_t s, bool fp32) {
+  dispatch(M, xk, EpiCheb{b, xk, xm, w, ith}, s, fp32);
+}
+
+void k_filter_step(const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
+                   double t, double cc, double mu, bool first, hipStream_t s, bool fp32) {
+  dispatch(A, x, EpiFilter{vk, vkm, y, t, cc, mu, first ? 1 : 0}, s, fp32);
+}
+
+// --------------------------------------------------------------------------
+// vector kernels
+__global__ void fill_k(int64_t n, double v, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = v;
+}
+__global__ void scale_copy_k(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = a * x[i];
+}
+__global__ void mul_k(int64_t n, const double* __restrict__ d, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    x[i] *= d[i];
+}
+__global__ void axpy_dev_k(int64_t n, const double* __restrict__ a, const double* __restrict__ x,
+                           double* __restrict__ y) {
+  const double av = a[0];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] -= av * x[i];
+}
+__global__ void invscale_dev_k(int64_t n, const double* __restrict__ beta, double* __restrict__ x,
+                               double* __restrict__ z) {
+  const double ib = 1.0 / beta[0];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    x[i] *= ib;
+    if (z) z[i] *= ib;
+  }
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+void k_fill(int64_t n, double v, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(fill_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, v, y);
+  NMG_HIP(hipGetLastError());
+}
+void k_copy(int64_t n, const double* x, double* y, hipStream_t s) {
+  NMG_HIP(hipMemcpyAsync(y, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+}
+void k_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(scale_copy_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, y);
+  NMG_HIP(hipGetLastError());
+}
+void k_mul(int64_t n, const double* d, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(mul_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, x);
+  NMG_HIP(hipGetLastError());
+}
+void k_axpy_dev(int64_t n, const double* a, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(axpy_dev_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, y);
+  NMG_HIP(hipGetLastError());
+}
+void k_invscale_dev(int64_t n, const double* beta, double* x, double* z, hipStream_t s) {
+  hipLaunchKernelGGL(invscale_dev_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, beta, x, z);
+  NMG_HIP(hipGetLastError());
+}
+
+// deterministic two-stage reductions -----------------------------------------
+__global__ __launch_bounds__(kBlock) void dot_part_k(int64_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* part) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    s += x[i] * y[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kBlock) void sum_final_k(const double* part, int np, double* out,
+                                                      int sqrt_out) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[0] = sqrt_out ? sqrt(s) : s;
+}
+
+size_t dot_work_size() { return 2 * kDotBlocks; }
+
+void k_dot(int64_t n, const double* x, const double* y, double* out, bool sqrt_out, double* work,
+           hipStream_t s) {
+  hipLaunchKernelGGL(dot_part_k, dim3(kDotBlocks), dim3(kBlock), 0, s, n, x, y, work);
+  hipLaunchKernelGGL(sum_final_k, dim3(1), dim3(kBlock), 0, s, work, kDotBlocks, out,
+                     sqrt_out ? 1 : 0);
+  NMG_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void resid_part_k(int64_t n, const double* __restrict__ ax,
+                                                       const double* __restrict__ bx,
+                                                       const double* __restrict__ x, double lam,
+                                                       double* part) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0, q = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double r = ax[i] - lam * bx[i];
+    s += r * r;
+    q += x[i] * bx[i];
+  }
+  s = block_sum(s, sh);
+  q = block_sum(q, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    part[kDotBlocks + blockIdx.x] = q;
+  }
+}
+
+void k_resid(int64_t n, const double* ax, const double* bx, const double* x, double lam, double* out,
+             double* work, hipStream_t s) {
+  hipLaunchKernelGGL(resid_part_k, dim3(kDotBlocks), dim3(kBlock), 0, s, n, ax, bx, x, lam, work);
+  hipLaunchKernelGGL(sum_final_k, dim3(1), dim3(kBlock), 0, s, work, kDotBlocks, out, 0);
+  hipLaunchKernelGGL(sum_final_k, dim3(1), dim3(kBlock), 0, s, work + kDotBlocks, kDotBlocks,
+                     out + 1, 0);
+  NMG_HIP(hipGetLastError());
+}
+
+// tall-skinny GEMV for the re-orthogonalisation ------------------------------
+constexpr int kCG = 8;  // columns per block in V^T z
+
+static int gemv_chunks(int64_t n, int k) {
+  const int cgroups = (k + kCG - 1) / kCG;
+  int64_t ch = 2048 / (cgroups > 0 ? cgroups : 1);
+  int64_t maxch = (n + 4 * kBlock - 1) / (4 * kBlock);
+  if (ch > maxch) ch = maxch;
+  if (ch < 1) ch = 1;
+  if (ch > 1024) ch = 1024;
+  return (int)ch;
+}
+
+// partial sums of V^T z: at most (k + kCG) columns x 1024 chunks
+size_t gemv_work_size(int64_t n, int k) {
+  (void)n;
+  return (size_t)(k + kCG) * 1024 + 16;
+}
+
+__global__ __launch_bounds__(kBlock) void gemv_t_part_k(int64_t n, int k, const double* __restrict__ V,
+                                                        int64_t ld, const double* __restrict__ z,
+                                                        double* part, int nch) {
+  __shared__ double sh[kCG][kBlock / 64];
+  const int j0 = blockIdx.y * kCG;
+  const int64_t per = ((n + nch - 1) / nch + kBlock - 1) / kBlock * kBlock;
+  const int64_t start = (int64_t)blockIdx.x * per;
+  int64_t end = start + per;
+  if (end > n) end = n;
+  double acc[kCG];
+#pragma unroll
+  for (int c = 0; c < kCG; ++c) acc[c] = 0.0;
+  const int ncol = (k - j0) < kCG ? (k - j0) : kCG;
+  for (int64_t i = start + threadIdx.x; i < end; i += kBlock) {
+    const double zi = z[i];
+#pragma unroll
+    for (int c = 0; c < kCG; ++c)
+      if (c < ncol) acc[c] += V[(int64_t)(j0 + c) * ld + i] * zi;
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < kCG; ++c) {
+    double v = group_sum<64>(acc[c]);
+    if (l == 0) sh[c][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < ncol) {
+    double s = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) s += sh[threadIdx.x][i];
+    part[(int64_t)(j0 + threadIdx.x) * nch + blockIdx.x] = s;
+  }
+}
+
+__global__ void gemv_t_final_k(int k, const double* __restrict__ part, int nch, double* h) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[(int64_t)j * nch + c];
+  h[j] = s;
+}
+
+void k_gemv_t(int64_t n, int k, const double* V, int64_t ld, const double* z, double* h, double* work,
+              hipStream_t s) {
+  if (k <= 0) return;
+  const int nch = gemv_chunks(n, k);
+  dim3 grid(nch, (k + kCG - 1) / kCG);
+  hipLaunchKernelGGL(gemv_t_part_k, grid, dim3(kBlock), 0, s, n, k, V, ld, z, work, nch);
+  hipLaunchKernelGGL(gemv_t_final_k, dim3((k + 127) / 128), dim3(128), 0, s, k, work, nch, h);
+  NMG_HIP(hipGetLastError());
+}
+
+constexpr int kHT = 256;  // h tile staged in LDS
+
+__global__ __launch_bounds__(kBlock) void gemv_n_sub_k(int64_t n, int k, const double* __restrict__ Z,
+                                                       int64_t ld, const double* __restrict__ h,
+                                                       double* __restrict__ z) {
+  __shared__ double hs[kHT];
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double s = 0.0;
+  for (int j0 = 0; j0 < k; j0 += kHT) {
+    const int nt = (k - j0) < kHT ? (k - j0) : kHT;
+    __syncthreads();
+    if ((int)threadIdx.x < nt) hs[threadIdx.x] = h[j0 + threadIdx.x];
+    __syncthreads();
+    if (i < n)
+      for (int j = 0; j < nt; ++j) s += Z[(int64_t)(j0 + j) * ld + i] * hs[j];
+  }
+  if (i < n) z[i] -= s;
+}
+
+void k_gemv_n_sub(int64_t n, int k, const double* Z, int64_t ld, const double* h, double* z,
+                  hipStream_t s) {
+  if (k <= 0) return;
+  const int64_t grid = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(gemv_n_sub_k, dim3((unsigned)grid), dim3(kBlock), 0, s, n, k, Z, ld, h, z);
+  NMG_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------------------------
+// matrix maintenance kernels
+__global__ void csr_scale_k(int64_t nrow, const int64_t* rp, const int32_t* ci, double* v,
+                            const double* dl, const double* dr, double sign) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrow) return;
+  const double a = sign * dl[r];
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k) v[k] = a * v[k] * dr[ci[k]];
+}
+__global__ void bsr3_scale_k(int64_t nb, int64_t nbnz, const int64_t* rp, const int32_t* ci, double* bv,
+                             const double* dl, const double* dr, double sign) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nb) return;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+    const int64_t c = ci[k];
+    for (int p = 0; p < 9; ++p) {
+      const int rr = p / 3, cc = p % 3;
+      bv[p * nbnz + k] = sign * dl[3 * r + rr] * bv[p * nbnz + k] * dr[3 * c + cc];
+    }
+  }
+}
+__global__ void mass3_scale_k(int64_t nb, const int64_t* rp, const int32_t* ci, double* mv,
+                              const double* dl, const double* dr, double sign) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nb) return;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k) mv[k] = sign * dl[3 * r] * mv[k] * dr[3 * (int64_t)ci[k]];
+}
+
+void scale_devmat(DevMat& M, const double* dl, const double* dr, double sign, hipStream_t s) {
+  if (M.layout == L_CSR) {
+    hipLaunchKernelGGL(csr_scale_k, dim3((unsigned)((M.nrow + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nrow, M.rowptr, M.col, M.val, dl, dr, sign);
+  } else if (M.layout == L_BSR3) {
+    hipLaunchKernelGGL(bsr3_scale_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, dl, dr, sign);
+  } else {
+    hipLaunchKernelGGL(mass3_scale_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nb, M.browptr, M.bcol, M.bval, dl, dr, sign);
+  }
+  NMG_HIP(hipGetLastError());
+  if (M.val32 || M.bval32) make_fp32(M, s);
+}
+
+__global__ void csr_diag_k(int64_t nrow, const int64_t* rp, const int32_t* ci, const double* v, double* d) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrow) return;
+  double x = 0.0;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+    if (ci[k] == r) x = v[k];
+  d[r] = x;
+}
+__global__ void bsr3_diag_k(int64_t nb, int64_t nbnz, const int64_t* rp, const int32_t* ci,
+                            const double* bv, double* d) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nb) return;
+  double x0 = 0, x1 = 0, x2 = 0;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+    if (ci[k] == r) {
+      x0 = bv[0 * nbnz + k];
+      x1 = bv[4 * nbnz + k];
+      x2 = bv[8 * nbnz + k];
+    }
+  d[3 * r] = x0;
+  d[3 * r + 1] = x1;
+  d[3 * r + 2] = x2;
+}
+__global__ void mass3_diag_k(int64_t nb, const int64_t* rp, const int32_t* ci, const double* mv, double* d) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nb) return;
+  double x = 0;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+    if (ci[k] == r) x = mv[k];
+  d[3 * r] = d[3 * r + 1] = d[3 * r + 2] = x;
+}
+
+void diag_devmat(const DevMat& M, double* d, hipStream_t s) {
+  if (M.layout == L_CSR)
+    hipLaunchKernelGGL(csr_diag_k, dim3((unsigned)((M.nrow + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nrow, M.rowptr, M.col, M.val, d);
+  else if (M.layout == L_BSR3)
+    hipLaunchKernelGGL(bsr3_diag_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, d);
+  else
+    hipLaunchKernelGGL(mass3_diag_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       M.nb, M.browptr, M.bcol, M.bval, d);
+  NMG_HIP(hipGetLastError());
+}
+
+__global__ void to_f32_k(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = (float)x[i];
+}
+
+void make_fp32(DevMat& M, hipStream_t s) {
+  if (M.layout == L_CSR) {
+    if (!M.val32) NMG_HIP(hipMalloc(&M.val32, sizeof(float) * (M.nnz > 0 ? M.nnz : 1)));
+    hipLaunchKernelGGL(to_f32_k, dim3(grid_for(M.nnz)), dim3(kBlock), 0, s, M.nnz, M.val, M.val32);
+  } else {
+    const int64_t nv = (M.layout == L_BSR3 ? 9 : 1) * M.nbnz;
+    if (!M.bval32) NMG_HIP(hipMalloc(&M.bval32, sizeof(float) * (nv > 0 ? nv : 1)));
+    hipLaunchKernelGGL(to_f32_k, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, M.bval, M.bval32);
+  }
+  NMG_HIP(hipGetLastError());
+}
+
+}  // namespace nmg

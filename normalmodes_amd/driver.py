@@ -1,0 +1,185 @@
+"""The reference's operator set-up and eigensolver driver on the nmgpu C-ABI.
+
+``setupmatvec`` mirrors src/mod_matvec.f90:24-250 (solid case) and
+``apply_pevsl`` mirrors pnm_apply_pevsl, src/mod_pevsl.f90:16-222: same call
+order, same parameters (MLAN 3000, LANSTEP 5000, TOL 1e-5, thresholds
+0.8/0.7, MAXIT = 3*max(4*(EVINT+2), 2000), B-bounds (1000, 2000, 1e-12),
+chebiter degree 45 for P2 and 25 for P1).  The interval may be given in mHz
+(as the reference's lowfreq/upfreq) or directly in lambda = omega^2.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import pevsl as pv
+
+__all__ = ["MatvecSetup", "setupmatvec", "apply_pevsl", "EigResult", "chebiter_degree",
+           "freq_to_lambda", "lambda_to_freq"]
+
+EVINT = 800
+
+
+def freq_to_lambda(f_mhz: float) -> float:
+    """lambda = (2 pi f)^2 * 1e-6 for f in mHz (src/mod_pevsl.f90:93-94)."""
+    return (2.0 * math.pi * f_mhz) ** 2 * 1.0e-6
+
+
+def lambda_to_freq(lam):
+    """f [mHz] = sqrt(lambda)/(2 pi) * 1e3 (src/mod_pevsl.f90:177-181)."""
+    return np.sqrt(np.maximum(lam, 0.0)) / (2.0 * math.pi) * 1.0e3
+
+
+def chebiter_degree(porder: int, blmin: float, blmax: float, policy="reference", eps=1e-13) -> int:
+    """Degree of the Chebyshev B-solve.
+
+    ``reference``: 45 for P2, 25 for P1 (src/mod_matvec.f90:88-92).
+    ``auto``: the smallest c with 2 rho^c <= eps, rho = (sqrt k - 1)/(sqrt k + 1),
+    k = blmax/blmin -- enough for the B-solve error to stay below eps.
+    An integer selects that degree.
+    """
+    if isinstance(policy, int):
+        return policy
+    ref = 45 if porder == 2 else 25
+    if policy == "reference":
+        return ref
+    if policy == "auto":
+        k = blmax / blmin
+        rho = (math.sqrt(k) - 1.0) / (math.sqrt(k) + 1.0)
+        return max(1, int(math.ceil(math.log(eps / 2.0) / math.log(rho))))
+    raise ValueError(policy)
+
+
+@dataclass
+class MatvecSetup:
+    """Device-side counterpart of the reference's ``mymatvec``
+    (src/mod_cg_datatype.f90:88-115)."""
+    n: int
+    matA: int
+    matB: int
+    d: np.ndarray
+    blmin: float
+    blmax: float
+    bdeg: int
+    chebB: int
+    porder: int
+    timings: dict = field(default_factory=dict)
+
+    def free(self):
+        pv.pevsl_chebiter_free(self.chebB)
+        pv.csr_free(self.matA)
+        pv.csr_free(self.matB)
+
+
+def setupmatvec(problem, *, porder: int | None = None, bdeg="reference", seed: int = 7,
+                bounds=(1000, 2000, 1e-12)) -> MatvecSetup:
+    """Upload A and B, scale B, bound its spectrum and set up its Chebyshev
+    solve (src/mod_matvec.f90:24-95, 225-247)."""
+    porder = porder or problem.mesh.porder
+    t = {}
+    t0 = time.perf_counter()
+    n = problem.n
+    matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
+    matA = pv.csr_create(problem.A.rowptr, problem.A.col, problem.A.val)
+    t["upload"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d = pv.csr_diagscale(matB, 1.0)                      # Bdiagscaling, :252-342
+    hB = pv.pevsl_start(0)                               # :75
+    pv.pevsl_setprobsizes(hB, n, n, -1)
+    pv.pevsl_set_seed(hB, seed)
+    pv.pevsl_setamv(hB, matB)                            # :80
+    blmin, blmax = pv.pevsl_lanbounds(hB, *bounds)       # :83-85
+    pv.pevsl_finish(hB)
+    deg = chebiter_degree(porder, blmin, blmax, bdeg)
+    chebB = pv.pevsl_setup_chebiter(blmin, blmax, deg, matB)   # :93
+    t["setup_B"] = time.perf_counter() - t0
+    return MatvecSetup(n=n, matA=matA, matB=matB, d=d, blmin=blmin, blmax=blmax, bdeg=deg,
+                       chebB=chebB, porder=porder, timings=t)
+
+
+@dataclass
+class EigResult:
+    lam: np.ndarray              # ascending
+    X: np.ndarray | None         # (nev, n) scaled-coordinate eigenvectors x~
+    res_ref: np.ndarray          # reference residual definition
+    res2: np.ndarray             # ||A~x - lam B~x|| / |lam|, x B~-normalised
+    lmin: float
+    lmax: float
+    xintv: np.ndarray
+    deg: int
+    bar: float
+    lan_steps: int
+    filter_applies: int
+    ncand: int
+    timings: dict
+
+    @property
+    def freq_mhz(self):
+        return lambda_to_freq(self.lam)
+
+    def physical_vectors(self, d):
+        """x = D x~ as saved by the reference (src/mod_pevsl.f90:199)."""
+        return None if self.X is None else self.X * d[None, :]
+
+
+def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float | None = None,
+                interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
+                lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
+                rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True) -> EigResult:
+    """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
+    if interval is None:
+        a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
+    else:
+        a, b = interval
+    t = {}
+    t0 = time.perf_counter()
+    h = pv.pevsl_start(0)                                     # :54
+    try:
+        pv.pevsl_setprobsizes(h, mv.n, mv.n, -1)              # :57
+        pv.pevsl_set_seed(h, seed)
+        pv.pevsl_setbmv(h, mv.matB)                           # :69
+        pv.pevsl_setbsol_chebiter(h, 2, mv.chebB)             # :72-73
+        pv.pevsl_setamv_dad(h, mv.matA, mv.d)                 # :77
+        pv.pevsl_set_geneig(h)                                # :82
+        if precision != pv.FP64:
+            pv.pevsl_set_precision(h, precision)
+        pv.set_lanczos_controls(h, rtol=rtol)
+        t["setup"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lmin, lmax = pv.pevsl_lanbounds(h, *lanbounds)        # :84
+        t["lanbounds"] = time.perf_counter() - t0
+        if a < 1.0e-10:                                       # :98-100
+            a = lmin
+        xintv = np.array([a, b, lmin, lmax])
+        t0 = time.perf_counter()
+        pol = pv.pevsl_findpol(xintv, thr_int, thr_ext)       # :115
+        pinfo = pv.pol_info(pol)
+        t["findpol"] = time.perf_counter() - t0
+        nev = EVINT + 2
+        mlan = max(4 * nev, 2000)
+        maxit = 3 * mlan                                      # :116-119
+        t0 = time.perf_counter()
+        pv.pevsl_cheblannr(h, xintv, maxit, tol, pol)         # :122
+        t["cheblannr"] = time.perf_counter() - t0
+        st = pv.solve_stats(h)
+        t0 = time.perf_counter()
+        if residual_check:
+            r1, r2 = pv.residuals(h)                          # :137-162
+        else:
+            k = pv.pevsl_get_nev(h)
+            r1 = r2 = np.full(k, np.nan)
+        t["residual"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lam, X = pv.pevsl_copy_result(h, mv.n, vectors)       # :124-130
+        t["copy_result"] = time.perf_counter() - t0
+        pv.pevsl_freepol(pol)                                 # :215
+    finally:
+        pv.pevsl_finish(h)                                    # :220
+    order = np.argsort(lam, kind="stable")                    # ssort_real, :165
+    return EigResult(lam=lam[order], X=None if X is None else X[order], res_ref=r1[order],
+                     res2=r2[order], lmin=lmin, lmax=lmax, xintv=xintv, deg=pinfo.deg,
+                     bar=pinfo.bar, lan_steps=st["lan_steps"], filter_applies=st["filter_applies"],
+                     ncand=st["ncand"], timings=t)

@@ -1,0 +1,133 @@
+"""Material models evaluated per element node.
+
+The reference stores vp, vs and rho per element node (discontinuous across
+elements) in km/s and g/cm^3 (src/mod_cg_models.f90:301-360) and takes
+lam = rho (vp^2 - 2 vs^2), mu = rho vs^2 (src/mod_cg_create_matrix.f90:1018-1020).
+Elements with sum(vs) <= TOL are fluid (separate_fs, src/mod_cg_models.f90:235-277).
+This module produces the same per-element-node arrays for the synthetic bodies
+used here: a homogeneous ball and radially layered balls whose layer is picked
+by the element centroid so that interfaces are sharp, as in the reference data.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mesh import TetMesh
+
+__all__ = ["Layer", "RadialModel", "homogeneous", "three_layer", "three_layer_solid",
+           "element_node_values", "lame", "gaussian_random_field"]
+
+
+@dataclass
+class Layer:
+    r0: float
+    r1: float
+    rho: tuple[float, float]   # value at r0, value at r1 (linear in r)
+    vp: tuple[float, float]
+    vs: tuple[float, float]
+
+    def eval(self, r):
+        t = (r - self.r0) / (self.r1 - self.r0)
+        lin = lambda ab: ab[0] + (ab[1] - ab[0]) * t  # noqa: E731
+        return lin(self.rho), lin(self.vp), lin(self.vs)
+
+
+@dataclass
+class RadialModel:
+    layers: list[Layer]
+    name: str = "radial"
+    axes: tuple[float, float, float] = (1.0, 1.0, 1.0)
+    meta: dict = field(default_factory=dict)
+
+    def layer_index(self, r):
+        edges = np.array([L.r1 for L in self.layers[:-1]])
+        return np.searchsorted(edges, r, side="right")
+
+
+def homogeneous(rho=1.0, vp=1.73, vs=1.0) -> RadialModel:
+    return RadialModel([Layer(0.0, 1.0, (rho, rho), (vp, vp), (vs, vs))], name="homogeneous")
+
+
+def _profile(r0, r1, lo, hi):
+    # linear in r from the centre value lo (r=0) to the surface value hi (r=1)
+    return (lo + (hi - lo) * r0, lo + (hi - lo) * r1)
+
+
+def three_layer() -> RadialModel:
+    """Solid inner core r<0.19, fluid outer core 0.19-0.55, solid mantle 0.55-1;
+    rho 13->4, vp 11->8, vs 3.5->6.5 linear in r (vs = 0 in the fluid)."""
+    L = []
+    for r0, r1, fluid in ((0.0, 0.19, False), (0.19, 0.55, True), (0.55, 1.0, False)):
+        rho = _profile(r0, r1, 13.0, 4.0)
+        vp = _profile(r0, r1, 11.0, 8.0)
+        vs = (0.0, 0.0) if fluid else _profile(r0, r1, 3.5, 6.5)
+        L.append(Layer(r0, r1, rho, vp, vs))
+    return RadialModel(L, name="three_layer")
+
+
+def three_layer_solid() -> RadialModel:
+    """The three-layer profile with the outer core kept solid (vs from the same
+    linear profile): the layered body on the solid-only operator path."""
+    L = []
+    for r0, r1 in ((0.0, 0.19), (0.19, 0.55), (0.55, 1.0)):
+        L.append(Layer(r0, r1, _profile(r0, r1, 13.0, 4.0), _profile(r0, r1, 11.0, 8.0),
+                       _profile(r0, r1, 3.5, 6.5)))
+    return RadialModel(L, name="three_layer_solid")
+
+
+def element_node_values(mesh: TetMesh, model: RadialModel, perturb=None):
+    """Return (rho, vp, vs), each (n_elem, pNp) float64.
+
+    ``perturb`` is an optional callable (x[n,3]) -> (drho, dvp, dvs) relative
+    perturbations applied multiplicatively at the nodes.
+    """
+    ax = np.asarray(model.axes, dtype=np.float64)
+    X = mesh.nodes / ax[None, :]
+    xe = X[mesh.elems]                       # (ne, np, 3)
+    rn = np.linalg.norm(xe, axis=2)          # node radius
+    rc = np.linalg.norm(X[mesh.tets].mean(axis=1), axis=1)
+    li = model.layer_index(rc)
+    rho = np.empty_like(rn)
+    vp = np.empty_like(rn)
+    vs = np.empty_like(rn)
+    for k, L in enumerate(model.layers):
+        sel = li == k
+        if not sel.any():
+            continue
+        a, b, c = L.eval(rn[sel])
+        rho[sel], vp[sel], vs[sel] = a, b, c
+    if perturb is not None:
+        drho, dvp, dvs = perturb(mesh.nodes[mesh.elems].reshape(-1, 3))
+        shp = rho.shape
+        rho = rho * (1.0 + drho.reshape(shp))
+        vp = vp * (1.0 + dvp.reshape(shp))
+        vs = vs * (1.0 + dvs.reshape(shp))
+    return rho, vp, vs
+
+
+def lame(rho, vp, vs):
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2.0 * mu
+    return lam, mu
+
+
+def gaussian_random_field(seed=2018, rms=0.02, corr_len=0.1, n_modes=256):
+    """Seeded smooth random field (random Fourier features, Gaussian covariance
+    with correlation length ``corr_len``), scaled to ``rms``.  Returns a callable
+    x[n,3] -> (drho, dvp, dvs) with independent fields."""
+    rng = np.random.default_rng(seed)
+    fields = []
+    for _ in range(3):
+        k = rng.normal(scale=1.0 / corr_len, size=(n_modes, 3))
+        ph = rng.uniform(0, 2 * np.pi, size=n_modes)
+        fields.append((k, ph))
+
+    def f(x):
+        out = []
+        for k, ph in fields:
+            v = np.sqrt(2.0 / n_modes) * np.cos(x @ k.T + ph).sum(axis=1)
+            out.append(rms * v)
+        return tuple(out)
+    return f

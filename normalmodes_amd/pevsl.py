@@ -1,0 +1,300 @@
+"""Python binding of the nmgpu C-ABI, named after the pEVSL Fortran-90 calls
+that NormalModes makes (src/mod_pevsl.f90, src/mod_matvec.f90).
+
+Every function raises :class:`PevslError` on a non-zero status; there is no
+fallback implementation: the HIP library must be built and a GPU present.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._native import load_gpu
+
+__all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprobsizes",
+           "pevsl_parcsrcreate", "csr_create", "csr_free", "csr_info", "parcsr_matvec",
+           "csr_diagscale", "pevsl_setamv_dad", "pevsl_setamv", "pevsl_setbmv",
+           "pevsl_setbsol_chebiter", "pevsl_set_geneig", "pevsl_set_precision",
+           "pevsl_set_seed", "pevsl_setup_chebiter", "pevsl_chebiter", "pevsl_chebiter_free",
+           "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
+           "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
+           "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
+           "bench_spmv", "bench_chebstep", "FP64", "FP32_FILTER", "LAYOUTS"]
+
+FP64 = 0
+FP32_FILTER = 1
+LAYOUTS = {0: "csr", 1: "bsr3", 2: "mass3"}
+
+_i64 = ctypes.c_int64
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_ip = ctypes.POINTER(ctypes.c_int)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_d = ctypes.c_double
+
+
+class PevslError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nmgpu error {code}: {msg}")
+        self.code = code
+
+
+_L = None
+
+
+def _lib():
+    global _L
+    if _L is None:
+        L = load_gpu()
+        L.nmg_last_error.restype = ctypes.c_char_p
+        _L = L
+    return _L
+
+
+def _check(rc):
+    if rc != 0:
+        raise PevslError(rc, _lib().nmg_last_error().decode(errors="replace"))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+Handle = int
+
+
+# --- solver handle (src/mod_pevsl.f90:54-57, 220) ----------------------------
+def pevsl_start(comm: int = 0) -> Handle:
+    h = _i64()
+    _check(_lib().nmg_start(ctypes.c_int(comm), ctypes.byref(h)))
+    return h.value
+
+
+def pevsl_finish(h: Handle):
+    _check(_lib().nmg_finish(_i64(h)))
+
+
+def pevsl_setprobsizes(h: Handle, n_glob: int, n_loc: int, nfirst: int = -1):
+    _check(_lib().nmg_setprobsizes(_i64(h), _i64(n_glob), _i64(n_loc), _i64(nfirst)))
+
+
+# --- matrices (src/mod_matvec.f90:69-71 etc.) --------------------------------
+def pevsl_parcsrcreate(nrow_g, ncol_g, row_starts, col_starts, ia, ja, a, comm=0) -> Handle:
+    ia = np.ascontiguousarray(ia, dtype=np.int32)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    a = _f64(a)
+    rs = np.ascontiguousarray(row_starts, dtype=np.int32)
+    cs = np.ascontiguousarray(col_starts, dtype=np.int32)
+    m = _i64()
+    _check(_lib().nmg_parcsr_create(ctypes.c_int(nrow_g), ctypes.c_int(ncol_g), _p(rs, _ip), _p(cs, _ip),
+                                    _p(ia, _ip), _p(ja, _ip), _p(a, _f64p), ctypes.c_int(comm),
+                                    ctypes.byref(m)))
+    return m.value
+
+
+def csr_create(rowptr, col, val, n_cols=None) -> Handle:
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = _f64(val)
+    n = rowptr.size - 1
+    m = _i64()
+    _check(_lib().nmg_csr_create(_i64(n), _i64(n if n_cols is None else n_cols), _p(rowptr, _i64p),
+                                 _p(col, _i32p), _p(val, _f64p), ctypes.byref(m)))
+    return m.value
+
+
+def csr_free(mat: Handle):
+    _check(_lib().nmg_csr_free(_i64(mat)))
+
+
+def csr_info(mat: Handle):
+    nr, nc, nz = _i64(), _i64(), _i64()
+    lay = ctypes.c_int()
+    _check(_lib().nmg_csr_info(_i64(mat), ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nz),
+                               ctypes.byref(lay)))
+    return dict(nrow=nr.value, ncol=nc.value, nnz=nz.value, layout=LAYOUTS[lay.value])
+
+
+def parcsr_matvec(x, mat: Handle, n_rows=None):
+    x = _f64(x)
+    info = csr_info(mat) if n_rows is None else dict(nrow=n_rows)
+    y = np.empty(info["nrow"])
+    _check(_lib().nmg_parcsr_matvec(_p(x, _f64p), _p(y, _f64p), _i64(mat)))
+    return y
+
+
+def csr_diagscale(mat: Handle, sign: float = 1.0):
+    n = csr_info(mat)["nrow"]
+    d = np.empty(n)
+    _check(_lib().nmg_csr_diagscale(_i64(mat), _d(sign), _p(d, _f64p)))
+    return d
+
+
+# --- operators (src/mod_pevsl.f90:69-82) ------------------------------------
+def pevsl_setamv_dad(h: Handle, matA: Handle, d):
+    d = _f64(d)
+    _check(_lib().nmg_set_amv_dad(_i64(h), _i64(matA), _p(d, _f64p)))
+
+
+def pevsl_setamv(h: Handle, mat: Handle):
+    _check(_lib().nmg_set_amv(_i64(h), _i64(mat)))
+
+
+def pevsl_setbmv(h: Handle, matB: Handle):
+    _check(_lib().nmg_set_bmv(_i64(h), _i64(matB)))
+
+
+def pevsl_setbsol_chebiter(h: Handle, chebtype: int, cheb: Handle):
+    _check(_lib().nmg_set_bsol_chebiter(_i64(h), ctypes.c_int(chebtype), _i64(cheb)))
+
+
+def pevsl_set_geneig(h: Handle):
+    _check(_lib().nmg_set_geneig(_i64(h)))
+
+
+def pevsl_set_precision(h: Handle, mode: int):
+    _check(_lib().nmg_set_precision(_i64(h), ctypes.c_int(mode)))
+
+
+def pevsl_set_seed(h: Handle, seed: int):
+    _check(_lib().nmg_set_seed(_i64(h), ctypes.c_uint64(seed)))
+
+
+def set_lanczos_controls(h: Handle, ntest=30, cycle=20, rtol=1e-9):
+    _check(_lib().nmg_set_lanczos_controls(_i64(h), ctypes.c_int(ntest), ctypes.c_int(cycle), _d(rtol)))
+
+
+# --- Chebyshev iteration (src/mod_matvec.f90:88-93, 475-483) ----------------
+def pevsl_setup_chebiter(lmin: float, lmax: float, deg: int, mat: Handle) -> Handle:
+    c = _i64()
+    _check(_lib().nmg_setup_chebiter(_d(lmin), _d(lmax), ctypes.c_int(deg), _i64(mat), ctypes.byref(c)))
+    return c.value
+
+
+def pevsl_chebiter(chebtype: int, b, cheb: Handle):
+    b = _f64(b)
+    x = np.empty_like(b)
+    _check(_lib().nmg_chebiter(ctypes.c_int(chebtype), _p(b, _f64p), _p(x, _f64p), _i64(cheb)))
+    return x
+
+
+def pevsl_chebiter_stats(cheb: Handle):
+    a, b = _i64(), _i64()
+    s = _d()
+    _check(_lib().nmg_chebiter_stats(_i64(cheb), ctypes.byref(a), ctypes.byref(b), ctypes.byref(s)))
+    return dict(ncalls=a.value, nmatvec=b.value, seconds=s.value)
+
+
+def pevsl_chebiter_free(cheb: Handle):
+    _check(_lib().nmg_chebiter_free(_i64(cheb)))
+
+
+# --- eigensolver (src/mod_pevsl.f90:84-130) ---------------------------------
+def pevsl_lanbounds(h: Handle, mlan: int, maxit: int, tol: float):
+    lo, hi = _d(), _d()
+    _check(_lib().nmg_lanbounds(_i64(h), ctypes.c_int(mlan), ctypes.c_int(maxit), _d(tol),
+                                ctypes.byref(lo), ctypes.byref(hi)))
+    return lo.value, hi.value
+
+
+def pevsl_findpol(xintv, thr_int: float, thr_ext: float) -> Handle:
+    x = _f64(xintv)
+    p = _i64()
+    _check(_lib().nmg_findpol(_p(x, _f64p), _d(thr_int), _d(thr_ext), ctypes.byref(p)))
+    return p.value
+
+
+@dataclass
+class PolInfo:
+    deg: int
+    cc: float
+    dd: float
+    gam: float
+    bar: float
+    mu: np.ndarray
+
+
+def pol_info(pol: Handle) -> PolInfo:
+    deg = ctypes.c_int()
+    cc, dd, gam, bar = _d(), _d(), _d(), _d()
+    _check(_lib().nmg_pol_info(_i64(pol), ctypes.byref(deg), ctypes.byref(cc), ctypes.byref(dd),
+                               ctypes.byref(gam), ctypes.byref(bar), None))
+    mu = np.empty(deg.value + 1)
+    _check(_lib().nmg_pol_info(_i64(pol), None, None, None, None, None, _p(mu, _f64p)))
+    return PolInfo(deg.value, cc.value, dd.value, gam.value, bar.value, mu)
+
+
+def pevsl_freepol(pol: Handle):
+    _check(_lib().nmg_freepol(_i64(pol)))
+
+
+def pevsl_cheblannr(h: Handle, xintv, maxit: int, tol: float, pol: Handle):
+    x = _f64(xintv)
+    _check(_lib().nmg_cheblannr(_i64(h), _p(x, _f64p), ctypes.c_int(maxit), _d(tol), _i64(pol)))
+
+
+def pevsl_get_nev(h: Handle) -> int:
+    n = ctypes.c_int()
+    _check(_lib().nmg_get_nev(_i64(h), ctypes.byref(n)))
+    return n.value
+
+
+def pevsl_copy_result(h: Handle, n_loc: int, vectors: bool = True):
+    nev = pevsl_get_nev(h)
+    val = np.empty(nev)
+    vec = np.empty((nev, n_loc)) if vectors else None
+    _check(_lib().nmg_copy_result(_i64(h), _p(val, _f64p), _p(vec, _f64p) if vectors else None, _i64(n_loc)))
+    return val, vec
+
+
+def residuals(h: Handle):
+    nev = pevsl_get_nev(h)
+    r1 = np.empty(nev)
+    r2 = np.empty(nev)
+    _check(_lib().nmg_residuals(_i64(h), _p(r1, _f64p), _p(r2, _f64p)))
+    return r1, r2
+
+
+def solve_stats(h: Handle):
+    ls = ctypes.c_int()
+    fa = _i64()
+    nc = ctypes.c_int()
+    sec = _d()
+    _check(_lib().nmg_solve_stats(_i64(h), ctypes.byref(ls), ctypes.byref(fa), ctypes.byref(nc),
+                                  ctypes.byref(sec)))
+    return dict(lan_steps=ls.value, filter_applies=fa.value, ncand=nc.value, seconds=sec.value)
+
+
+def filter_apply(h: Handle, pol: Handle, z):
+    z = _f64(z)
+    y = np.empty_like(z)
+    _check(_lib().nmg_filter_apply(_i64(h), _i64(pol), _p(z, _f64p), _p(y, _f64p)))
+    return y
+
+
+def apply(h: Handle, which: int, x):
+    """which: 0 A~ (DAD), 1 B~, 2 B-solve."""
+    x = _f64(x)
+    y = np.empty_like(x)
+    _check(_lib().nmg_apply(_i64(h), ctypes.c_int(which), _p(x, _f64p), _p(y, _f64p)))
+    return y
+
+
+def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):
+    s, b = _d(), _d()
+    _check(_lib().nmg_bench_spmv(_i64(mat), ctypes.c_int(reps), ctypes.c_int(int(fp32)), ctypes.byref(s),
+                                 ctypes.byref(b)))
+    return s.value, b.value
+
+
+def bench_chebstep(cheb: Handle, reps: int = 20, fp32: bool = False):
+    s, b = _d(), _d()
+    _check(_lib().nmg_bench_chebstep(_i64(cheb), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
+                                     ctypes.byref(s), ctypes.byref(b)))
+    return s.value, b.value

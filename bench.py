@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark of the filtered-Lanczos path on one MI355X.
+
+Metric: seconds to all eigenpairs of A~ x = lambda B~ x in an interval, with
+every pair at ||A~x - lambda B~x||_2/|lambda| <= 1e-10 (x B~-normalised).
+A "step" is one complete solve starting from the device-resident operator:
+Lanczos bounds of B~ and Chebyshev B-solve set-up, LanBounds of B~^-1 A~,
+FindPol, ChebLanNr with Rayleigh-Ritz extraction and the residual check
+(the span of src/mod_pevsl.f90:54-162 plus src/mod_matvec.f90:75-95).
+Mesh generation, assembly and the host->HBM upload are outside the timed
+region and reported separately.
+
+Workload (default ``c1``): homogeneous solid unit ball, seeded ~12k-tet P2
+mesh, ~52k DoF, the interval holding the ~30 lowest non-rigid modes.
+
+Also reported on the same JSON line:
+  roofline      the dominant kernel (fused Chebyshev B-solve step), algorithmic
+                bytes per launch over its mean duration from HIP events sampled
+                on the solver's stream during the timed solves;
+  spmv_hbm      sparse-product and Chebyshev-step rates at the 2M-DoF layered
+                ball size (``--spmv-workload c2s``; ``none`` to skip);
+  cpu_baseline  the CPU restatement (oracle, C + OpenMP) timed on a bounded
+                sample of the same operator, scaled to a full solve.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+GATE = 1e-10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def metric_name():
+    try:
+        return json.loads((ROOT / "BASELINE.json").read_text())["metric"]
+    except Exception:
+        return "s to all eigpairs in interval @resid<=1e-10, FP32-filter vs FP64; SpMV HBM GB/s"
+
+
+def solve_once(mv, interval, bdeg, precision, probe_every):
+    from normalmodes_amd import driver, pevsl as pv
+    t0 = time.perf_counter()
+    driver.setup_bsolve(mv, bdeg=bdeg)
+    r = driver.apply_pevsl(mv, interval=interval, precision=precision, vectors=False,
+                           probe_every=probe_every)
+    return time.perf_counter() - t0, r
+
+
+def spmv_rates(name, reps=50):
+    from normalmodes_amd import driver, pevsl as pv
+    from normalmodes_amd.workloads import build_problem
+    t0 = time.perf_counter()
+    P = build_problem(name)
+    t_build = time.perf_counter() - t0
+    mv = driver.upload(P)
+    driver.setup_bsolve(mv, bdeg="auto")
+    out = dict(workload=name, n_dof=P.n, nnz_A=P.A.nnz, nnz_B=P.B.nnz, build_s=round(t_build, 2))
+    try:
+        for key, mat in (("A_bsr3", mv.matA), ("B_mass3", mv.matB)):
+            for fp32 in (False, True):
+                s, b = pv.bench_spmv(mat, reps, fp32)
+                out[f"{key}{'_fp32' if fp32 else ''}"] = dict(us=round(s * 1e6, 2), gbs=round(b / s / 1e9, 1),
+                                                               frac=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
+        for fp32 in (False, True):
+            s, b = pv.bench_chebstep(mv.chebB, reps, fp32)
+            out[f"chebstep{'_fp32' if fp32 else ''}"] = dict(us=round(s * 1e6, 2), gbs=round(b / s / 1e9, 1),
+                                                             frac=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
+    finally:
+        mv.free()
+    return out
+
+
+def cpu_baseline(P, mv, r, seconds_budget):
+    """Time the oracle (CPU restatement) on a bounded sample of the same
+    operator: degrees of the same filter, each one A~ product plus one
+    Chebyshev B-solve of the same degree; scale to the full solve by the
+    filter degrees and Lanczos steps the GPU run took."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    threads = orc.num_threads()
+    Pn = orc.Pencil(P.A, P.B, bdeg=mv.bdeg, bounds_B=(mv.blmin, mv.blmax))
+    pol = orc.findpol(r.xintv, 0.8, 0.7)
+    # calibrate on a 2-degree filter, then size the sample to the budget
+    z = orc.start_vector(P.n, 7)
+    small = orc.Pol(**{**pol.__dict__, "deg": 2, "mu": pol.mu[:3].copy()})
+    t0 = orc.wtime()
+    Pn.chebav(small, z)
+    per_deg = (orc.wtime() - t0) / 2.0
+    m = int(max(2, min(pol.deg, seconds_budget / max(per_deg, 1e-6))))
+    samp = orc.Pol(**{**pol.__dict__, "deg": m, "mu": pol.mu[:m + 1].copy()})
+    t0 = orc.wtime()
+    Pn.chebav(samp, z)
+    t_s = orc.wtime() - t0
+    per_deg = t_s / m
+    # full solve: filter degrees of ChebLanNr + one product pair per LanBounds
+    # step (already counted in the GPU's filter_applies) -> per-degree time
+    degrees = r.deg * r.filter_applies + r.lan_steps_bounds
+    est = per_deg * degrees
+    return dict(value=round(est, 2), unit="s", cores=threads, kind="port",
+                sample=(f"oracle filter of degree {m} (of {pol.deg}) with B-solve degree {mv.bdeg} on the "
+                        f"same {P.n}-DoF operator: {t_s:.1f} s; scaled by the {degrees} filter degrees "
+                        f"(ChebLanNr {r.filter_applies} applications x {r.deg} + {r.lan_steps_bounds} "
+                        f"LanBounds steps) of the GPU solve"))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed full solves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed full solves before timing")
+    ap.add_argument("--workload", default="c1", choices=["c1", "tiny"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--bdeg", default="auto", help="B-solve degree: auto | reference | <int>")
+    ap.add_argument("--probe-every", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
+    ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        log("this build runs on one GPU: collectives and sharding are not part of it "
+            "(see DESIGN.md for how the path would shard); run with --gpus 1")
+        sys.exit(2)
+    bdeg = args.bdeg if args.bdeg in ("auto", "reference") else int(args.bdeg)
+
+    from normalmodes_amd import driver, pevsl as pv
+    from normalmodes_amd.workloads import build_problem, INTERVALS
+
+    t0 = time.perf_counter()
+    P = build_problem(args.workload)
+    t_build = time.perf_counter() - t0
+    interval = INTERVALS[args.workload]
+    log(f"[bench] built {args.workload}: n={P.n} nnzA={P.A.nnz} nnzB={P.B.nnz} in {t_build:.1f}s")
+    t0 = time.perf_counter()
+    mv = driver.upload(P)
+    t_upload = time.perf_counter() - t0
+    precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
+
+    for i in range(args.warmup):
+        el, r = solve_once(mv, interval, bdeg, precision, 0)
+        log(f"[bench] warmup {i}: {el:.2f}s nev={r.lam.size} steps={r.lan_steps}")
+    times = []
+    r = None
+    for i in range(args.steps):
+        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every)
+        times.append(el)
+        log(f"[bench] step {i}: {el:.2f}s nev={r.lam.size} deg={r.deg} steps={r.lan_steps} "
+            f"max res2={np.max(r.res2) if r.res2.size else float('nan'):.2e}")
+    T = max(times)
+    gate_ok = bool(r.res2.size > 0 and np.all(r.res2 <= GATE))
+    chk = r.probe["chebstep"]
+    achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
+    line = {
+        "metric": metric_name(),
+        "value": round(T, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(T * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == pv.FP64 else "f32-filter/f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.workload}: homogeneous solid unit ball (rho=1, vp=1.73, vs=1), "
+                         f"seeded jittered-lattice Delaunay mesh of {P.mesh.n_elem} P2 tets, "
+                         f"{P.n} DoF, all eigenpairs in [{interval[0]:.6g}, {interval[1]:.6g}]"),
+            "n_dof": P.n, "nnz_A": P.A.nnz, "nnz_B": P.B.nnz, "n_tets": P.mesh.n_elem,
+            "interval": list(interval), "nev": int(r.lam.size), "filter_degree": r.deg,
+            "bsolve_degree": mv.bdeg, "lanczos_steps": r.lan_steps,
+            "lanbounds_steps": r.lan_steps_bounds, "precision": args.precision,
+            "parallelism": "1 GPU", "global_batch": 1, "seq_len": P.n,
+        },
+        "gate": {"resid_max": float(np.max(r.res2)) if r.res2.size else None,
+                 "resid_ref_max": float(np.max(r.res_ref)) if r.res_ref.size else None,
+                 "passed": gate_ok, "threshold": GATE},
+        "phases_s": {k: round(v, 4) for k, v in r.timings.items()},
+        "setup_s": {"build": round(t_build, 2), "upload": round(t_upload, 3)},
+        "roofline": {
+            "bound": "hbm", "kernel": "fused Chebyshev B-solve step (scalar mass x I3 product + update)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": chk["bytes"], "mean_launch_us": round(chk["seconds"] * 1e6, 3),
+            "samples": chk["samples"],
+            "filter_step": r.probe["filterstep"],
+        },
+    }
+    if args.spmv_workload and args.spmv_workload != "none":
+        try:
+            line["spmv_hbm"] = spmv_rates(args.spmv_workload)
+            log(f"[bench] spmv rates: {line['spmv_hbm']}")
+        except Exception as e:  # report, but do not lose the headline line
+            line["spmv_hbm"] = {"error": str(e)}
+    if args.cpu_seconds > 0:
+        try:
+            line["cpu_baseline"] = cpu_baseline(P, mv, r, args.cpu_seconds)
+            log(f"[bench] cpu baseline: {line['cpu_baseline']}")
+        except Exception as e:
+            line["cpu_baseline"] = {"error": str(e)}
+    mv.free()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

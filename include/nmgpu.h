@@ -134,6 +134,15 @@ int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *
  * rtol 1e-9 on the filtered Ritz residual). */
 int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
 
+/* Kernel timing probe (no reference counterpart): with every > 0, every
+ * `every`-th launch of the fused Chebyshev B-solve step (which = 0) and of the
+ * fused filter step (which = 1) is bracketed by HIP events on the solver's
+ * stream.  nmg_kernel_probe returns the mean sampled duration, the sample
+ * count and the algorithmic bytes of one launch (matrix + vector streams). */
+int nmg_set_kernel_probe(nmg_handle h, int every);
+int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *samples,
+                     double *bytes_per_launch);
+
 /* --- device-vector access (for tests and benchmarks) ------------------- */
 /* One application of the filter operator of the handle:
  * y = p((A B^-1 - cc)/dd) z (z-space of ChebLanNr) with pol; host vectors. */

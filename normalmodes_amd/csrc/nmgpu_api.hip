@@ -263,6 +263,53 @@ struct Cheb {
   double seconds = 0;
 };
 
+// Sampled kernel timing on the solver's stream: every `every`-th launch of a
+// kernel class is bracketed by two HIP events; the mean of the sampled
+// durations is reported with the algorithmic bytes of one launch.
+struct KernelProbe {
+  int every = 0;
+  int64_t count = 0;
+  double bytes = 0.0;
+  std::vector<hipEvent_t> e0, e1;
+  size_t used = 0;
+  bool armed(void) { return every > 0 && (count++ % every) == 0 && used < 4096; }
+  void before(hipStream_t s) {
+    if (used >= e0.size()) {
+      hipEvent_t a, b;
+      NMG_HIP(hipEventCreate(&a));
+      NMG_HIP(hipEventCreate(&b));
+      e0.push_back(a);
+      e1.push_back(b);
+    }
+    NMG_HIP(hipEventRecord(e0[used], s));
+  }
+  void after(hipStream_t s) {
+    NMG_HIP(hipEventRecord(e1[used], s));
+    used++;
+  }
+  void reset() {
+    count = 0;
+    used = 0;
+  }
+  double mean_seconds(int64_t* n) {
+    double tot = 0.0;
+    int64_t k = 0;
+    for (size_t i = 0; i < used; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0[i], e1[i]) == hipSuccess) {
+        tot += ms * 1e-3;
+        k++;
+      }
+    }
+    if (n) *n = k;
+    return k ? tot / k : 0.0;
+  }
+  ~KernelProbe() {
+    for (auto e : e0) (void)hipEventDestroy(e);
+    for (auto e : e1) (void)hipEventDestroy(e);
+  }
+};
+
 struct DVec {
   double* p = nullptr;
   int64_t n = 0;
@@ -301,6 +348,7 @@ struct Solver {
   std::vector<double> lam;
   double* X = nullptr;
   // stats
+  KernelProbe probe[2];  // 0: fused Chebyshev B-solve step, 1: fused filter step
   int lan_steps = 0;
   int64_t filter_applies = 0;
   int ncand = 0;
@@ -313,9 +361,10 @@ struct Solver {
     if (s) (void)hipStreamDestroy(s);
   }
   void free_basis() {
-    dfree(V);
+    // Z aliases V for a standard problem: free it only when it is separate
     if (Z && Z != V) dfree(Z);
     Z = nullptr;
+    dfree(V);
     cap = 0;
   }
   const DevMat& A() const {
@@ -349,7 +398,7 @@ static void ensure_basis(Solver& S, int need, int keep, int maxcap) {
 //   w_1 = 2, w_{k+1} = 1/(1 - w_k/(4 s^2)), th = (lmax+lmin)/2, s = th/de.
 // The result lands in `out`; `tmp` is a second buffer.
 static void cheb_solve(const Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
-                       bool fp32) {
+                       bool fp32, KernelProbe* probe = nullptr) {
   const int64_t n = C.mat->nrow;
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
   double* xk = (C.deg % 2 == 0) ? out : tmp;
@@ -359,7 +408,10 @@ static void cheb_solve(const Cheb& C, const double* b, double* out, double* tmp,
   double w = 2.0;
   for (int k = 1; k <= C.deg; ++k) {
     w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    const bool pr = probe && probe->armed();
+    if (pr) probe->before(s);
     k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    if (pr) probe->after(s);
     std::swap(xk, xm);
   }
 }
@@ -375,7 +427,7 @@ static void bsolve(Solver& S, const double* b, double* out, double* tmp) {
     return;
   }
   NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
-  cheb_solve(*S.bsol, b, out, tmp, S.s, S.fp32());
+  cheb_solve(*S.bsol, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
   S.bsol->ncalls++;
   S.bsol->nmatvec += S.bsol->deg;
 }
@@ -398,7 +450,10 @@ static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double
       bsolve(S, vk, b1, b2);
       in = b1;
     }
+    const bool pr = S.probe[1].armed();
+    if (pr) S.probe[1].before(S.s);
     k_filter_step(A, in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, S.fp32());
+    if (pr) S.probe[1].after(S.s);
     std::swap(vk, vkm);
   }
   S.filter_applies++;
@@ -1068,6 +1123,38 @@ int nmg_solve_stats(nmg_handle h, int* lan_steps, int64_t* filter_applies, int* 
     if (filter_applies) *filter_applies = S->filter_applies;
     if (ncand) *ncand = S->ncand;
     if (seconds) *seconds = S->seconds;
+  });
+}
+
+int nmg_set_kernel_probe(nmg_handle h, int every) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(every >= 0, NMG_EINVAL, "every must be >= 0");
+    for (auto& p : S->probe) {
+      p.every = every;
+      p.reset();
+    }
+  });
+}
+
+int nmg_kernel_probe(nmg_handle h, int which, double* mean_seconds, int64_t* samples,
+                     double* bytes_per_launch) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(which == 0 || which == 1, NMG_EINVAL, "which must be 0 (B-solve step) or 1 (filter step)");
+    NMG_HIP(hipStreamSynchronize(S->s));
+    KernelProbe& p = S->probe[which];
+    if (mean_seconds) *mean_seconds = p.mean_seconds(samples);
+    if (bytes_per_launch) {
+      const bool f32 = S->fp32();
+      const int64_t n = S->n;
+      if (which == 0) {
+        NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
+        *bytes_per_launch = S->bsol->mat->algorithmic_bytes(f32) + 2.0 * 8.0 * n;
+      } else {
+        *bytes_per_launch = S->A().algorithmic_bytes(f32) + 4.0 * 8.0 * n;
+      }
+    }
   });
 }
 

@@ -18,11 +18,14 @@ struct Error {
   std::string msg;
 };
 void set_error(int code, const std::string& msg);
-#define NMG_HIP(call)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (call);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      throw ::nmg::Error{NMG_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+#define NMG_STR2(x) #x
+#define NMG_STR(x) NMG_STR2(x)
+#define NMG_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::nmg::Error{NMG_EHIP, std::string(#call) + " at " __FILE__ ":" NMG_STR(__LINE__) \
+                                       ": " + hipGetErrorString(e_)};                     \
   } while (0)
 #define NMG_CHECK(cond, code, msg)                    \
   do {                                                \

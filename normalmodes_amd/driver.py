@@ -17,7 +17,7 @@ import numpy as np
 
 from . import pevsl as pv
 
-__all__ = ["MatvecSetup", "setupmatvec", "apply_pevsl", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
@@ -69,35 +69,52 @@ class MatvecSetup:
     timings: dict = field(default_factory=dict)
 
     def free(self):
-        pv.pevsl_chebiter_free(self.chebB)
+        if self.chebB:
+            pv.pevsl_chebiter_free(self.chebB)
         pv.csr_free(self.matA)
         pv.csr_free(self.matB)
+
+
+def upload(problem, *, porder: int | None = None) -> MatvecSetup:
+    """Copy A and B to HBM once and scale B in place (src/mod_matvec.f90:47-71,
+    252-342).  Returns a set-up without the B-solve (see ``setup_bsolve``)."""
+    porder = porder or problem.mesh.porder
+    t0 = time.perf_counter()
+    n = problem.n
+    matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
+    matA = pv.csr_create(problem.A.rowptr, problem.A.col, problem.A.val)
+    d = pv.csr_diagscale(matB, 1.0)                      # Bdiagscaling, :252-342
+    return MatvecSetup(n=n, matA=matA, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
+                       porder=porder, timings=dict(upload=time.perf_counter() - t0))
+
+
+def setup_bsolve(mv: MatvecSetup, *, bdeg="reference", seed: int = 7,
+                 bounds=(1000, 2000, 1e-12)) -> MatvecSetup:
+    """Lanczos bounds of B~ and its Chebyshev solve (src/mod_matvec.f90:75-95)."""
+    t0 = time.perf_counter()
+    if mv.chebB:
+        pv.pevsl_chebiter_free(mv.chebB)
+        mv.chebB = 0
+    hB = pv.pevsl_start(0)                               # :75
+    try:
+        pv.pevsl_setprobsizes(hB, mv.n, mv.n, -1)
+        pv.pevsl_set_seed(hB, seed)
+        pv.pevsl_setamv(hB, mv.matB)                     # :80
+        mv.blmin, mv.blmax = pv.pevsl_lanbounds(hB, *bounds)   # :83-85
+    finally:
+        pv.pevsl_finish(hB)                              # :95
+    mv.bdeg = chebiter_degree(mv.porder, mv.blmin, mv.blmax, bdeg)
+    mv.chebB = pv.pevsl_setup_chebiter(mv.blmin, mv.blmax, mv.bdeg, mv.matB)   # :93
+    mv.timings["setup_B"] = time.perf_counter() - t0
+    return mv
 
 
 def setupmatvec(problem, *, porder: int | None = None, bdeg="reference", seed: int = 7,
                 bounds=(1000, 2000, 1e-12)) -> MatvecSetup:
     """Upload A and B, scale B, bound its spectrum and set up its Chebyshev
     solve (src/mod_matvec.f90:24-95, 225-247)."""
-    porder = porder or problem.mesh.porder
-    t = {}
-    t0 = time.perf_counter()
-    n = problem.n
-    matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
-    matA = pv.csr_create(problem.A.rowptr, problem.A.col, problem.A.val)
-    t["upload"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    d = pv.csr_diagscale(matB, 1.0)                      # Bdiagscaling, :252-342
-    hB = pv.pevsl_start(0)                               # :75
-    pv.pevsl_setprobsizes(hB, n, n, -1)
-    pv.pevsl_set_seed(hB, seed)
-    pv.pevsl_setamv(hB, matB)                            # :80
-    blmin, blmax = pv.pevsl_lanbounds(hB, *bounds)       # :83-85
-    pv.pevsl_finish(hB)
-    deg = chebiter_degree(porder, blmin, blmax, bdeg)
-    chebB = pv.pevsl_setup_chebiter(blmin, blmax, deg, matB)   # :93
-    t["setup_B"] = time.perf_counter() - t0
-    return MatvecSetup(n=n, matA=matA, matB=matB, d=d, blmin=blmin, blmax=blmax, bdeg=deg,
-                       chebB=chebB, porder=porder, timings=t)
+    mv = upload(problem, porder=porder)
+    return setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds)
 
 
 @dataclass
@@ -115,6 +132,8 @@ class EigResult:
     filter_applies: int
     ncand: int
     timings: dict
+    lan_steps_bounds: int = 0
+    probe: dict = field(default_factory=dict)
 
     @property
     def freq_mhz(self):
@@ -128,7 +147,8 @@ class EigResult:
 def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float | None = None,
                 interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
-                rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True) -> EigResult:
+                rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
+                probe_every: int = 0) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
@@ -147,10 +167,13 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         if precision != pv.FP64:
             pv.pevsl_set_precision(h, precision)
         pv.set_lanczos_controls(h, rtol=rtol)
+        if probe_every > 0:
+            pv.set_kernel_probe(h, probe_every)
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         lmin, lmax = pv.pevsl_lanbounds(h, *lanbounds)        # :84
         t["lanbounds"] = time.perf_counter() - t0
+        lb_steps = pv.solve_stats(h)["lan_steps"]
         if a < 1.0e-10:                                       # :98-100
             a = lmin
         xintv = np.array([a, b, lmin, lmax])
@@ -165,6 +188,11 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         pv.pevsl_cheblannr(h, xintv, maxit, tol, pol)         # :122
         t["cheblannr"] = time.perf_counter() - t0
         st = pv.solve_stats(h)
+        probe = {}
+        if probe_every > 0:
+            for key, which in (("chebstep", 0), ("filterstep", 1)):
+                sec, ns, by = pv.kernel_probe(h, which)
+                probe[key] = dict(seconds=sec, samples=ns, bytes=by)
         t0 = time.perf_counter()
         if residual_check:
             r1, r2 = pv.residuals(h)                          # :137-162
@@ -182,4 +210,4 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
     return EigResult(lam=lam[order], X=None if X is None else X[order], res_ref=r1[order],
                      res2=r2[order], lmin=lmin, lmax=lmax, xintv=xintv, deg=pinfo.deg,
                      bar=pinfo.bar, lan_steps=st["lan_steps"], filter_applies=st["filter_applies"],
-                     ncand=st["ncand"], timings=t)
+                     ncand=st["ncand"], timings=t, lan_steps_bounds=lb_steps, probe=probe)

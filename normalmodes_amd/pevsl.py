@@ -21,7 +21,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
-           "bench_spmv", "bench_chebstep", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -269,6 +269,20 @@ def solve_stats(h: Handle):
     _check(_lib().nmg_solve_stats(_i64(h), ctypes.byref(ls), ctypes.byref(fa), ctypes.byref(nc),
                                   ctypes.byref(sec)))
     return dict(lan_steps=ls.value, filter_applies=fa.value, ncand=nc.value, seconds=sec.value)
+
+
+def set_kernel_probe(h: Handle, every: int):
+    _check(_lib().nmg_set_kernel_probe(_i64(h), ctypes.c_int(every)))
+
+
+def kernel_probe(h: Handle, which: int):
+    """which: 0 fused Chebyshev B-solve step, 1 fused filter step.
+    Returns (mean seconds per launch, samples, algorithmic bytes per launch)."""
+    s, b = _d(), _d()
+    n = _i64()
+    _check(_lib().nmg_kernel_probe(_i64(h), ctypes.c_int(which), ctypes.byref(s), ctypes.byref(n),
+                                   ctypes.byref(b)))
+    return s.value, n.value, b.value
 
 
 def filter_apply(h: Handle, pol: Handle, z):

@@ -1,0 +1,38 @@
+"""Named synthetic workloads (seeded meshes and models).
+
+c1   homogeneous solid unit ball (rho=1, vp=1.73, vs=1), ~12k P2 tets, ~52k DoF
+c2s  three-layer radial ball, all layers solid (rho 13->4, vp 11->8,
+     vs 3.5->6.5 linear in r), ~450k P2 tets, ~2M DoF
+tiny homogeneous ball, ~300 P2 tets (tests and smoke)
+Mesh seed 1019, Lanczos start-vector seed 7.
+"""
+from __future__ import annotations
+
+from .assemble import assemble
+from .mesh import ball_mesh
+from .models import homogeneous, three_layer_solid
+
+MESH_SEED = 1019
+LANCZOS_SEED = 7
+
+SPECS = {
+    "tiny": dict(n_tets=300, model="homogeneous"),
+    "c1": dict(n_tets=12000, model="homogeneous"),
+    "c2s": dict(n_tets=450000, model="three_layer_solid"),
+}
+
+
+def build_problem(name: str, porder: int = 2, seed: int = MESH_SEED):
+    spec = SPECS[name]
+    mesh = ball_mesh(spec["n_tets"], porder=porder, seed=seed)
+    model = homogeneous() if spec["model"] == "homogeneous" else three_layer_solid()
+    return assemble(mesh, model, name=name)
+
+# Eigenvalue intervals (lambda = omega^2 in the model's units).
+#   c1: a = lambda_7 / 2 (above the six rigid-body zero modes) and b at the
+#   midpoint of the first gap after the 36th eigenvalue, from the shift-invert
+#   eigenvalues of the c1 pencil (tests/golden/c1_eigs.json): 33 eigenpairs.
+INTERVALS = {
+    "tiny": (1.0, 12.0),
+    "c1": (3.1423366861823694, 24.499667699964398),
+}

@@ -244,3 +244,26 @@ def test_bad_handle_and_state():
             pv.pevsl_set_geneig(h)               # no B
     finally:
         pv.pevsl_finish(h)
+
+
+def test_c1_matches_shift_invert_golden():
+    """The C1 workload (~52k DoF) on the device against the committed
+    shift-invert eigenvalues: same count in the interval, 1e-12 relative."""
+    import json
+    from pathlib import Path
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    from normalmodes_amd.workloads import build_problem, INTERVALS
+    g = json.loads((Path(__file__).resolve().parent / "golden" / "c1_eigs.json").read_text())
+    w = np.array(g["eigenvalues"])
+    a, b = INTERVALS["c1"]
+    ref = w[(w >= a) & (w <= b)]
+    P = build_problem("c1")
+    assert P.n == g["n_dof"] and P.A.nnz == g["nnz_A"]
+    mv = setupmatvec(P, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(a, b), vectors=False)
+    finally:
+        mv.free()
+    assert r.lam.size == ref.size == g["n_in_interval"]
+    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
+    assert np.all(r.res2 <= 1e-10)

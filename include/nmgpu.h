@@ -134,6 +134,21 @@ int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *
  * rtol 1e-9 on the filtered Ritz residual). */
 int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
 
+/* Residual target of nmg_cheblannr (default 5e-11; 0 = stop on the
+ * reference's trace test alone).  Once the trace test and the filtered Ritz
+ * residuals pass, the pairs are extracted and the iteration stops when every
+ * ||A~x - lam B~x||_2/|lam| (x B~-normalised) is at or below the target, or
+ * when that residual stops improving (its floor is set by the B-solve
+ * accuracy). */
+int nmg_set_residual_target(nmg_handle h, double target);
+
+/* Lanczos block width (no reference counterpart; pEVSL's ChebLanNr is the
+ * block width 1, the default).  With block > 1, nmg_cheblannr runs the block
+ * filtered Lanczos: the same filter, B-solve and extraction, applied to
+ * `block` vectors at once so that each sparse product streams the matrices
+ * once per block (1, 2, 4, 8 or 16). */
+int nmg_set_block_size(nmg_handle h, int block);
+
 /* Kernel timing probe (no reference counterpart): with every > 0, every
  * `every`-th launch of the fused Chebyshev B-solve step (which = 0) and of the
  * fused filter step (which = 1) is bracketed by HIP events on the solver's
@@ -149,11 +164,34 @@ int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *sam
 int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double *z, double *y);
 /* Operators of the handle on host vectors: which = 0 A, 1 B, 2 B-solve. */
 int nmg_apply(nmg_handle h, int which, const double *x, double *y);
+/* Residual target of nmg_cheblannr (default 5e-11; 0 = stop on the
+ * reference's trace test alone).  Once the trace test and the filtered Ritz
+ * residuals pass, the pairs are extracted and the iteration stops when every
+ * ||A~x - lam B~x||_2/|lam| (x B~-normalised) is at or below the target, or
+ * when that residual stops improving (its floor is set by the B-solve
+ * accuracy). */
+int nmg_set_residual_target(nmg_handle h, double target);
+
+/* Lanczos block width (no reference counterpart; pEVSL's ChebLanNr is the
+ * block width 1, the default).  With block > 1, nmg_cheblannr runs the block
+ * filtered Lanczos: the same filter, B-solve and extraction, applied to
+ * `block` vectors at once so that each sparse product streams the matrices
+ * once per block (1, 2, 4, 8 or 16). */
+int nmg_set_block_size(nmg_handle h, int block);
+
+/* The same on K host columns at once (X, Y: K blocks of n, column c at c*n)
+ * through the k-column kernels of the block filter; K in {1, 2, 4, 8, 16}. */
+int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);
 /* Kernel timing probe: runs `reps` products of mat on device-resident
  * vectors and returns the mean kernel time in seconds (HIP events on the
  * library's own stream) and the algorithmic bytes of one product. */
 int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
 /* Same for one fused Chebyshev B-solve step (one product + vector update). */
+int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double *seconds, double *bytes);
+/* Kernel mapping of a matrix (tuning): lanes per row of the single-column
+ * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
+ * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */
+int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave);
 int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 
 #ifdef __cplusplus

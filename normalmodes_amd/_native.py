@@ -61,9 +61,30 @@ def build_gpu(verbose=False):
         print("built", GPU_SO)
 
 
+FORTRAN_DIR = PKG / "fortran"
+FDRIVE = FORTRAN_DIR / "nmgpu_drive"
+
+
+def build_fortran(verbose=False):
+    """Fortran host (ISO_C_BINDING module + driver) with amdflang, linked
+    against libnmgpu.so.  Skipped (returns False) when no Fortran compiler."""
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    if not Path(fc).exists():
+        return False
+    bdir = FORTRAN_DIR / "_build"
+    bdir.mkdir(exist_ok=True)
+    _run([fc, "-O2", "-J", str(bdir), "-c", str(FORTRAN_DIR / "nmgpu_f90.f90"), "-o", str(bdir / "nmgpu_f90.o")])
+    _run([fc, "-O2", "-J", str(bdir), "-I", str(bdir), str(FORTRAN_DIR / "nmgpu_drive.f90"),
+          str(bdir / "nmgpu_f90.o"), f"-L{PKG}", "-lnmgpu", "-Wl,-rpath,$ORIGIN/..", "-o", str(FDRIVE)])
+    if verbose:
+        print("built", FDRIVE)
+    return True
+
+
 def build_all(verbose=False):
     build_asm(verbose)
     build_gpu(verbose)
+    build_fortran(verbose)
 
 
 _asm = None

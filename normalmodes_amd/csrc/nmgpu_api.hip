@@ -120,13 +120,15 @@ void DevMat::release() {
 }
 DevMat::~DevMat() { release(); }
 
-double DevMat::algorithmic_bytes(bool fp32) const {
+double DevMat::matrix_bytes(bool fp32) const {
   const double sv = fp32 ? 4.0 : 8.0;
-  if (layout == L_CSR)
-    return (double)nnz * (sv + 4.0) + (double)(nrow + 1) * 8.0 + (double)ncol * 8.0 + (double)nrow * 8.0;
+  if (layout == L_CSR) return (double)nnz * (sv + 4.0) + (double)(nrow + 1) * 8.0;
   const double vals = layout == L_BSR3 ? 9.0 : 1.0;
-  return (double)nbnz * (vals * sv + 4.0) + (double)(nb + 1) * 8.0 + (double)ncol * 8.0 +
-         (double)nrow * 8.0;
+  return (double)nbnz * (vals * sv + 4.0) + (double)(nb + 1) * 8.0;
+}
+
+double DevMat::algorithmic_bytes(bool fp32) const {
+  return matrix_bytes(fp32) + (double)ncol * 8.0 + (double)nrow * 8.0;
 }
 
 // Layout detection on the host copy.
@@ -336,6 +338,9 @@ struct Solver {
   uint64_t seed = 7;
   int ntest = 30, cycle = 20;
   double rtol = 1e-9;
+  double resid_target = 5e-11;  // stop when every extracted pair meets this (0 = trace test only)
+  int block = 1;     // Lanczos block width (1 = the reference's single-vector ChebLanNr)
+  DVec bw[6];        // interleaved n*block work blocks
   // work vectors
   DVec w[8];
   DVec red;  // reductions
@@ -578,72 +583,19 @@ static void do_lanbounds(Solver& S, int mlan, int maxit, double tol, double* lmi
   S.seconds = now() - t0;
 }
 
-static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, const PolyFilter& P) {
-  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
-  NMG_CHECK(xintv[1] > xintv[0], NMG_EINVAL, "empty interval");
-  const double t0 = now();
+// Ritz vectors of the candidates (theta >= bar) from the Lanczos basis V(:, 0:kdim)
+// and the eigenvectors dC (kdim x kdim, device) of the projected matrix, then
+// Rayleigh-Ritz of (A~, B~) on their span; keeps the pairs inside xintv[0..1].
+static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vector<double>& theta, double bar,
+                         const double* xintv) {
   const int64_t n = S.n;
-  maxit = (int)std::min<int64_t>(maxit, n);
-  setup_work(S);
-  S.ab.ensure(2 * (int64_t)(maxit + 2));
-  S.free_basis();
-  ensure_basis(S, std::min(maxit + 1, 64), 0, maxit + 1);
-  S.filter_applies = 0;
-  DVec hw;
-  lanczos_start(S);
-  std::vector<double> al, be, w, sv;
-  double tr0 = 0.0;
-  int kdim = 0;
-  for (int k = 0; k < maxit; ++k) {
-    ensure_basis(S, k + 2, k + 1, maxit + 1);
-    hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
-    lanczos_step(S, k, 1, &P, hw.p);
-    kdim = k + 1;
-    if (kdim < S.ntest || ((kdim - S.ntest) % S.cycle) != 0) continue;
-    fetch_tridiag(S, kdim, al, be);
-    if (be[kdim - 1] == 0.0) break;
-    if (tridiag_eigvals(kdim, al.data(), be.data(), w) != NMG_OK)
-      throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
-    double tr1 = 0.0;
-    int cnt = 0;
-    for (int i = 0; i < kdim; ++i)
-      if (w[i] >= P.bar) {
-        tr1 += w[i];
-        cnt++;
-      }
-    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
-    tr0 = tr1;
-    if (!conv) continue;
-    double rmax = 0.0;
-    for (int i = 0; i < kdim; ++i)
-      if (w[i] >= P.bar) {
-        tridiag_eigvec(kdim, al.data(), be.data(), w[i], sv);
-        rmax = std::max(rmax, std::fabs(be[kdim - 1] * sv[kdim - 1]));
-      }
-    if (rmax <= S.rtol) break;
-  }
-  S.lan_steps = kdim;
-  fetch_tridiag(S, kdim, al, be);
-
-  // ---- extraction: eigenvectors of T on the device (rocSOLVER divide & conquer)
-  DVec dD, dE, dC;
-  dD.ensure(kdim);
-  dE.ensure(kdim);
-  dC.ensure((int64_t)kdim * kdim);
-  NMG_HIP(hipMemcpyAsync(dD.p, al.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
-  NMG_HIP(hipMemcpyAsync(dE.p, be.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
-  rocblas_int* dinfo = dmalloc<rocblas_int>(1);
-  std::unique_ptr<rocblas_int, void (*)(rocblas_int*)> info_guard(dinfo, [](rocblas_int* p) { (void)hipFree(p); });
-  NMG_LIB(rocsolver_dstedc(S.blas, rocblas_evect_tridiagonal, kdim, dD.p, dE.p, dC.p, kdim, dinfo), "rocsolver_dstedc failed");
-  std::vector<double> theta(kdim);
+  DVec dinfo_v;
+  dinfo_v.ensure(1);
+  rocblas_int* dinfo = (rocblas_int*)dinfo_v.p;
   rocblas_int info = 0;
-  NMG_HIP(hipMemcpyAsync(theta.data(), dD.p, sizeof(double) * kdim, hipMemcpyDeviceToHost, S.s));
-  NMG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, S.s));
-  NMG_HIP(hipStreamSynchronize(S.s));
-  NMG_CHECK(info == 0, NMG_ENUM, "tridiagonal eigensolver failed");
   std::vector<int> sel;
   for (int i = 0; i < kdim; ++i)
-    if (theta[i] >= P.bar) sel.push_back(i);
+    if (theta[i] >= bar) sel.push_back(i);
   const int nc = (int)sel.size();
   S.ncand = nc;
   S.nev = 0;
@@ -653,7 +605,7 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
     DVec Ssel, Y, AY, BY, GA, GB, mu, ee;
     Ssel.ensure((int64_t)kdim * nc);
     for (int j = 0; j < nc; ++j)
-      NMG_HIP(hipMemcpyAsync(Ssel.p + (size_t)j * kdim, dC.p + (size_t)sel[j] * kdim,
+      NMG_HIP(hipMemcpyAsync(Ssel.p + (size_t)j * kdim, dC + (size_t)sel[j] * kdim,
                              sizeof(double) * kdim, hipMemcpyDeviceToDevice, S.s));
     Y.ensure(n * nc);
     const double one = 1.0, zero = 0.0;
@@ -719,6 +671,446 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
     }
     NMG_HIP(hipStreamSynchronize(S.s));
   }
+}
+
+
+// eigenpairs of the Lanczos tridiagonal on the device (rocSOLVER divide & conquer)
+static void tridiag_eig_device(Solver& S, int kdim, const std::vector<double>& al, const std::vector<double>& be,
+                               DVec& dC, std::vector<double>& theta) {
+  DVec dD, dE, dinfo_v;
+  dD.ensure(kdim);
+  dE.ensure(kdim);
+  dC.ensure((int64_t)kdim * kdim);
+  dinfo_v.ensure(1);
+  NMG_HIP(hipMemcpyAsync(dD.p, al.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
+  NMG_HIP(hipMemcpyAsync(dE.p, be.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, S.s));
+  NMG_LIB(rocsolver_dstedc(S.blas, rocblas_evect_tridiagonal, kdim, dD.p, dE.p, dC.p, kdim,
+                           (rocblas_int*)dinfo_v.p), "rocsolver_dstedc failed");
+  theta.resize(kdim);
+  rocblas_int info = 0;
+  NMG_HIP(hipMemcpyAsync(theta.data(), dD.p, sizeof(double) * kdim, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipMemcpyAsync(&info, dinfo_v.p, sizeof(info), hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  NMG_CHECK(info == 0, NMG_ENUM, "tridiagonal eigensolver failed");
+}
+
+// Residual-based stop: after the trace test has passed, extract the pairs and
+// accept when max ||A~x - lam B~x||/|lam| <= resid_target.  The residual has
+// a floor set by the B-solve accuracy; when it has not improved by 10% over
+// the last max(60, kdim/4) Lanczos vectors the iteration stops with the best
+// pairs found.  Returns true when the iteration should stop.
+static void do_residuals(Solver& S, double* res_ref, double* res2);
+struct ResidualTracker {
+  double best = 1e300;
+  int best_k = 0;
+};
+static bool residual_stop(Solver& S, int kdim, const double* dC, const std::vector<double>& theta, double bar,
+                          const double* xintv, ResidualTracker& rt) {
+  if (S.resid_target <= 0.0) return true;
+  extract_ritz(S, kdim, dC, theta, bar, xintv);
+  if (S.nev == 0) return false;
+  std::vector<double> r2(S.nev);
+  do_residuals(S, nullptr, r2.data());
+  double mx = 0.0;
+  for (double v : r2) mx = std::max(mx, std::isfinite(v) ? v : 1e300);
+  if (mx <= S.resid_target) return true;
+  if (mx < 0.9 * rt.best) {
+    rt.best = mx;
+    rt.best_k = kdim;
+    return false;
+  }
+  return kdim - rt.best_k >= std::max(60, kdim / 4);
+}
+
+static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, const PolyFilter& P) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  NMG_CHECK(xintv[1] > xintv[0], NMG_EINVAL, "empty interval");
+  const double t0 = now();
+  const int64_t n = S.n;
+  maxit = (int)std::min<int64_t>(maxit, n);
+  setup_work(S);
+  S.ab.ensure(2 * (int64_t)(maxit + 2));
+  S.free_basis();
+  ensure_basis(S, std::min(maxit + 1, 64), 0, maxit + 1);
+  S.filter_applies = 0;
+  DVec hw;
+  lanczos_start(S);
+  std::vector<double> al, be, w, sv;
+  double tr0 = 0.0;
+  ResidualTracker rt;
+  int kdim = 0;
+  bool stopped_with_result = false;
+  for (int k = 0; k < maxit; ++k) {
+    ensure_basis(S, k + 2, k + 1, maxit + 1);
+    hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
+    lanczos_step(S, k, 1, &P, hw.p);
+    kdim = k + 1;
+    if (kdim < S.ntest || ((kdim - S.ntest) % S.cycle) != 0) continue;
+    fetch_tridiag(S, kdim, al, be);
+    if (be[kdim - 1] == 0.0) break;
+    if (tridiag_eigvals(kdim, al.data(), be.data(), w) != NMG_OK)
+      throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
+    double tr1 = 0.0;
+    int cnt = 0;
+    for (int i = 0; i < kdim; ++i)
+      if (w[i] >= P.bar) {
+        tr1 += w[i];
+        cnt++;
+      }
+    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
+    tr0 = tr1;
+    if (!conv) continue;
+    double rmax = 0.0;
+    for (int i = 0; i < kdim; ++i)
+      if (w[i] >= P.bar) {
+        tridiag_eigvec(kdim, al.data(), be.data(), w[i], sv);
+        rmax = std::max(rmax, std::fabs(be[kdim - 1] * sv[kdim - 1]));
+      }
+    if (rmax > S.rtol) continue;
+    if (S.resid_target <= 0.0) break;
+    {
+      std::vector<double> th;
+      DVec dC;
+      tridiag_eig_device(S, kdim, al, be, dC, th);
+      if (residual_stop(S, kdim, dC.p, th, P.bar, xintv, rt)) {
+        stopped_with_result = true;
+        break;
+      }
+    }
+  }
+  S.lan_steps = kdim;
+  if (stopped_with_result) {
+    S.free_basis();
+    S.seconds = now() - t0;
+    return;
+  }
+  fetch_tridiag(S, kdim, al, be);
+
+  // ---- extraction: eigenvectors of T on the device (rocSOLVER divide & conquer)
+  DVec dC;
+  std::vector<double> theta;
+  tridiag_eig_device(S, kdim, al, be, dC, theta);
+  extract_ritz(S, kdim, dC.p, theta, P.bar, xintv);
+  S.free_basis();
+  S.seconds = now() - t0;
+}
+
+// ---------------------------------------------------------------------------
+// Block filtered Lanczos (block width K > 1): the same filter and extraction
+// as ChebLanNr, applied to K vectors at once, so that every sparse product of
+// the filter and of the Chebyshev B-solve reads the matrices once for K
+// columns.  Recurrence in the z-space with B-inner products:
+//   W = p(A B^-1) Z_j - Z_{j-1} R_j^T - Z_j A_j,  A_j = V_j^T p(A B^-1) Z_j,
+//   full re-orthogonalisation W -= Z (V^T W) twice (tall-skinny GEMMs),
+//   V' = B^-1 W,  V'^T W = R^T R (Cholesky),  V_{j+1} = V' R^-1, Z_{j+1} = W R^-1,
+// giving the block tridiagonal T with diagonal blocks A_j and sub-diagonal
+// blocks R_{j+1}.  Candidates are the Ritz values of T >= bar; Ritz residual
+// of candidate s: ||R_{j+1} s_last||.
+
+// y (interleaved n*K) ~ B~^{-1} b, all K columns at once
+static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
+  const Cheb& C = *S.bsol;
+  const int64_t nk = S.n * K;
+  const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
+  double* xk = (C.deg % 2 == 0) ? out : tmp;
+  double* xm = (C.deg % 2 == 0) ? tmp : out;
+  k_scale_copy(nk, 1.0 / th, b, xk, S.s);
+  k_fill(nk, 0.0, xm, S.s);
+  double w = 2.0;
+  KernelProbe* probe = &S.probe[0];
+  for (int k = 1; k <= C.deg; ++k) {
+    w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    const bool pr = probe->armed();
+    if (pr) probe->before(S.s);
+    k_block_cheb_step(K, *C.mat, b, xk, xm, w, 1.0 / th, S.s, S.fp32());
+    if (pr) probe->after(S.s);
+    std::swap(xk, xm);
+  }
+  S.bsol->ncalls += K;
+  S.bsol->nmatvec += (int64_t)C.deg * K;
+}
+
+// y = p((A B^-1 - cc)/dd) z for K interleaved columns; mu_dev: (deg+1)*K
+static void block_filter(Solver& S, int K, const PolyFilter& P, const double* mu_dev, const double* z,
+                         double* y) {
+  const int64_t nk = S.n * K;
+  for (int i = 0; i < 4; ++i) S.bw[i].ensure(nk);
+  double* vk = S.bw[0].p;
+  double* vkm = S.bw[1].p;
+  double* b1 = S.bw[2].p;
+  double* b2 = S.bw[3].p;
+  k_copy(nk, z, vk, S.s);
+  k_block_colscale(S.n, K, mu_dev, vk, y, S.s);
+  const DevMat& A = S.A();
+  for (int k = 1; k <= P.deg; ++k) {
+    const double t = (k == 1 ? 1.0 : 2.0) / P.dd;
+    const double* in = vk;
+    if (S.geneig) {
+      block_bsolve(S, K, vk, b1, b2);
+      in = b1;
+    }
+    const bool pr = S.probe[1].armed();
+    if (pr) S.probe[1].before(S.s);
+    k_block_filter_step(K, A, in, vk, vkm, y, mu_dev + (size_t)k * K, t, P.cc, k == 1, S.s, S.fp32());
+    if (pr) S.probe[1].after(S.s);
+    std::swap(vk, vkm);
+  }
+  S.filter_applies += K;
+}
+
+// G = R^T R for a small SPD G (row-major K x K); returns R^-1 (row-major, upper)
+static void chol_inv_upper(int K, std::vector<double>& G, std::vector<double>& R, std::vector<double>& Ri) {
+  R.assign((size_t)K * K, 0.0);
+  for (int j = 0; j < K; ++j) {
+    double s = G[(size_t)j * K + j];
+    for (int k = 0; k < j; ++k) s -= R[(size_t)k * K + j] * R[(size_t)k * K + j];
+    NMG_CHECK(s > 0.0 && std::isfinite(s), NMG_ENUM,
+              "block Lanczos: the new block lost rank (B-Gram matrix not positive definite)");
+    const double rjj = std::sqrt(s);
+    R[(size_t)j * K + j] = rjj;
+    for (int i = j + 1; i < K; ++i) {
+      double t = G[(size_t)j * K + i];
+      for (int k = 0; k < j; ++k) t -= R[(size_t)k * K + j] * R[(size_t)k * K + i];
+      R[(size_t)j * K + i] = t / rjj;
+    }
+  }
+  Ri.assign((size_t)K * K, 0.0);
+  for (int c = 0; c < K; ++c) {  // solve R x = e_c (back substitution)
+    for (int i = K - 1; i >= 0; --i) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = i + 1; k < K; ++k) t -= R[(size_t)i * K + k] * Ri[(size_t)k * K + c];
+      Ri[(size_t)i * K + c] = t / R[(size_t)i * K + i];
+    }
+  }
+}
+
+// small K x K products on the device: C = op(A) op(B) with rocBLAS (column-major)
+static void gemm(Solver& S, rocblas_operation ta, rocblas_operation tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc, const char* what) {
+  NMG_LIB(rocblas_dgemm(S.blas, ta, tb, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
+                        (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C, (rocblas_int)ldc),
+          what);
+}
+
+// B-orthonormalise the column block V (n x K col-major) with Z = B V given in Zc:
+// V <- V R^-1, Z <- Z R^-1; returns R (row-major upper) on the host
+static void block_borth(Solver& S, int K, double* V, double* Z, std::vector<double>& R) {
+  const int64_t n = S.n;
+  DVec G, Rinv, T;
+  G.ensure((int64_t)K * K);
+  Rinv.ensure((int64_t)K * K);
+  T.ensure(n * K);
+  gemm(S, rocblas_operation_transpose, rocblas_operation_none, K, K, n, 1.0, V, n, Z, n, 0.0, G.p, K,
+       "block Gram V^T Z");
+  std::vector<double> g((size_t)K * K), Ri;
+  NMG_HIP(hipMemcpyAsync(g.data(), G.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  for (int i = 0; i < K; ++i)
+    for (int j = i + 1; j < K; ++j) g[(size_t)i * K + j] = g[(size_t)j * K + i] = 0.5 * (g[(size_t)i * K + j] + g[(size_t)j * K + i]);
+  chol_inv_upper(K, g, R, Ri);
+  // column-major copy of R^-1 (row-major Ri -> transpose)
+  std::vector<double> ric((size_t)K * K);
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) ric[(size_t)j * K + i] = Ri[(size_t)i * K + j];
+  NMG_HIP(hipMemcpyAsync(Rinv.p, ric.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, S.s));
+  gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, 1.0, V, n, Rinv.p, K, 0.0, T.p, n,
+       "V R^-1");
+  NMG_HIP(hipMemcpyAsync(V, T.p, sizeof(double) * n * K, hipMemcpyDeviceToDevice, S.s));
+  if (Z != V) {
+    gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, 1.0, Z, n, Rinv.p, K, 0.0, T.p, n,
+         "Z R^-1");
+    NMG_HIP(hipMemcpyAsync(Z, T.p, sizeof(double) * n * K, hipMemcpyDeviceToDevice, S.s));
+  }
+  NMG_HIP(hipStreamSynchronize(S.s));
+}
+
+// eigenvalues/vectors of the dense symmetric T (m x m, column-major) on the device
+static void dense_eig(Solver& S, int m, const std::vector<double>& T, DVec& dT, std::vector<double>& w) {
+  dT.ensure((int64_t)m * m);
+  DVec dw, de, dinfo_v;
+  dw.ensure(m);
+  de.ensure(m);
+  dinfo_v.ensure(1);
+  NMG_HIP(hipMemcpyAsync(dT.p, T.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, S.s));
+  NMG_LIB(rocsolver_dsyevd(S.blas, rocblas_evect_original, rocblas_fill_upper, m, dT.p, m, dw.p, de.p,
+                           (rocblas_int*)dinfo_v.p),
+          "rocsolver_dsyevd failed");
+  w.resize(m);
+  rocblas_int info = 0;
+  NMG_HIP(hipMemcpyAsync(w.data(), dw.p, sizeof(double) * m, hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipMemcpyAsync(&info, dinfo_v.p, sizeof(info), hipMemcpyDeviceToHost, S.s));
+  NMG_HIP(hipStreamSynchronize(S.s));
+  NMG_CHECK(info == 0, NMG_ENUM, "dense eigensolver failed");
+}
+
+static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double tol, const PolyFilter& P) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  NMG_CHECK(xintv[1] > xintv[0], NMG_EINVAL, "empty interval");
+  const int K = S.block;
+  const double t0 = now();
+  const int64_t n = S.n;
+  maxit = (int)std::min<int64_t>(maxit, n);
+  const int maxblk = std::max(2, maxit / K);
+  const int maxcols = (maxblk + 1) * K;
+  setup_work(S);
+  for (int i = 0; i < 6; ++i) S.bw[i].ensure(n * K);
+  S.free_basis();
+  ensure_basis(S, std::min(maxcols, std::max(2 * K, 64)), 0, maxcols);
+  S.filter_applies = 0;
+  // per-column filter coefficients (all columns share the interval here)
+  DVec mu_dev;
+  mu_dev.ensure((int64_t)(P.deg + 1) * K);
+  {
+    std::vector<double> m((size_t)(P.deg + 1) * K);
+    for (int k = 0; k <= P.deg; ++k)
+      for (int c = 0; c < K; ++c) m[(size_t)k * K + c] = P.mu[k];
+    NMG_HIP(hipMemcpyAsync(mu_dev.p, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, S.s));
+  }
+  // start block: column c is the start vector of seed S.seed over indices c*n + i
+  {
+    std::vector<double> v0(n * K);
+    for (int c = 0; c < K; ++c) {
+      std::vector<double> col(n);
+      start_vector(n, S.seed + 0x9E3779B97F4A7C15ULL * (uint64_t)c, col.data());
+      std::memcpy(v0.data() + (size_t)c * n, col.data(), sizeof(double) * n);
+    }
+    NMG_HIP(hipMemcpyAsync(S.V, v0.data(), sizeof(double) * n * K, hipMemcpyHostToDevice, S.s));
+    for (int c = 0; c < K; ++c) apply_B(S, S.V + (size_t)c * n, S.Z + (size_t)c * n);
+    std::vector<double> R0;
+    block_borth(S, K, S.V, S.Z, R0);
+  }
+  std::vector<std::vector<double>> Ablk, Rblk;  // row-major K x K
+  DVec Hd, Ad, Rd, dT;
+  std::vector<double> Tm, w;
+  double tr0 = 0.0;
+  ResidualTracker rt;
+  int nblk = 0;
+  bool converged = false, stopped_with_result = false;
+  for (int j = 0; j < maxblk && !converged; ++j) {
+    ensure_basis(S, (j + 2) * K, (j + 1) * K, maxcols);
+    double* Vj = S.V + (size_t)j * K * n;
+    double* Zj = S.Z + (size_t)j * K * n;
+    double* W = S.Z + (size_t)(j + 1) * K * n;
+    double* Vn = S.V + (size_t)(j + 1) * K * n;
+    // filter in the interleaved layout
+    k_to_interleaved(n, K, Zj, n, S.bw[4].p, S.s);
+    block_filter(S, K, P, mu_dev.p, S.bw[4].p, S.bw[5].p);
+    k_from_interleaved(n, K, S.bw[5].p, W, n, S.s);
+    // three-term part
+    Ad.ensure((int64_t)K * K);
+    Rd.ensure((int64_t)K * K);
+    if (j > 0) {
+      std::vector<double> rc((size_t)K * K);  // column-major R_j
+      for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) rc[(size_t)c * K + r] = Rblk[j - 1][(size_t)r * K + c];
+      NMG_HIP(hipMemcpyAsync(Rd.p, rc.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, S.s));
+      gemm(S, rocblas_operation_none, rocblas_operation_transpose, n, K, K, -1.0,
+           S.Z + (size_t)(j - 1) * K * n, n, Rd.p, K, 1.0, W, n, "W -= Z_{j-1} R_j^T");
+    }
+    gemm(S, rocblas_operation_transpose, rocblas_operation_none, K, K, n, 1.0, Vj, n, W, n, 0.0, Ad.p, K,
+         "A_j = V_j^T W");
+    std::vector<double> a((size_t)K * K);
+    NMG_HIP(hipMemcpyAsync(a.data(), Ad.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
+    gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, -1.0, Zj, n, Ad.p, K, 1.0, W, n,
+         "W -= Z_j A_j");
+    // full re-orthogonalisation, classical Gram-Schmidt twice, as GEMMs
+    const int64_t m = (int64_t)(j + 1) * K;
+    Hd.ensure(m * K);
+    for (int pass = 0; pass < 2; ++pass) {
+      gemm(S, rocblas_operation_transpose, rocblas_operation_none, m, K, n, 1.0, S.V, n, W, n, 0.0, Hd.p, m,
+           "H = V^T W");
+      gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, m, -1.0, S.Z, n, Hd.p, m, 1.0, W, n,
+           "W -= Z H");
+    }
+    // V_{j+1} = B^-1 W, B-orthonormalise
+    if (S.geneig) {
+      k_to_interleaved(n, K, W, n, S.bw[4].p, S.s);
+      block_bsolve(S, K, S.bw[4].p, S.bw[5].p, S.bw[2].p);
+      k_from_interleaved(n, K, S.bw[5].p, Vn, n, S.s);
+    }
+    NMG_HIP(hipStreamSynchronize(S.s));
+    for (int r = 0; r < K; ++r)  // column-major -> row-major, symmetrised
+      for (int c = r; c < K; ++c) {
+        const double v = 0.5 * (a[(size_t)c * K + r] + a[(size_t)r * K + c]);
+        a[(size_t)c * K + r] = a[(size_t)r * K + c] = v;
+      }
+    for (double v : a) NMG_CHECK(std::isfinite(v), NMG_ENUM, "block Lanczos breakdown: non-finite coefficient");
+    std::vector<double> R;
+    block_borth(S, K, Vn, W, R);
+    Ablk.push_back(a);
+    Rblk.push_back(R);
+    nblk = j + 1;
+    const int mm = nblk * K;
+    if (mm < S.ntest) continue;
+    // dense block tridiagonal T (column-major) and its eigenpairs
+    Tm.assign((size_t)mm * mm, 0.0);
+    for (int b = 0; b < nblk; ++b)
+      for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) {
+          Tm[(size_t)(b * K + c) * mm + b * K + r] = Ablk[b][(size_t)r * K + c];
+          if (b + 1 < nblk) {
+            const double v = Rblk[b][(size_t)r * K + c];  // T[(b+1)K + r, bK + c] = R_{b+1}[r][c]
+            Tm[(size_t)(b * K + c) * mm + (b + 1) * K + r] = v;
+            Tm[(size_t)((b + 1) * K + r) * mm + b * K + c] = v;
+          }
+        }
+    dense_eig(S, mm, Tm, dT, w);
+    double tr1 = 0.0;
+    int cnt = 0;
+    for (int i = 0; i < mm; ++i)
+      if (w[i] >= P.bar) {
+        tr1 += w[i];
+        cnt++;
+      }
+    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
+    tr0 = tr1;
+    if (!conv) continue;
+    // Ritz residuals ||R_{j+1} s_last|| of the candidates
+    std::vector<double> last((size_t)K * mm);
+    for (int i = 0; i < mm; ++i)
+      NMG_HIP(hipMemcpyAsync(last.data() + (size_t)i * K, dT.p + (size_t)i * mm + (mm - K), sizeof(double) * K,
+                             hipMemcpyDeviceToHost, S.s));
+    NMG_HIP(hipStreamSynchronize(S.s));
+    double rmax = 0.0;
+    for (int i = 0; i < mm; ++i) {
+      if (w[i] < P.bar) continue;
+      double nrm = 0.0;
+      for (int r = 0; r < K; ++r) {
+        double t = 0.0;
+        for (int c = r; c < K; ++c) t += R[(size_t)r * K + c] * last[(size_t)i * K + c];
+        nrm += t * t;
+      }
+      rmax = std::max(rmax, std::sqrt(nrm));
+    }
+    if (rmax > S.rtol) continue;
+    if (residual_stop(S, mm, dT.p, w, P.bar, xintv, rt)) {
+      converged = true;
+      stopped_with_result = S.resid_target > 0.0;
+    }
+  }
+  const int kdim = nblk * K;
+  S.lan_steps = kdim;
+  if (stopped_with_result) {
+    S.free_basis();
+    S.seconds = now() - t0;
+    return;
+  }
+  if (w.size() != (size_t)kdim) {
+    Tm.assign((size_t)kdim * kdim, 0.0);
+    for (int b = 0; b < nblk; ++b)
+      for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) {
+          Tm[(size_t)(b * K + c) * kdim + b * K + r] = Ablk[b][(size_t)r * K + c];
+          if (b + 1 < nblk) {
+            const double v = Rblk[b][(size_t)r * K + c];
+            Tm[(size_t)(b * K + c) * kdim + (b + 1) * K + r] = v;
+            Tm[(size_t)((b + 1) * K + r) * kdim + b * K + c] = v;
+          }
+        }
+    dense_eig(S, kdim, Tm, dT, w);
+  }
+  extract_ritz(S, kdim, dT.p, w, P.bar, xintv);
   S.free_basis();
   S.seconds = now() - t0;
 }
@@ -1085,7 +1477,8 @@ int nmg_cheblannr(nmg_handle h, const double* xintv, int maxit, double tol, nmg_
     NMG_CHECK(xintv != nullptr, NMG_EINVAL, "null interval");
     NMG_CHECK(maxit >= 2 && tol > 0.0, NMG_EINVAL, "bad maxit/tol");
     (void)S->A();
-    do_cheblannr(*S, xintv, maxit, tol, *P);
+    if (S->block > 1) do_cheblannr_block(*S, xintv, maxit, tol, *P);
+    else do_cheblannr(*S, xintv, maxit, tol, *P);
   });
 }
 
@@ -1126,6 +1519,23 @@ int nmg_solve_stats(nmg_handle h, int* lan_steps, int64_t* filter_applies, int* 
   });
 }
 
+int nmg_set_residual_target(nmg_handle h, double target) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(target >= 0.0 && std::isfinite(target), NMG_EINVAL, "target must be >= 0");
+    S->resid_target = target;
+  });
+}
+
+int nmg_set_block_size(nmg_handle h, int block) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(block == 1 || block == 2 || block == 4 || block == 8 || block == 16, NMG_EINVAL,
+              "block size must be 1, 2, 4, 8 or 16");
+    S->block = block;
+  });
+}
+
 int nmg_set_kernel_probe(nmg_handle h, int every) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
@@ -1146,13 +1556,15 @@ int nmg_kernel_probe(nmg_handle h, int which, double* mean_seconds, int64_t* sam
     KernelProbe& p = S->probe[which];
     if (mean_seconds) *mean_seconds = p.mean_seconds(samples);
     if (bytes_per_launch) {
+      // matrix once per launch; vector streams per column: Chebyshev step
+      // x, b, xm read, xm write (4); filter step x, vk, vkm r/w, y r/w (6)
       const bool f32 = S->fp32();
-      const int64_t n = S->n;
+      const double nk = (double)S->n * (double)S->block;
       if (which == 0) {
         NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
-        *bytes_per_launch = S->bsol->mat->algorithmic_bytes(f32) + 2.0 * 8.0 * n;
+        *bytes_per_launch = S->bsol->mat->matrix_bytes(f32) + 4.0 * 8.0 * nk;
       } else {
-        *bytes_per_launch = S->A().algorithmic_bytes(f32) + 4.0 * 8.0 * n;
+        *bytes_per_launch = S->A().matrix_bytes(f32) + 6.0 * 8.0 * nk;
       }
     }
   });
@@ -1195,6 +1607,90 @@ int nmg_apply(nmg_handle h, int which, const double* x, double* y) {
     }
     NMG_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_apply_block(nmg_handle h, int which, int K, const double* X, double* Y) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(X && Y, NMG_EINVAL, "null block");
+    NMG_CHECK(which >= 0 && which <= 2, NMG_EINVAL, "which must be 0 (A), 1 (B) or 2 (B-solve)");
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    const int64_t n = S->n;
+    setup_work(*S);
+    for (int i = 0; i < 6; ++i) S->bw[i].ensure(n * K);
+    DVec cm;
+    cm.ensure(n * K);
+    NMG_HIP(hipMemcpyAsync(cm.p, X, sizeof(double) * n * K, hipMemcpyHostToDevice, S->s));
+    k_to_interleaved(n, K, cm.p, n, S->bw[4].p, S->s);
+    if (which == 0) k_block_spmv(K, S->A(), S->bw[4].p, S->bw[5].p, S->s, S->fp32());
+    else if (which == 1) {
+      NMG_CHECK(S->bmat != nullptr, NMG_ESTATE, "no B set");
+      k_block_spmv(K, *S->bmat, S->bw[4].p, S->bw[5].p, S->s);
+    } else {
+      NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
+      block_bsolve(*S, K, S->bw[4].p, S->bw[5].p, S->bw[2].p);
+    }
+    k_from_interleaved(n, K, S->bw[5].p, cm.p, n, S->s);
+    NMG_HIP(hipMemcpyAsync(Y, cm.p, sizeof(double) * n * K, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(group == 0 || group == 8 || group == 16 || group == 32 || group == 64, NMG_EINVAL,
+              "group must be 0, 8, 16, 32 or 64");
+    M->group = group;
+    if (K > 0) {
+      int idx = -1;
+      for (int i = 0; i < 5; ++i)
+        if ((1 << i) == K) idx = i;
+      NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+      NMG_CHECK(rows_per_wave == 0 || rows_per_wave == 1 || rows_per_wave == 2 || rows_per_wave == 4 ||
+                    rows_per_wave == 8,
+                NMG_EINVAL, "rows per wavefront must be 0, 1, 2, 4 or 8");
+      NMG_CHECK(K * std::max(rows_per_wave, 1) <= 64, NMG_EINVAL, "K x rows per wavefront must be <= 64");
+      M->block_map[idx] = rows_per_wave;
+    }
+  });
+}
+
+int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double* seconds, double* bytes) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    DevMat* M = C->mat;
+    hipStream_t s = default_stream();
+    if (fp32 && !(M->val32 || M->bval32)) make_fp32(*M, s);
+    const int64_t nk = M->nrow * K;
+    DVec b, xk, xm;
+    b.ensure(nk);
+    xk.ensure(nk);
+    xm.ensure(nk);
+    k_fill(nk, 1.0, b.p, s);
+    k_fill(nk, 0.5, xk.p, s);
+    k_fill(nk, 0.0, xm.p, s);
+    const double th = 0.5 * (C->lmax + C->lmin);
+    k_block_cheb_step(K, *M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
+    hipEvent_t e0, e1;
+    NMG_HIP(hipEventCreate(&e0));
+    NMG_HIP(hipEventCreate(&e1));
+    NMG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+      k_block_cheb_step(K, *M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
+      std::swap(xk.p, xm.p);
+    }
+    NMG_HIP(hipEventRecord(e1, s));
+    NMG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = ms * 1e-3 / reps;
+    if (bytes) *bytes = M->matrix_bytes(fp32 != 0) + 4.0 * 8.0 * (double)nk;
   });
 }
 

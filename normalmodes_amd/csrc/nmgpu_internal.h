@@ -52,8 +52,13 @@ struct DevMat {
   // BSR3: 9 value planes bval[p*nbnz + k], p = 3*r + c;  MASS3: 1 plane
   double* bval = nullptr;
   float* bval32 = nullptr;
+  // kernel mapping knobs (0 = default): lanes per row of the single-column
+  // products; rows per wavefront of the k-column products for K = 1,2,4,8,16
+  int group = 0;
+  int block_map[5] = {0, 0, 0, 0, 0};
   ~DevMat();
   void release();
+  double matrix_bytes(bool fp32) const;       // values + indices + row offsets
   double algorithmic_bytes(bool fp32) const;  // one product: matrix + x + y
 };
 
@@ -98,6 +103,16 @@ void k_gemv_n_sub(int64_t n, int k, const double* Z, int64_t ld, const double* h
 // residual pieces for one vector: out[0] = sum r^2, out[1] = x^T bx, r = ax - lam bx
 void k_resid(int64_t n, const double* ax, const double* bx, const double* x, double lam,
              double* out, double* work, hipStream_t s);
+// k-column products on column-interleaved block vectors (entry (i, c) at i*K + c),
+// K in {1, 2, 4, 8, 16}; mu: K per-column filter coefficients of this degree
+void k_block_spmv(int K, const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32 = false);
+void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
+                       double ith, hipStream_t s, bool fp32 = false);
+void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
+                         const double* mu, double t, double cc, bool first, hipStream_t s, bool fp32 = false);
+void k_block_colscale(int64_t n, int K, const double* mu, const double* x, double* y, hipStream_t s);
+void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s);
+void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s);
 size_t dot_work_size();
 size_t gemv_work_size(int64_t n, int k);
 

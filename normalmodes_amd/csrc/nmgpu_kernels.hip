@@ -42,18 +42,25 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // --------------------------------------------------------------------------
 // epilogues
+// Epilogues load their own operands first (load) and combine them with the
+// product once it is reduced (store), so that the loads overlap the product.
 struct EpiStore {
   double* y;
-  __device__ __forceinline__ void operator()(int64_t i, double v) const { y[i] = v; }
+  struct Pre {};
+  __device__ __forceinline__ Pre load(int64_t) const { return Pre{}; }
+  __device__ __forceinline__ void store(int64_t i, double v, const Pre&) const { y[i] = v; }
 };
 struct EpiCheb {
   const double* b;
   const double* xk;
   double* xm;
   double w, ith;
-  __device__ __forceinline__ void operator()(int64_t i, double v) const {
-    const double xki = xk[i], xmi = xm[i];
-    xm[i] = xmi + w * (xki - xmi + (b[i] - v) * ith);
+  struct Pre {
+    double b, xk, xm;
+  };
+  __device__ __forceinline__ Pre load(int64_t i) const { return Pre{b[i], xk[i], xm[i]}; }
+  __device__ __forceinline__ void store(int64_t i, double v, const Pre& p) const {
+    xm[i] = p.xm + w * (p.xk - p.xm + (p.b - v) * ith);
   }
 };
 struct EpiFilter {
@@ -62,11 +69,14 @@ struct EpiFilter {
   double* y;
   double t, cc, mu;
   int first;
-  __device__ __forceinline__ void operator()(int64_t i, double v) const {
-    double vn = t * (v - cc * vk[i]);
-    if (!first) vn -= vkm[i];
+  struct Pre {
+    double vk, vkm, y;
+  };
+  __device__ __forceinline__ Pre load(int64_t i) const { return Pre{vk[i], first ? 0.0 : vkm[i], y[i]}; }
+  __device__ __forceinline__ void store(int64_t i, double v, const Pre& p) const {
+    const double vn = t * (v - cc * p.vk) - p.vkm;
     vkm[i] = vn;
-    y[i] += mu * vn;
+    y[i] = p.y + mu * vn;
   }
 };
 
@@ -80,13 +90,16 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int64_t nrow, const int64_t* 
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
+  typename Epi::Pre pre{};
+  const bool wr = row < nrow && lane == 0;
+  if (wr) pre = epi.load(row);
   double s = 0.0;
   if (row < nrow) {
     const int64_t e = rp[row + 1];
     for (int64_t k = rp[row] + lane; k < e; k += G) s += (double)va[k] * x[ci[k]];
   }
   s = group_sum<G>(s);
-  if (row < nrow && lane == 0) epi(row, s);
+  if (wr) epi.store(row, s, pre);
 }
 
 template <int G, class VT, class Epi>
@@ -98,6 +111,9 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
+  typename Epi::Pre pre{};
+  const bool wr = row < nb && lane < 3;
+  if (wr) pre = epi.load(3 * row + lane);
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
   if (row < nb) {
     const int64_t e = rp[row + 1];
@@ -114,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
   y0 = group_sum<G>(y0);
   y1 = group_sum<G>(y1);
   y2 = group_sum<G>(y2);
-  if (row < nb && lane < 3) epi(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+  if (wr) epi.store(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2), pre);
 }
 
 template <int G, class VT, class Epi>
@@ -125,6 +141,9 @@ __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* 
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
+  typename Epi::Pre pre{};
+  const bool wr = row < nb && lane < 3;
+  if (wr) pre = epi.load(3 * row + lane);
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
   if (row < nb) {
     const int64_t e = rp[row + 1];
@@ -139,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* 
   y0 = group_sum<G>(y0);
   y1 = group_sum<G>(y1);
   y2 = group_sum<G>(y2);
-  if (row < nb && lane < 3) epi(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+  if (wr) epi.store(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2), pre);
 }
 
 static int pick_group(double avg) {
@@ -174,7 +193,8 @@ template <class VT, class Epi>
 static void launch_rows(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
   double avg = M.layout == L_CSR ? (double)M.nnz / (double)(M.nrow ? M.nrow : 1)
                                  : (double)M.nbnz / (double)(M.nb ? M.nb : 1);
-  switch (pick_group(avg)) {
+  const int g = M.group > 0 ? M.group : pick_group(avg);
+  switch (g) {
     case 64: launch_G<64>(M, v, x, epi, s); break;
     case 32: launch_G<32>(M, v, x, epi, s); break;
     case 16: launch_G<16>(M, v, x, epi, s); break;
@@ -524,6 +544,335 @@ void make_fp32(DevMat& M, hipStream_t s) {
     if (!M.bval32) NMG_HIP(hipMalloc(&M.bval32, sizeof(float) * (nv > 0 ? nv : 1)));
     hipLaunchKernelGGL(to_f32_k, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, M.bval, M.bval32);
   }
+  NMG_HIP(hipGetLastError());
+}
+
+}  // namespace nmg
+
+// ==========================================================================
+// k-column ("block") products for the block filter and the batched slices.
+// Block vectors are node-major and column-interleaved: entry (dof i, column c)
+// at i*K + c, so the three components of a neighbour node for all K columns
+// are 3K contiguous doubles.  One wavefront per node row: lane = e*K + c with
+// E = 64/K entry slots; the slots stride the row, the K columns share the
+// matrix entry; a butterfly over the slots leaves every lane with the sums,
+// and slot p < 3 writes component p of its column.
+namespace nmg {
+
+template <int K>
+struct BEpiStore {
+  double* y;
+  struct Pre {};
+  __device__ __forceinline__ Pre load(int64_t, int) const { return Pre{}; }
+  __device__ __forceinline__ void store(int64_t i, int c, double v, const Pre&) const { y[i * K + c] = v; }
+};
+template <int K>
+struct BEpiCheb {
+  const double* b;
+  const double* xk;
+  double* xm;
+  double w, ith;
+  struct Pre {
+    double b, xk, xm;
+  };
+  __device__ __forceinline__ Pre load(int64_t i, int c) const {
+    const int64_t o = i * K + c;
+    return Pre{b[o], xk[o], xm[o]};
+  }
+  __device__ __forceinline__ void store(int64_t i, int c, double v, const Pre& p) const {
+    xm[i * K + c] = p.xm + w * (p.xk - p.xm + (p.b - v) * ith);
+  }
+};
+template <int K>
+struct BEpiFilter {
+  const double* vk;
+  double* vkm;
+  double* y;
+  const double* mu;  // K coefficients of this degree (one per column)
+  double t, cc;
+  int first;
+  struct Pre {
+    double vk, vkm, y, mu;
+  };
+  __device__ __forceinline__ Pre load(int64_t i, int c) const {
+    const int64_t o = i * K + c;
+    return Pre{vk[o], first ? 0.0 : vkm[o], y[o], mu[c]};
+  }
+  __device__ __forceinline__ void store(int64_t i, int c, double v, const Pre& p) const {
+    const int64_t o = i * K + c;
+    const double vn = t * (v - cc * p.vk) - p.vkm;
+    vkm[o] = vn;
+    y[o] = p.y + p.mu * vn;
+  }
+};
+
+template <int K>
+__device__ __forceinline__ double slot_sum(double v) {
+#pragma unroll
+  for (int off = K; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Lane layout of the k-column kernels: a wavefront holds R node rows; each
+// row has K columns x E entry slots (R*K*E = 64).  Lane l: row r = l/(K*E),
+// column c = l%K, slot e = (l/K)%E.  Each lane covers U entries per pass:
+// all U matrix loads are issued, then all gathers, then the FMAs, so a pass
+// costs two dependent memory round trips.  Entries past the row end use
+// column c of node 0 with weight 0.  A butterfly over the slots leaves every
+// slot with the sums; slot e writes components e, e+E, ... (< 3).
+template <int K, int R>
+struct BlockMap {
+  static_assert(K * R <= 64 && (64 % (K * R)) == 0, "bad block map");
+  static constexpr int E = 64 / (K * R);
+  static constexpr int U = (32 / E) > 8 ? 8 : ((32 / E) > 0 ? (32 / E) : 1);
+};
+
+template <int K, int E>
+__device__ __forceinline__ double slots_sum(double v) {
+#pragma unroll
+  for (int off = K; off < K * E; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int K, int R, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int64_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci,
+                                                           const VT* __restrict__ mv,
+                                                           const double* __restrict__ x, Epi epi) {
+  constexpr int E = BlockMap<K, R>::E, U = BlockMap<K, R>::U;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int c = lane % K, e = (lane / K) % E;
+  const bool live = row < nb;
+  typename Epi::Pre pre[(3 + E - 1) / E];
+#pragma unroll
+  for (int q = 0; q < (3 + E - 1) / E; ++q)
+    if (live && e + q * E < 3) pre[q] = epi.load(3 * row + e + q * E, c);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  if (live) {
+    const int64_t beg = rp[row], end = rp[row + 1];
+    for (int64_t k0 = beg + e; k0 < end; k0 += (int64_t)U * E) {
+      int64_t base[U];
+      double m[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t k = k0 + (int64_t)u * E;
+        const bool ok = k < end;
+        base[u] = ok ? 3 * (int64_t)ci[k] * K + c : (int64_t)c;
+        m[u] = ok ? (double)mv[k] : 0.0;
+      }
+      double xa[U], xb[U], xc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xa[u] = x[base[u]];
+        xb[u] = x[base[u] + K];
+        xc[u] = x[base[u] + 2 * K];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        y0 += m[u] * xa[u];
+        y1 += m[u] * xb[u];
+        y2 += m[u] * xc[u];
+      }
+    }
+  }
+  y0 = slots_sum<K, E>(y0);
+  y1 = slots_sum<K, E>(y1);
+  y2 = slots_sum<K, E>(y2);
+#pragma unroll
+  for (int q = 0; q < (3 + E - 1) / E; ++q) {
+    const int p = e + q * E;
+    if (live && p < 3) epi.store(3 * row + p, c, p == 0 ? y0 : (p == 1 ? y1 : y2), pre[q]);
+  }
+}
+
+template <int K, int R, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nbnz,
+                                                          const int64_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ ci,
+                                                          const VT* __restrict__ bv,
+                                                          const double* __restrict__ x, Epi epi) {
+  constexpr int E = BlockMap<K, R>::E;
+  constexpr int U = BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U;  // 9 values per entry
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int c = lane % K, e = (lane / K) % E;
+  const bool live = row < nb;
+  typename Epi::Pre pre[(3 + E - 1) / E];
+#pragma unroll
+  for (int q = 0; q < (3 + E - 1) / E; ++q)
+    if (live && e + q * E < 3) pre[q] = epi.load(3 * row + e + q * E, c);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  if (live) {
+    const int64_t beg = rp[row], end = rp[row + 1];
+    for (int64_t k0 = beg + e; k0 < end; k0 += (int64_t)U * E) {
+      int64_t base[U], kk[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t k = k0 + (int64_t)u * E;
+        ok[u] = k < end;
+        kk[u] = ok[u] ? k : beg;
+        base[u] = ok[u] ? 3 * (int64_t)ci[k] * K + c : (int64_t)c;
+      }
+      double a[U][9], xa[U], xb[U], xc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)bv[p * nbnz + kk[u]] : 0.0;
+        xa[u] = x[base[u]];
+        xb[u] = x[base[u] + K];
+        xc[u] = x[base[u] + 2 * K];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        y0 += a[u][0] * xa[u] + a[u][1] * xb[u] + a[u][2] * xc[u];
+        y1 += a[u][3] * xa[u] + a[u][4] * xb[u] + a[u][5] * xc[u];
+        y2 += a[u][6] * xa[u] + a[u][7] * xb[u] + a[u][8] * xc[u];
+      }
+    }
+  }
+  y0 = slots_sum<K, E>(y0);
+  y1 = slots_sum<K, E>(y1);
+  y2 = slots_sum<K, E>(y2);
+#pragma unroll
+  for (int q = 0; q < (3 + E - 1) / E; ++q) {
+    const int p = e + q * E;
+    if (live && p < 3) epi.store(3 * row + p, c, p == 0 ? y0 : (p == 1 ? y1 : y2), pre[q]);
+  }
+}
+
+// generic CSR with K columns: one wavefront per scalar row, lanes (e, c)
+template <int K, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int64_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ci,
+                                                         const VT* __restrict__ va,
+                                                         const double* __restrict__ x, Epi epi) {
+  constexpr int E = 64 / K;
+  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K, e = lane / K;
+  const bool wr = row < nrow && e == 0;
+  typename Epi::Pre pre{};
+  if (wr) pre = epi.load(row, c);
+  double s = 0.0;
+  if (row < nrow) {
+    const int64_t end = rp[row + 1];
+    for (int64_t k = rp[row] + e; k < end; k += E) s += (double)va[k] * x[(int64_t)ci[k] * K + c];
+  }
+  s = slot_sum<K>(s);
+  if (wr) epi.store(row, c, s, pre);
+}
+
+// rows per wavefront of the k-column kernels: chosen per (matrix, K) by the
+// autotuner (tune_block_map); 0 means "not tuned" and uses R = 1
+static int block_rows(const DevMat& M, int K) {
+  for (int i = 0; i < 5; ++i)
+    if ((1 << i) == K) return M.block_map[i] > 0 ? M.block_map[i] : 1;
+  return 1;
+}
+
+template <int K, int R, class VT, class Epi>
+static void launch_block_R(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  const int64_t waves = (M.nb + R - 1) / R;
+  const int64_t grid = (waves * 64 + kBlock - 1) / kBlock;
+  if (grid <= 0) return;
+  if (M.layout == L_BSR3)
+    hipLaunchKernelGGL((bsr3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+                       M.nbnz, M.browptr, M.bcol, v, x, epi);
+  else
+    hipLaunchKernelGGL((mass3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+                       M.browptr, M.bcol, v, x, epi);
+}
+
+template <int K, class VT, class Epi>
+static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  if (M.layout == L_CSR) {
+    const int64_t grid = (M.nrow * 64 + kBlock - 1) / kBlock;
+    if (grid > 0)
+      hipLaunchKernelGGL((csr_block_rows<K, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
+                         M.rowptr, M.col, v, x, epi);
+  } else {
+    const int R = block_rows(M, K);
+    if (R == 1) launch_block_R<K, 1>(M, v, x, epi, s);
+    else if (R == 2 && K * 2 <= 64) launch_block_R<K, (K * 2 <= 64 ? 2 : 1)>(M, v, x, epi, s);
+    else if (R == 4 && K * 4 <= 64) launch_block_R<K, (K * 4 <= 64 ? 4 : 1)>(M, v, x, epi, s);
+    else if (R == 8 && K * 8 <= 64) launch_block_R<K, (K * 8 <= 64 ? 8 : 1)>(M, v, x, epi, s);
+    else launch_block_R<K, 1>(M, v, x, epi, s);
+  }
+  NMG_HIP(hipGetLastError());
+}
+
+template <int K, class Epi>
+static void dispatch_block(const DevMat& M, const double* x, Epi epi, hipStream_t s, bool fp32) {
+  if (M.layout == L_CSR) {
+    if (fp32 && M.val32) launch_block<K>(M, M.val32, x, epi, s);
+    else launch_block<K>(M, M.val, x, epi, s);
+  } else {
+    if (fp32 && M.bval32) launch_block<K>(M, M.bval32, x, epi, s);
+    else launch_block<K>(M, M.bval, x, epi, s);
+  }
+}
+
+#define NMG_BLOCK_SWITCH(K_, CALL)                                              \
+  switch (K_) {                                                                 \
+    case 1: { constexpr int KK = 1; CALL; break; }                              \
+    case 2: { constexpr int KK = 2; CALL; break; }                              \
+    case 4: { constexpr int KK = 4; CALL; break; }                              \
+    case 8: { constexpr int KK = 8; CALL; break; }                              \
+    case 16: { constexpr int KK = 16; CALL; break; }                            \
+    default: throw Error{NMG_EINVAL, "block width must be 1, 2, 4, 8 or 16"};  \
+  }
+
+void k_block_spmv(int K, const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32) {
+  NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(M, x, BEpiStore<KK>{y}, s, fp32)));
+}
+
+void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
+                       double ith, hipStream_t s, bool fp32) {
+  NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(M, xk, BEpiCheb<KK>{b, xk, xm, w, ith}, s, fp32)));
+}
+
+void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
+                         const double* mu, double t, double cc, bool first, hipStream_t s, bool fp32) {
+  NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(A, x, BEpiFilter<KK>{vk, vkm, y, mu, t, cc, first ? 1 : 0}, s, fp32)));
+}
+
+// y[i*K + c] = mu[c] * x[i*K + c]
+__global__ void block_colscale_k(int64_t n, int K, const double* __restrict__ mu, const double* __restrict__ x,
+                                 double* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock)
+    y[t] = mu[t % K] * x[t];
+}
+void k_block_colscale(int64_t n, int K, const double* mu, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(block_colscale_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, mu, x, y);
+  NMG_HIP(hipGetLastError());
+}
+
+// column-major (n x K, ld) <-> interleaved (i*K + c)
+__global__ void to_interleaved_k(int64_t n, int K, const double* __restrict__ cm, int64_t ld,
+                                 double* __restrict__ il) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = t / K;
+    const int c = (int)(t % K);
+    il[t] = cm[(int64_t)c * ld + i];
+  }
+}
+__global__ void from_interleaved_k(int64_t n, int K, const double* __restrict__ il, double* __restrict__ cm,
+                                   int64_t ld) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = t / K;
+    const int c = (int)(t % K);
+    cm[(int64_t)c * ld + i] = il[t];
+  }
+}
+
+void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s) {
+  hipLaunchKernelGGL(to_interleaved_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, cm, ld, il);
+  NMG_HIP(hipGetLastError());
+}
+void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s) {
+  hipLaunchKernelGGL(from_interleaved_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, il, cm, ld);
   NMG_HIP(hipGetLastError());
 }
 

@@ -148,7 +148,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
-                probe_every: int = 0) -> EigResult:
+                probe_every: int = 0, block: int = 1, resid_target: float | None = None) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
@@ -169,6 +169,10 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         pv.set_lanczos_controls(h, rtol=rtol)
         if probe_every > 0:
             pv.set_kernel_probe(h, probe_every)
+        if block > 1:
+            pv.set_block_size(h, block)
+        if resid_target is not None:
+            pv.set_residual_target(h, resid_target)
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         lmin, lmax = pv.pevsl_lanbounds(h, *lanbounds)        # :84

@@ -1,0 +1,200 @@
+!-----------------------------------------------------------------------
+! nmgpu_f90: ISO_C_BINDING interfaces to the nmgpu C-ABI (include/nmgpu.h)
+! for a Fortran host.  Each interface replaces one pEVSL F90 call that
+! NormalModes makes (src/mod_pevsl.f90, src/mod_matvec.f90); scalars are
+! passed by value, arrays by reference, handles are integer(c_int64_t)
+! (the reference keeps pEVSL handles in integer*8,
+! src/mod_cg_datatype.f90:92-96).  Every function returns 0 or a negative
+! status; nmg_errmsg() returns the message of the last failure.
+!-----------------------------------------------------------------------
+module nmgpu_f90
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: NMG_FP64 = 0, NMG_FP32_FILTER = 1
+
+  interface
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+
+    function nmg_last_error() bind(C, name="nmg_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+
+    ! pEVSL_Start_F90 (src/mod_pevsl.f90:54)
+    integer(c_int) function nmg_start(comm, h) bind(C, name="nmg_start")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: comm
+      integer(c_int64_t), intent(out) :: h
+    end function
+    ! PEVSL_FINISH_F90 (src/mod_pevsl.f90:220)
+    integer(c_int) function nmg_finish(h) bind(C, name="nmg_finish")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+    end function
+    ! PEVSL_SETPROBSIZES_F90 (src/mod_pevsl.f90:57)
+    integer(c_int) function nmg_setprobsizes(h, n_glob, n_loc, nfirst) bind(C, name="nmg_setprobsizes")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h, n_glob, n_loc, nfirst
+    end function
+    ! PEVSL_PARCSRCREATE_F90 (src/mod_matvec.f90:69-71)
+    integer(c_int) function nmg_parcsr_create(nrow_g, ncol_g, row_starts, col_starts, ia, ja, a, comm, mat) &
+        bind(C, name="nmg_parcsr_create")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: nrow_g, ncol_g, comm
+      integer(c_int), intent(in) :: row_starts(*), col_starts(*), ia(*), ja(*)
+      real(c_double), intent(in) :: a(*)
+      integer(c_int64_t), intent(out) :: mat
+    end function
+    integer(c_int) function nmg_csr_free(mat) bind(C, name="nmg_csr_free")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: mat
+    end function
+    ! pevsl_parcsrmatvec_f90 (src/mod_matvec.f90:453)
+    integer(c_int) function nmg_parcsr_matvec(x, y, mat) bind(C, name="nmg_parcsr_matvec")
+      import :: c_int, c_int64_t, c_double
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int64_t), value :: mat
+    end function
+    ! Bdiagscaling (src/mod_matvec.f90:252-342); sign -1 for Apdiagscaling
+    integer(c_int) function nmg_csr_diagscale(mat, sign, d) bind(C, name="nmg_csr_diagscale")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: mat
+      real(c_double), value :: sign
+      real(c_double), intent(out) :: d(*)
+    end function
+    ! PEVSL_SETAMV_F90(h, sparseAV, mymatvec) (src/mod_pevsl.f90:77)
+    integer(c_int) function nmg_set_amv_dad(h, matA, d) bind(C, name="nmg_set_amv_dad")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h, matA
+      real(c_double), intent(in) :: d(*)
+    end function
+    ! PEVSL_SETAMV_F90(h, sparseBV, mymatvec) for the bounds of B (src/mod_matvec.f90:80)
+    integer(c_int) function nmg_set_amv(h, mat) bind(C, name="nmg_set_amv")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h, mat
+    end function
+    ! PEVSL_SETBMV_F90 (src/mod_pevsl.f90:69)
+    integer(c_int) function nmg_set_bmv(h, matB) bind(C, name="nmg_set_bmv")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h, matB
+    end function
+    ! PEVSL_SETBSOL_CHEBITER_F90 (src/mod_pevsl.f90:72-73)
+    integer(c_int) function nmg_set_bsol_chebiter(h, chebtype, cheb) bind(C, name="nmg_set_bsol_chebiter")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h, cheb
+      integer(c_int), value :: chebtype
+    end function
+    ! PEVSL_SET_GENEIG_F90 (src/mod_pevsl.f90:82)
+    integer(c_int) function nmg_set_geneig(h) bind(C, name="nmg_set_geneig")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+    end function
+    integer(c_int) function nmg_set_precision(h, mode) bind(C, name="nmg_set_precision")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+      integer(c_int), value :: mode
+    end function
+    ! pEVSL_SETUP_CHEBITER_F90 (src/mod_matvec.f90:93)
+    integer(c_int) function nmg_setup_chebiter(lmin, lmax, deg, mat, cheb) bind(C, name="nmg_setup_chebiter")
+      import :: c_int, c_int64_t, c_double
+      real(c_double), value :: lmin, lmax
+      integer(c_int), value :: deg
+      integer(c_int64_t), value :: mat
+      integer(c_int64_t), intent(out) :: cheb
+    end function
+    ! pevsl_chebiter_f90 (src/mod_matvec.f90:480)
+    integer(c_int) function nmg_chebiter(chebtype, b, x, cheb) bind(C, name="nmg_chebiter")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: chebtype
+      real(c_double), intent(in) :: b(*)
+      real(c_double), intent(out) :: x(*)
+      integer(c_int64_t), value :: cheb
+    end function
+    integer(c_int) function nmg_chebiter_free(cheb) bind(C, name="nmg_chebiter_free")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: cheb
+    end function
+    ! pEVSL_LANBOUNDS_F90 (src/mod_pevsl.f90:84)
+    integer(c_int) function nmg_lanbounds(h, mlan, maxit, tol, lmin, lmax) bind(C, name="nmg_lanbounds")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      integer(c_int), value :: mlan, maxit
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: lmin, lmax
+    end function
+    ! pEVSL_FINDPOL_F90 (src/mod_pevsl.f90:115)
+    integer(c_int) function nmg_findpol(xintv, thr_int, thr_ext, pol) bind(C, name="nmg_findpol")
+      import :: c_int, c_int64_t, c_double
+      real(c_double), intent(in) :: xintv(4)
+      real(c_double), value :: thr_int, thr_ext
+      integer(c_int64_t), intent(out) :: pol
+    end function
+    ! pEVSL_FREEPOL_F90 (src/mod_pevsl.f90:215)
+    integer(c_int) function nmg_freepol(pol) bind(C, name="nmg_freepol")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: pol
+    end function
+    ! pEVSL_CHEBLANNR_F90 (src/mod_pevsl.f90:122)
+    integer(c_int) function nmg_cheblannr(h, xintv, maxit, tol, pol) bind(C, name="nmg_cheblannr")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h, pol
+      real(c_double), intent(in) :: xintv(4)
+      integer(c_int), value :: maxit
+      real(c_double), value :: tol
+    end function
+    ! pEVSL_GET_NEV_F90 (src/mod_pevsl.f90:124)
+    integer(c_int) function nmg_get_nev(h, nev) bind(C, name="nmg_get_nev")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+      integer(c_int), intent(out) :: nev
+    end function
+    ! pEVSL_COPY_RESULT_F90 (src/mod_pevsl.f90:130)
+    integer(c_int) function nmg_copy_result(h, val, vec, n_loc) bind(C, name="nmg_copy_result")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h, n_loc
+      real(c_double), intent(out) :: val(*), vec(*)
+    end function
+    ! residual check of src/mod_pevsl.f90:137-162, on the device
+    integer(c_int) function nmg_residuals(h, res_ref, res2) bind(C, name="nmg_residuals")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double), intent(out) :: res_ref(*), res2(*)
+    end function
+  end interface
+
+contains
+
+  ! message of the last failed call
+  function nmg_errmsg() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: i, n
+    p = nmg_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = s(i)
+    end do
+  end function
+
+  ! stop with the library's message on a failed call (the reference stops
+  ! on fatal conditions, e.g. src/mod_utility.f90:1011-1013)
+  subroutine nmg_check(ierr, what)
+    integer(c_int), intent(in) :: ierr
+    character(len=*), intent(in) :: what
+    if (ierr /= 0) then
+      write (*, '(a,a,i0,a,a)') trim(what), ': status ', ierr, ': ', nmg_errmsg()
+      stop 1
+    end if
+  end subroutine
+
+end module nmgpu_f90

@@ -21,7 +21,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
-           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -271,6 +271,14 @@ def solve_stats(h: Handle):
     return dict(lan_steps=ls.value, filter_applies=fa.value, ncand=nc.value, seconds=sec.value)
 
 
+def set_residual_target(h: Handle, target: float):
+    _check(_lib().nmg_set_residual_target(_i64(h), _d(target)))
+
+
+def set_block_size(h: Handle, block: int):
+    _check(_lib().nmg_set_block_size(_i64(h), ctypes.c_int(block)))
+
+
 def set_kernel_probe(h: Handle, every: int):
     _check(_lib().nmg_set_kernel_probe(_i64(h), ctypes.c_int(every)))
 
@@ -300,11 +308,31 @@ def apply(h: Handle, which: int, x):
     return y
 
 
+def apply_block(h: Handle, which: int, X):
+    """X: (K, n) host block; returns op applied to every row (column of the block)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.empty_like(X)
+    _check(_lib().nmg_apply_block(_i64(h), ctypes.c_int(which), ctypes.c_int(X.shape[0]), _p(X, _f64p),
+                                  _p(Y, _f64p)))
+    return Y
+
+
 def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):
     s, b = _d(), _d()
     _check(_lib().nmg_bench_spmv(_i64(mat), ctypes.c_int(reps), ctypes.c_int(int(fp32)), ctypes.byref(s),
                                  ctypes.byref(b)))
     return s.value, b.value
+
+
+def bench_block_chebstep(cheb: Handle, K: int, reps: int = 20, fp32: bool = False):
+    s, b = _d(), _d()
+    _check(_lib().nmg_bench_block_chebstep(_i64(cheb), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
+                                           ctypes.byref(s), ctypes.byref(b)))
+    return s.value, b.value
+
+
+def set_kernel_map(mat: Handle, group: int = 0, K: int = 0, rows_per_wave: int = 0):
+    _check(_lib().nmg_set_kernel_map(_i64(mat), ctypes.c_int(group), ctypes.c_int(K), ctypes.c_int(rows_per_wave)))
 
 
 def bench_chebstep(cheb: Handle, reps: int = 20, fp32: bool = False):

@@ -52,12 +52,12 @@ def metric_name():
         return "s to all eigpairs in interval @resid<=1e-10, FP32-filter vs FP64; SpMV HBM GB/s"
 
 
-def solve_once(mv, interval, bdeg, precision, probe_every):
+def solve_once(mv, interval, bdeg, precision, probe_every, block):
     from normalmodes_amd import driver, pevsl as pv
     t0 = time.perf_counter()
     driver.setup_bsolve(mv, bdeg=bdeg)
     r = driver.apply_pevsl(mv, interval=interval, precision=precision, vectors=False,
-                           probe_every=probe_every)
+                           probe_every=probe_every, block=block)
     return time.perf_counter() - t0, r
 
 
@@ -67,7 +67,7 @@ def spmv_rates(name, reps=50):
     t0 = time.perf_counter()
     P = build_problem(name)
     t_build = time.perf_counter() - t0
-    mv = driver.upload(P)
+    mv = driver.upload(P, tune=(1,))
     driver.setup_bsolve(mv, bdeg="auto")
     out = dict(workload=name, n_dof=P.n, nnz_A=P.A.nnz, nnz_B=P.B.nnz, build_s=round(t_build, 2))
     try:
@@ -127,6 +127,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--bdeg", default="auto", help="B-solve degree: auto | reference | <int>")
     ap.add_argument("--probe-every", type=int, default=64)
+    ap.add_argument("--block", type=int, default=1, help="Lanczos block width (1 = ChebLanNr)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     args = ap.parse_args()
@@ -146,17 +147,17 @@ def main():
     interval = INTERVALS[args.workload]
     log(f"[bench] built {args.workload}: n={P.n} nnzA={P.A.nnz} nnzB={P.B.nnz} in {t_build:.1f}s")
     t0 = time.perf_counter()
-    mv = driver.upload(P)
+    mv = driver.upload(P, tune=sorted({1, args.block}))
     t_upload = time.perf_counter() - t0
     precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
 
     for i in range(args.warmup):
-        el, r = solve_once(mv, interval, bdeg, precision, 0)
+        el, r = solve_once(mv, interval, bdeg, precision, 0, args.block)
         log(f"[bench] warmup {i}: {el:.2f}s nev={r.lam.size} steps={r.lan_steps}")
     times = []
     r = None
     for i in range(args.steps):
-        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every)
+        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every, args.block)
         times.append(el)
         log(f"[bench] step {i}: {el:.2f}s nev={r.lam.size} deg={r.deg} steps={r.lan_steps} "
             f"max res2={np.max(r.res2) if r.res2.size else float('nan'):.2e}")
@@ -184,14 +185,15 @@ def main():
             "n_dof": P.n, "nnz_A": P.A.nnz, "nnz_B": P.B.nnz, "n_tets": P.mesh.n_elem,
             "interval": list(interval), "nev": int(r.lam.size), "filter_degree": r.deg,
             "bsolve_degree": mv.bdeg, "lanczos_steps": r.lan_steps,
-            "lanbounds_steps": r.lan_steps_bounds, "precision": args.precision,
+            "lanbounds_steps": r.lan_steps_bounds, "precision": args.precision, "block": args.block,
+            "kernel_layouts": mv.tuned,
             "parallelism": "1 GPU", "global_batch": 1, "seq_len": P.n,
         },
         "gate": {"resid_max": float(np.max(r.res2)) if r.res2.size else None,
                  "resid_ref_max": float(np.max(r.res_ref)) if r.res_ref.size else None,
                  "passed": gate_ok, "threshold": GATE},
         "phases_s": {k: round(v, 4) for k, v in r.timings.items()},
-        "setup_s": {"build": round(t_build, 2), "upload": round(t_upload, 3)},
+        "setup_s": {"build": round(t_build, 2), "upload_and_autotune": round(t_upload, 3)},
         "roofline": {
             "bound": "hbm", "kernel": "fused Chebyshev B-solve step (scalar mass x I3 product + update)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -108,7 +108,18 @@ static hipStream_t default_stream() {
 
 // ---------------------------------------------------------------------------
 // matrices
+Tiling::~Tiling() {
+  dfree(trp);
+  dfree(tcp);
+  dfree(tcols);
+  dfree(lcol);
+}
+
 void DevMat::release() {
+  for (auto& t : tiling) {
+    delete t;
+    t = nullptr;
+  }
   dfree(rowptr);
   dfree(col);
   dfree(val);
@@ -220,6 +231,8 @@ DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rp, const int32_t
   }
   M->nb = nb;
   M->nbnz = nbnz;
+  M->h_browptr = brp;
+  M->h_bcol = bc;
   M->browptr = dmalloc<int64_t>(nb + 1);
   M->bcol = dmalloc<int32_t>(nbnz);
   M->bval = dmalloc<double>((size_t)planes * nbnz);
@@ -227,6 +240,85 @@ DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rp, const int32_t
   NMG_HIP(hipMemcpy(M->bcol, bc.data(), sizeof(int32_t) * nbnz, hipMemcpyHostToDevice));
   NMG_HIP(hipMemcpy(M->bval, bv.data(), sizeof(double) * planes * nbnz, hipMemcpyHostToDevice));
   return M.release();
+}
+
+// Greedy row tiling on the host: consecutive node rows are added to the
+// current tile while the number of distinct referenced nodes stays within
+// max_nodes.  Outputs tile row offsets, node-list offsets, the sorted node
+// lists and each entry's index within its tile's list.
+struct HostTiling {
+  std::vector<int64_t> trp{0}, tcp{0};
+  std::vector<int32_t> tcols;
+  std::vector<uint16_t> lcol;
+  int max_nodes = 0;
+};
+static void compute_tiling(int64_t nb, const int64_t* rp, const int32_t* ci, int max_nodes, HostTiling& H) {
+  NMG_CHECK(max_nodes > 0 && max_nodes <= 65535, NMG_EINVAL, "tile node list must fit 16-bit local indices");
+  const int64_t nnz = rp[nb];
+  std::vector<int64_t> stamp(nb, -1);
+  std::vector<int32_t> local(nb, 0), cur;
+  H.lcol.assign((size_t)nnz, 0);
+  int64_t t = 0, start = 0, row = 0;
+  auto close_tile = [&]() {
+    std::sort(cur.begin(), cur.end());
+    for (size_t i = 0; i < cur.size(); ++i) local[cur[i]] = (int32_t)i;
+    for (int64_t r = start; r < row; ++r)
+      for (int64_t k = rp[r]; k < rp[r + 1]; ++k) H.lcol[k] = (uint16_t)local[ci[k]];
+    H.tcols.insert(H.tcols.end(), cur.begin(), cur.end());
+    H.max_nodes = std::max(H.max_nodes, (int)cur.size());
+    H.trp.push_back(row);
+    H.tcp.push_back((int64_t)H.tcols.size());
+    cur.clear();
+    start = row;
+    t++;
+  };
+  while (row < nb) {
+    int add = 0;
+    for (int64_t k = rp[row]; k < rp[row + 1]; ++k) {
+      NMG_CHECK(ci[k] >= 0 && ci[k] < nb, NMG_EINVAL, "column out of range");
+      if (stamp[ci[k]] != t) add++;
+    }
+    if ((int)cur.size() + add > max_nodes && row > start) {
+      close_tile();
+      continue;
+    }
+    NMG_CHECK(add <= max_nodes, NMG_EINVAL, "a row references more nodes than one tile holds");
+    for (int64_t k = rp[row]; k < rp[row + 1]; ++k)
+      if (stamp[ci[k]] != t) {
+        stamp[ci[k]] = t;
+        cur.push_back(ci[k]);
+      }
+    row++;
+  }
+  if (row > start) close_tile();
+}
+
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave) {
+  int idx = -1;
+  for (int i = 0; i < 5; ++i)
+    if ((1 << i) == K) idx = i;
+  NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+  delete M.tiling[idx];
+  M.tiling[idx] = nullptr;
+  if (max_nodes <= 0 || M.layout == L_CSR) return;
+  NMG_CHECK((int64_t)M.h_browptr.size() == M.nb + 1, NMG_ESTATE, "no host pattern for tiling");
+  HostTiling H;
+  compute_tiling(M.nb, M.h_browptr.data(), M.h_bcol.data(), max_nodes, H);
+  std::unique_ptr<Tiling> T(new Tiling());
+  T->ntiles = (int64_t)H.trp.size() - 1;
+  T->max_nodes = H.max_nodes;
+  T->cap = max_nodes;
+  T->rows_per_wave = rows_per_wave > 0 ? rows_per_wave : 1;
+  T->staged_nodes = (double)H.tcols.size();
+  T->trp = dmalloc<int64_t>(H.trp.size());
+  T->tcp = dmalloc<int64_t>(H.tcp.size());
+  T->tcols = dmalloc<int32_t>(H.tcols.size());
+  T->lcol = dmalloc<uint16_t>(H.lcol.size());
+  NMG_HIP(hipMemcpy(T->trp, H.trp.data(), sizeof(int64_t) * H.trp.size(), hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(T->tcp, H.tcp.data(), sizeof(int64_t) * H.tcp.size(), hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(T->tcols, H.tcols.data(), sizeof(int32_t) * H.tcols.size(), hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(T->lcol, H.lcol.data(), sizeof(uint16_t) * H.lcol.size(), hipMemcpyHostToDevice));
+  M.tiling[idx] = T.release();
 }
 
 DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
@@ -237,6 +329,10 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->layout = S.layout;
   M->nb = S.nb;
   M->nbnz = S.nbnz;
+  M->h_browptr = S.h_browptr;
+  M->h_bcol = S.h_bcol;
+  M->group = S.group;
+  for (int i = 0; i < 5; ++i) M->block_map[i] = S.block_map[i];
   auto cp = [&](auto* src, auto*& dst, size_t count) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
     if (!src) return;
@@ -253,6 +349,9 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
     cp(S.bcol, M->bcol, S.nbnz);
     cp(S.bval, M->bval, planes * S.nbnz);
   }
+  M->single_blocked = S.single_blocked;
+  for (int i = 0; i < 5; ++i)
+    if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave);
   return M.release();
 }
 
@@ -1657,6 +1756,114 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
   });
 }
 
+}  // extern "C"
+
+namespace nmg {
+// ---------------------------------------------------------------------------
+// Autotuner of the sparse-product mapping of one matrix for K columns: times
+// the candidate lane layouts on the matrix itself (untiled rows-per-wavefront
+// 1..8; LDS-tiled at three tile capacities x rows-per-wavefront 1..4; for
+// K = 1 also the lanes-per-row layout) and keeps the fastest.
+template <class F>
+static double time_launches(hipStream_t s, int reps, F&& f) {
+  f();  // warm-up
+  hipEvent_t e0, e1;
+  NMG_HIP(hipEventCreate(&e0));
+  NMG_HIP(hipEventCreate(&e1));
+  NMG_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) f();
+  NMG_HIP(hipEventRecord(e1, s));
+  NMG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  NMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms * 1e-3 / reps;
+}
+
+struct TuneResult {
+  double seconds = 1e30;
+  int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block
+  int group = 0, rows = 1, cap = 0;
+  int64_t ntiles = 0;
+};
+
+static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t s) {
+  NMG_CHECK(M.layout != L_CSR, NMG_EINVAL, "autotuning applies to the blocked layouts");
+  int idx = 0;
+  while ((1 << idx) != K) ++idx;
+  const int64_t nk = M.nrow * K;
+  DVec x, y;
+  x.ensure(nk);
+  y.ensure(nk);
+  k_fill(nk, 1.0, x.p, s);
+  TuneResult best;
+  auto consider = [&](double t, int kind, int group, int rows, int cap, int64_t nt) {
+    if (t < best.seconds) best = TuneResult{t, kind, group, rows, cap, nt};
+  };
+  // untiled layouts
+  build_tiling(M, K, 0, 1);
+  M.single_blocked = false;
+  if (K == 1) {
+    for (int g : {8, 16, 32, 64}) {
+      M.group = g;
+      consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 0, g, 1, 0, 0);
+    }
+    M.group = 0;
+  }
+  for (int R : {1, 2, 4, 8}) {
+    if (K * R > 64) continue;
+    M.block_map[idx] = R;
+    consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 1, 0, R, 0, 0);
+  }
+  // tiled layouts (LDS budgets 64, 32, 16 KB) whose tiles hold the longest row
+  int64_t longest = 0;
+  for (int64_t r = 0; r < M.nb; ++r) longest = std::max(longest, M.h_browptr[r + 1] - M.h_browptr[r]);
+  for (int kb : {64, 32, 16}) {
+    const int cap = std::min(65535, kb * 1024 / (24 * K));
+    if (cap < longest) continue;
+    for (int R : {1, 2, 4, 8}) {
+      if (K * R > 64) continue;
+      build_tiling(M, K, cap, R);
+      consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 2, 0, R, cap,
+               M.tiling[idx] ? M.tiling[idx]->ntiles : 0);
+    }
+  }
+  // apply the winner
+  build_tiling(M, K, 0, 1);
+  M.block_map[idx] = 0;
+  if (best.kind == 0) {
+    M.group = best.group;
+  } else if (best.kind == 1) {
+    M.block_map[idx] = best.rows;
+    if (K == 1) M.single_blocked = true;
+  } else {
+    build_tiling(M, K, best.cap, best.rows);
+    if (K == 1) M.single_blocked = true;
+  }
+  NMG_HIP(hipStreamSynchronize(s));
+  return best;
+}
+}  // namespace nmg
+
+extern "C" {
+
+int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds, int* kind, int* rows,
+                     int* cap) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
+    hipStream_t s = default_stream();
+    if (fp32 && !M->bval32) make_fp32(*M, s);
+    TuneResult r = tune_matrix(*M, K, reps, fp32 != 0, s);
+    if (seconds) *seconds = r.seconds;
+    if (kind) *kind = r.kind;
+    if (rows) *rows = r.kind == 0 ? r.group : r.rows;
+    if (cap) *cap = r.cap;
+  });
+}
+
 int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double* seconds, double* bytes) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);
@@ -1691,6 +1898,32 @@ int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double*
     (void)hipEventDestroy(e1);
     if (seconds) *seconds = ms * 1e-3 / reps;
     if (bytes) *bytes = M->matrix_bytes(fp32 != 0) + 4.0 * 8.0 * (double)nk;
+  });
+}
+
+int nmg_tiling_host(int64_t nb, const int64_t* rowptr, const int32_t* col, int max_nodes, int64_t* ntiles,
+                    int* max_used, int64_t* trp, int64_t* tcp, int32_t* tcols, uint16_t* lcol) {
+  return guarded([&] {
+    NMG_CHECK(nb > 0 && rowptr && col, NMG_EINVAL, "null pattern");
+    HostTiling H;
+    compute_tiling(nb, rowptr, col, max_nodes, H);
+    if (ntiles) *ntiles = (int64_t)H.trp.size() - 1;
+    if (max_used) *max_used = H.max_nodes;
+    if (trp) std::memcpy(trp, H.trp.data(), sizeof(int64_t) * H.trp.size());
+    if (tcp) std::memcpy(tcp, H.tcp.data(), sizeof(int64_t) * H.tcp.size());
+    if (tcols) std::memcpy(tcols, H.tcols.data(), sizeof(int32_t) * H.tcols.size());
+    if (lcol) std::memcpy(lcol, H.lcol.data(), sizeof(uint16_t) * H.lcol.size());
+  });
+}
+
+int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(rows_per_wave >= 1 && rows_per_wave <= 8 && K * rows_per_wave <= 64, NMG_EINVAL,
+              "bad rows per wavefront");
+    NMG_CHECK(max_nodes * 24 * K <= 64 * 1024, NMG_EINVAL, "tile exceeds 64 KB of LDS");
+    build_tiling(*M, K, max_nodes, rows_per_wave);
+    if (K == 1) M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
   });
 }
 

@@ -36,6 +36,22 @@ void set_error(int code, const std::string& msg);
 // matrices in HBM
 enum Layout { L_CSR = 0, L_BSR3 = 1, L_MASS3 = 2 };
 
+// Row tiles of a BSR3 / MASS3 matrix for the LDS-staged k-column products:
+// tile t owns node rows [trp[t], trp[t+1]) and references the nodes
+// tcols[tcp[t] .. tcp[t+1]); lcol[k] is entry k's index within its tile's list.
+struct Tiling {
+  int64_t ntiles = 0;
+  int max_nodes = 0;      // largest node list (sets the LDS size)
+  int cap = 0;            // node capacity the tiles were cut for
+  int rows_per_wave = 1;  // R of the lane layout
+  int64_t* trp = nullptr;
+  int64_t* tcp = nullptr;
+  int32_t* tcols = nullptr;
+  uint16_t* lcol = nullptr;
+  double staged_nodes = 0;  // sum of list lengths (x blocks staged per product)
+  ~Tiling();
+};
+
 struct DevMat {
   int64_t nrow = 0, ncol = 0, nnz = 0;  // scalar (DoF-level) sizes
   int layout = L_CSR;
@@ -56,12 +72,18 @@ struct DevMat {
   // products; rows per wavefront of the k-column products for K = 1,2,4,8,16
   int group = 0;
   int block_map[5] = {0, 0, 0, 0, 0};
+  Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
+  bool single_blocked = false;  // single-column products through the K = 1 block kernels
+  std::vector<int64_t> h_browptr;  // host copy of the node pattern (tiling)
+  std::vector<int32_t> h_bcol;
   ~DevMat();
   void release();
   double matrix_bytes(bool fp32) const;       // values + indices + row offsets
   double algorithmic_bytes(bool fp32) const;  // one product: matrix + x + y
 };
 
+// builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);
 // creates a DevMat from host CSR (0-based); picks the layout
 DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rowptr, const int32_t* col,
                     const double* val, bool allow_blocked = true);

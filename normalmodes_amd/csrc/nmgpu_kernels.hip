@@ -215,18 +215,18 @@ static void dispatch(const DevMat& M, const double* x, Epi epi, hipStream_t s, b
 }
 
 void k_spmv(const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32) {
+  if (M.single_blocked) return k_block_spmv(1, M, x, y, s, fp32);
   dispatch(M, x, EpiStore{y}, s, fp32);
 }
 
 void k_cheb_step(const DevMat& M, const double* b, const double* xk, double* xm, double w,
                  double ith, hipStream_t s, bool fp32) {
+  if (M.single_blocked) return k_block_cheb_step(1, M, b, xk, xm, w, ith, s, fp32);
   dispatch(M, xk, EpiCheb{b, xk, xm, w, ith}, s, fp32);
 }
 
 void k_filter_step(const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
-                   double t, double cc, double mu, bool first, hipStream_t s, bool fp32) {
-  dispatch(A, x, EpiFilter{vk, vkm, y, t, cc, mu, first ? 1 : 0}, s, fp32);
-}
+                   double t, double cc, double mu, bool first, hipStream_t s, bool fp32);
 
 // --------------------------------------------------------------------------
 // vector kernels
@@ -803,8 +803,15 @@ static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi,
   NMG_HIP(hipGetLastError());
 }
 
+template <int K, class VT, class Epi>
+bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s);
+
 template <int K, class Epi>
 static void dispatch_block(const DevMat& M, const double* x, Epi epi, hipStream_t s, bool fp32) {
+  if (M.layout != L_CSR) {
+    const bool f = fp32 && M.bval32;
+    if (f ? launch_tiled<K>(M, M.bval32, x, epi, s) : launch_tiled<K>(M, M.bval, x, epi, s)) return;
+  }
   if (M.layout == L_CSR) {
     if (fp32 && M.val32) launch_block<K>(M, M.val32, x, epi, s);
     else launch_block<K>(M, M.val, x, epi, s);
@@ -823,6 +830,11 @@ static void dispatch_block(const DevMat& M, const double* x, Epi epi, hipStream_
     case 16: { constexpr int KK = 16; CALL; break; }                            \
     default: throw Error{NMG_EINVAL, "block width must be 1, 2, 4, 8 or 16"};  \
   }
+
+void k_filter_step(const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
+                   double t, double cc, double mu, bool first, hipStream_t s, bool fp32) {
+  dispatch(A, x, EpiFilter{vk, vkm, y, t, cc, mu, first ? 1 : 0}, s, fp32);
+}
 
 void k_block_spmv(int K, const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32) {
   NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(M, x, BEpiStore<KK>{y}, s, fp32)));
@@ -874,6 +886,114 @@ void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il
 void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s) {
   hipLaunchKernelGGL(from_interleaved_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, il, cm, ld);
   NMG_HIP(hipGetLastError());
+}
+
+}  // namespace nmg
+
+// ==========================================================================
+// LDS-tiled k-column products.  The node rows are cut into tiles of
+// consecutive rows (the mesh numbering is spatially local); a workgroup owns
+// one tile, first copies the x blocks (3K doubles) of every node its rows
+// reference into LDS -- each node once per tile instead of once per
+// referencing row -- and then forms the products from LDS through 16-bit
+// tile-local column indices.  Epilogues work on global memory as before.
+namespace nmg {
+
+constexpr int kTileBlock = 1024;  // 16 wavefronts share one tile's LDS
+
+template <int K, int R, class VT, class Epi, bool BSR>
+__global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __restrict__ trp,
+                                                           const int64_t* __restrict__ tcp,
+                                                           const int32_t* __restrict__ tcols,
+                                                           const int64_t* __restrict__ rp,
+                                                           const uint16_t* __restrict__ lcol,
+                                                           const VT* __restrict__ val, int64_t nbnz,
+                                                           const double* __restrict__ x, Epi epi) {
+  extern __shared__ double xs[];
+  constexpr int W3 = 3 * K;
+  constexpr int E = BlockMap<K, R>::E;
+  constexpr int U = BSR ? (BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U) : BlockMap<K, R>::U;
+  const int64_t tile = blockIdx.x;
+  const int64_t r0 = trp[tile], r1 = trp[tile + 1];
+  const int64_t c0 = tcp[tile];
+  const int nu = (int)(tcp[tile + 1] - c0);
+  // stage the tile's x blocks
+  for (int t = threadIdx.x; t < nu * W3; t += kTileBlock) {
+    const int node = t / W3, j = t - node * W3;
+    xs[t] = x[(int64_t)tcols[c0 + node] * W3 + j];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int c = lane % K, e = (lane / K) % E;
+  const int rsub = lane / (K * E);
+  for (int64_t rb = r0 + (int64_t)wave * R; rb < r1; rb += (int64_t)(kTileBlock / 64) * R) {
+    const int64_t row = rb + rsub;
+    const bool live = row < r1;
+    typename Epi::Pre pre[(3 + E - 1) / E];
+#pragma unroll
+    for (int q = 0; q < (3 + E - 1) / E; ++q)
+      if (live && e + q * E < 3) pre[q] = epi.load(3 * row + e + q * E, c);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (live) {
+      const int64_t beg = rp[row], end = rp[row + 1];
+      for (int64_t k0 = beg + e; k0 < end; k0 += (int64_t)U * E) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t k = k0 + (int64_t)u * E;
+          if (k < end) {
+            const double* xn = xs + (int)lcol[k] * W3 + c;
+            const double xa = xn[0], xb = xn[K], xc = xn[2 * K];
+            if (BSR) {
+              y0 += (double)val[k] * xa + (double)val[nbnz + k] * xb + (double)val[2 * nbnz + k] * xc;
+              y1 += (double)val[3 * nbnz + k] * xa + (double)val[4 * nbnz + k] * xb + (double)val[5 * nbnz + k] * xc;
+              y2 += (double)val[6 * nbnz + k] * xa + (double)val[7 * nbnz + k] * xb + (double)val[8 * nbnz + k] * xc;
+            } else {
+              const double m = (double)val[k];
+              y0 += m * xa;
+              y1 += m * xb;
+              y2 += m * xc;
+            }
+          }
+        }
+      }
+    }
+    y0 = slots_sum<K, E>(y0);
+    y1 = slots_sum<K, E>(y1);
+    y2 = slots_sum<K, E>(y2);
+#pragma unroll
+    for (int q = 0; q < (3 + E - 1) / E; ++q) {
+      const int p = e + q * E;
+      if (live && p < 3) epi.store(3 * row + p, c, p == 0 ? y0 : (p == 1 ? y1 : y2), pre[q]);
+    }
+  }
+}
+
+template <int K, int R, class VT, class Epi>
+static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const double* x, Epi epi,
+                           hipStream_t s) {
+  const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
+  if (M.layout == L_BSR3)
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true>), dim3((unsigned)T.ntiles), dim3(kTileBlock), shmem,
+                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi);
+  else
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false>), dim3((unsigned)T.ntiles), dim3(kTileBlock),
+                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi);
+}
+
+template <int K, class VT, class Epi>
+bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  int idx = 0;
+  while ((1 << idx) != K) ++idx;
+  const Tiling* T = M.tiling[idx];
+  if (!T || T->ntiles <= 0 || M.layout == L_CSR) return false;
+  const int R = T->rows_per_wave;
+  if (R == 2 && K * 2 <= 64) launch_tiled_R<K, (K * 2 <= 64 ? 2 : 1)>(M, *T, v, x, epi, s);
+  else if (R == 4 && K * 4 <= 64) launch_tiled_R<K, (K * 4 <= 64 ? 4 : 1)>(M, *T, v, x, epi, s);
+  else if (R == 8 && K * 8 <= 64) launch_tiled_R<K, (K * 8 <= 64 ? 8 : 1)>(M, *T, v, x, epi, s);
+  else launch_tiled_R<K, 1>(M, *T, v, x, epi, s);
+  NMG_HIP(hipGetLastError());
+  return true;
 }
 
 }  // namespace nmg

@@ -67,6 +67,7 @@ class MatvecSetup:
     chebB: int
     porder: int
     timings: dict = field(default_factory=dict)
+    tuned: dict = field(default_factory=dict)
 
     def free(self):
         if self.chebB:
@@ -75,17 +76,30 @@ class MatvecSetup:
         pv.csr_free(self.matB)
 
 
-def upload(problem, *, porder: int | None = None) -> MatvecSetup:
+def upload(problem, *, porder: int | None = None, tune=(1,)) -> MatvecSetup:
     """Copy A and B to HBM once and scale B in place (src/mod_matvec.f90:47-71,
-    252-342).  Returns a set-up without the B-solve (see ``setup_bsolve``)."""
+    252-342).  Returns a set-up without the B-solve (see ``setup_bsolve``).
+
+    ``tune``: block widths for which the product layouts of A and B are
+    autotuned on the device (empty to keep the defaults)."""
     porder = porder or problem.mesh.porder
     t0 = time.perf_counter()
     n = problem.n
     matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
     matA = pv.csr_create(problem.A.rowptr, problem.A.col, problem.A.val)
     d = pv.csr_diagscale(matB, 1.0)                      # Bdiagscaling, :252-342
-    return MatvecSetup(n=n, matA=matA, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
-                       porder=porder, timings=dict(upload=time.perf_counter() - t0))
+    t = dict(upload=time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    tuned = {}
+    for K in tune:
+        for key, mat in (("B", matB), ("A", matA)):
+            if pv.csr_info(mat)["layout"] != "csr":
+                tuned[f"{key}{K}"] = pv.csr_autotune(mat, K, 10)
+    t["autotune"] = time.perf_counter() - t0
+    mv = MatvecSetup(n=n, matA=matA, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
+                     porder=porder, timings=t)
+    mv.tuned = tuned
+    return mv
 
 
 def setup_bsolve(mv: MatvecSetup, *, bdeg="reference", seed: int = 7,
@@ -110,10 +124,10 @@ def setup_bsolve(mv: MatvecSetup, *, bdeg="reference", seed: int = 7,
 
 
 def setupmatvec(problem, *, porder: int | None = None, bdeg="reference", seed: int = 7,
-                bounds=(1000, 2000, 1e-12)) -> MatvecSetup:
+                bounds=(1000, 2000, 1e-12), tune=(1,)) -> MatvecSetup:
     """Upload A and B, scale B, bound its spectrum and set up its Chebyshev
     solve (src/mod_matvec.f90:24-95, 225-247)."""
-    mv = upload(problem, porder=porder)
+    mv = upload(problem, porder=porder, tune=tune)
     return setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds)
 
 

@@ -21,7 +21,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
-           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -333,6 +333,19 @@ def bench_block_chebstep(cheb: Handle, K: int, reps: int = 20, fp32: bool = Fals
 
 def set_kernel_map(mat: Handle, group: int = 0, K: int = 0, rows_per_wave: int = 0):
     _check(_lib().nmg_set_kernel_map(_i64(mat), ctypes.c_int(group), ctypes.c_int(K), ctypes.c_int(rows_per_wave)))
+
+
+def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1):
+    _check(_lib().nmg_set_tiling(_i64(mat), ctypes.c_int(K), ctypes.c_int(max_nodes), ctypes.c_int(rows_per_wave)))
+
+
+def csr_autotune(mat: Handle, K: int, reps: int = 10, fp32: bool = False):
+    s = _d()
+    kind, rows, cap = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(_lib().nmg_csr_autotune(_i64(mat), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
+                                   ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap)))
+    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled"][kind.value],
+                rows=rows.value, cap=cap.value)
 
 
 def bench_chebstep(cheb: Handle, reps: int = 20, fp32: bool = False):

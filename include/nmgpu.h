@@ -86,7 +86,11 @@ int nmg_set_bmv(nmg_handle h, nmg_handle matB);
 int nmg_set_bsol_chebiter(nmg_handle h, int type, nmg_handle cheb);
 /* PEVSL_SET_GENEIG_F90(h)  src/mod_pevsl.f90:82 */
 int nmg_set_geneig(nmg_handle h);
-/* Filter precision (no reference counterpart: pEVSL is FP64 only). */
+/* Filter precision (no reference counterpart: pEVSL is FP64 only).
+ * NMG_FP32_FILTER: the Chebyshev B-solve inside the filter reads FP32 values
+ * of B~ (FP64 vectors, FP64 row accumulation); A~, the Lanczos recurrence,
+ * re-orthogonalisation, Rayleigh-Ritz and residuals stay FP64.  Call after
+ * nmg_set_bsol_chebiter. */
 int nmg_set_precision(nmg_handle h, int mode);
 /* Lanczos start vector seed (pEVSL draws a random vinit). */
 int nmg_set_seed(nmg_handle h, uint64_t seed);

@@ -556,7 +556,7 @@ static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double
     }
     const bool pr = S.probe[1].armed();
     if (pr) S.probe[1].before(S.s);
-    k_filter_step(A, in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, S.fp32());
+    k_filter_step(A, in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, false);
     if (pr) S.probe[1].after(S.s);
     std::swap(vk, vkm);
   }
@@ -950,7 +950,7 @@ static void block_filter(Solver& S, int K, const PolyFilter& P, const double* mu
     }
     const bool pr = S.probe[1].armed();
     if (pr) S.probe[1].before(S.s);
-    k_block_filter_step(K, A, in, vk, vkm, y, mu_dev + (size_t)k * K, t, P.cc, k == 1, S.s, S.fp32());
+    k_block_filter_step(K, A, in, vk, vkm, y, mu_dev + (size_t)k * K, t, P.cc, k == 1, S.s, false);
     if (pr) S.probe[1].after(S.s);
     std::swap(vk, vkm);
   }
@@ -1449,9 +1449,11 @@ int nmg_set_precision(nmg_handle h, int mode) {
     NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER, NMG_EINVAL, "unknown precision");
     S->precision = mode;
     if (mode == NMG_FP32_FILTER) {
-      if (S->adad) make_fp32(*S->adad, S->s);
-      else if (S->amat) make_fp32(*S->amat, S->s);
-      if (S->bsol) make_fp32(*S->bsol->mat, S->s);
+      // FP32 values in the Chebyshev B-solve only: B~ is well conditioned
+      // (kappa ~ 15 at P2), while FP32 rounding of the stiffness would be
+      // amplified by lambda_max/lambda on the low modes
+      NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "set the B-solve before the precision");
+      make_fp32(*S->bsol->mat, S->s);
       NMG_HIP(hipStreamSynchronize(S->s));
     }
   });
@@ -1663,7 +1665,7 @@ int nmg_kernel_probe(nmg_handle h, int which, double* mean_seconds, int64_t* sam
         NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
         *bytes_per_launch = S->bsol->mat->matrix_bytes(f32) + 4.0 * 8.0 * nk;
       } else {
-        *bytes_per_launch = S->A().matrix_bytes(f32) + 6.0 * 8.0 * nk;
+        *bytes_per_launch = S->A().matrix_bytes(false) + 6.0 * 8.0 * nk;
       }
     }
   });

@@ -307,3 +307,20 @@ def test_block_products_match_single_column(small_p2):
     finally:
         pv.pevsl_finish(h)
         mv.free()
+
+
+def test_fp32_bsolve_eigenvalues_match_fp64(small_p2, dense_eigs):
+    """FP32 values in the Chebyshev B-solve: eigenvalues still agree with the
+    FP64 path to 1e-12 relative (quadratic in the vector error); the 2-norm
+    residual is bounded by the FP32 B-solve near 1e-6 and is reported."""
+    from normalmodes_amd import pevsl as pv
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    mv = setupmatvec(small_p2, bdeg="auto")
+    try:
+        r64 = apply_pevsl(mv, interval=(1.0, 12.0))
+        r32 = apply_pevsl(mv, interval=(1.0, 12.0), precision=pv.FP32_FILTER)
+    finally:
+        mv.free()
+    assert r32.lam.size == r64.lam.size
+    assert np.max(np.abs(r32.lam - r64.lam) / r64.lam) < 1e-12
+    assert np.all(r32.res2 < 1e-4)

@@ -130,6 +130,20 @@ int nmg_copy_result(nmg_handle h, double *val, double *vec, int64_t n_loc);
  * res2[j] = ||r||_2/|lam_j| with x~ B~-normalised, r = A~x~ - lam B~x~. */
 int nmg_residuals(nmg_handle h, double *res_ref, double *res2);
 
+/* Adjoint backward (no reference counterpart; the element formulas are those
+ * of src/mod_cg_create_matrix.f90:1018-1219): for every eigenpair j of the
+ * last nmg_cheblannr and element e, the derivatives of lambda_j with respect
+ * to a uniform change of the nodal density, first Lame parameter and shear
+ * modulus of element e, x_j^T (dA_e - lambda_j dB_e) x_j with x_j = D x~_j
+ * (physical, B-normalised).  elems: int32[n_elem*np] node ids (DoF 3*v+p),
+ * detJ[n_elem], invJ[n_elem*9] (row-major, invJ = 2 B^-1 of the affine map),
+ * refM[np*np], refD[3*np*np] reference-element mass and derivative matrices.
+ * Outputs d_rho, d_lam, d_mu: f64[nev*n_elem], eigenpair-major.  job = 1
+ * (no reference gravity). */
+int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t *elems, const double *detJ,
+                 const double *invJ, const double *refM, const double *refD, int job, double *d_rho,
+                 double *d_lam, double *d_mu);
+
 /* Tuning/diagnostics of the last nmg_cheblannr / nmg_lanbounds call:
  * Lanczos steps, filter-operator applications, candidate count, seconds. */
 int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *ncand,

@@ -1,0 +1,92 @@
+"""Per-element eigenvalue sensitivities (the adjoint backward of the path).
+
+d lambda_j / d m_e = x_j^T (dA_e/dm_e - lambda_j dB_e/dm_e) x_j for m_e in
+{rho, lambda (Lame), mu} changed uniformly at the nodes of element e, with
+x_j the physical, B-normalised eigenvector x = D x~.  The element matrices
+are those of the elastic operator (src/mod_cg_create_matrix.f90:1018-1219);
+the device computes the quadratic forms directly from the element's nodal
+fields (nmg_backward in include/nmgpu.h).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import pevsl as pv
+from ._native import load_asm, load_gpu
+from .refelem import reference_element, vertex_slots
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+__all__ = ["element_geometry", "backward", "element_matrices"]
+
+
+def element_geometry(mesh):
+    """detJ (n_elem,) and invJ (n_elem, 3, 3) of the affine element maps."""
+    L = load_asm()
+    ne, npl = mesh.n_elem, mesh.elems.shape[1]
+    detJ = np.empty(ne)
+    invJ = np.empty((ne, 9))
+    vs = np.asarray(vertex_slots(mesh.porder), dtype=np.int32)
+    nodes = np.ascontiguousarray(mesh.nodes)
+    elems = np.ascontiguousarray(mesh.elems, dtype=np.int32)
+    rc = L.nm_element_geometry(ctypes.c_int64(ne), ctypes.c_int(npl), nodes.ctypes.data_as(_f64p),
+                               elems.ctypes.data_as(_i32p), vs.ctypes.data_as(_i32p), detJ.ctypes.data_as(_f64p),
+                               invJ.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise RuntimeError(f"flat element {-rc - 1}")
+    return detJ, invJ.reshape(ne, 3, 3)
+
+
+def backward(h, mesh, job: int = 1):
+    """Sensitivities of the eigenpairs held by solver handle h (after
+    apply_pevsl(..., keep=True)).  Returns (d_rho, d_lam, d_mu), each
+    (nev, n_elem), rows in ascending eigenvalue order as apply_pevsl returns
+    the pairs."""
+    L = load_gpu()
+    detJ, invJ = element_geometry(mesh)
+    _, M, D = reference_element(mesh.porder)
+    ne, npl = mesh.n_elem, mesh.elems.shape[1]
+    nev = pv.pevsl_get_nev(h)
+    out = [np.empty((nev, ne)) for _ in range(3)]
+    elems = np.ascontiguousarray(mesh.elems, dtype=np.int32)
+    M = np.ascontiguousarray(M)
+    D = np.ascontiguousarray(D)
+    invJ = np.ascontiguousarray(invJ.reshape(ne, 9))
+    pv._check(L.nmg_backward(ctypes.c_int64(h), ctypes.c_int64(ne), ctypes.c_int(npl), elems.ctypes.data_as(_i32p),
+                             detJ.ctypes.data_as(_f64p), invJ.ctypes.data_as(_f64p), M.ctypes.data_as(_f64p),
+                             D.ctypes.data_as(_f64p), ctypes.c_int(job), out[0].ctypes.data_as(_f64p),
+                             out[1].ctypes.data_as(_f64p), out[2].ctypes.data_as(_f64p)))
+    lam, _ = pv.pevsl_copy_result(h, 3 * mesh.n_nodes, False)
+    order = np.argsort(lam, kind="stable")
+    return tuple(o[order] for o in out)
+
+
+def element_matrices(mesh, sel, lam, mu, rho, g=None):
+    """Element stiffness K (len(sel), 3np, 3np) and scalar mass M (len(sel),
+    np, np) from the FE producer for the given per-element-node parameters
+    (len(sel), np)."""
+    L = load_asm()
+    sel = np.ascontiguousarray(sel, dtype=np.int64)
+    npl = mesh.elems.shape[1]
+    _, Mr, Dr = reference_element(mesh.porder)
+    K = np.empty((sel.size, 3 * npl, 3 * npl))
+    Me = np.empty((sel.size, npl, npl))
+    vs = np.asarray(vertex_slots(mesh.porder), dtype=np.int32)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (lam, mu, rho)]
+    gp = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+    rc = L.nm_element_matrices(ctypes.c_int64(sel.size), ctypes.c_int(npl),
+                               np.ascontiguousarray(mesh.nodes).ctypes.data_as(_f64p),
+                               np.ascontiguousarray(mesh.elems, dtype=np.int32).ctypes.data_as(_i32p),
+                               sel.ctypes.data_as(_i64p), vs.ctypes.data_as(_i32p),
+                               *(a.ctypes.data_as(_f64p) for a in args),
+                               None if gp is None else gp.ctypes.data_as(_f64p),
+                               np.ascontiguousarray(Mr).ctypes.data_as(_f64p),
+                               np.ascontiguousarray(Dr).ctypes.data_as(_f64p),
+                               K.ctypes.data_as(_f64p), Me.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise RuntimeError(f"flat element {-rc - 1}")
+    return K, Me

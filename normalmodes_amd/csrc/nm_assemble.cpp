@@ -378,3 +378,30 @@ int nm_element_geometry(int64_t n_elem, int np, const double* nodes, const int32
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Element matrices for a list of elements (used to check the adjoint):
+// K_e (3np x 3np) and M_e (np x np) for the given per-element-node lam, mu,
+// rho (n_sel x np each) and optional gravity g (n_sel x np x 3).
+int nm_element_matrices(int64_t n_sel, int np, const double* nodes, const int32_t* elems,
+                        const int64_t* sel, const int32_t* vslot, const double* lam, const double* mu,
+                        const double* rho, const double* g, const double* refM, const double* refD,
+                        double* K, double* Me) {
+  Elem E{np, refM, refD};
+  const int L = 3 * np;
+  for (int64_t q = 0; q < n_sel; ++q) {
+    const int64_t e = sel[q];
+    double X[12], xn[kMaxNp * 3];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) X[k * 3 + c] = nodes[(int64_t)elems[e * np + vslot[k]] * 3 + c];
+    for (int a = 0; a < np; ++a)
+      for (int c = 0; c < 3; ++c) xn[a * 3 + c] = nodes[(int64_t)elems[e * np + a] * 3 + c];
+    if (!element_matrices(E, X, lam + q * np, mu + q * np, rho + q * np, g ? g + q * np * 3 : nullptr, xn,
+                          K + q * L * L, Me + q * np * np, nullptr, nullptr))
+      return (int)(-(1 + e));
+  }
+  return 0;
+}
+
+}  // extern "C"

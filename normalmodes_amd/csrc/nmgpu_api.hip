@@ -437,6 +437,7 @@ struct Solver {
   uint64_t seed = 7;
   int ntest = 30, cycle = 20;
   double rtol = 1e-9;
+  DVec dscale;  // D of the D A D operator (for physical eigenvectors)
   double resid_target = 5e-11;  // stop when every extracted pair meets this (0 = trace test only)
   int block = 1;     // Lanczos block width (1 = the reference's single-vector ChebLanNr)
   DVec bw[6];        // interleaved n*block work blocks
@@ -1403,6 +1404,9 @@ int nmg_set_amv_dad(nmg_handle h, nmg_handle matA, const double* d) {
     NMG_HIP(hipStreamSynchronize(S->s));
     S->adad = std::move(C);
     S->amat = nullptr;
+    S->dscale.ensure(S->n);
+    NMG_HIP(hipMemcpyAsync(S->dscale.p, d, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
   });
 }
 
@@ -1607,6 +1611,46 @@ int nmg_residuals(nmg_handle h, double* res_ref, double* res2) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
     do_residuals(*S, res_ref, res2);
+  });
+}
+
+int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, const double* detJ,
+                 const double* invJ, const double* refM, const double* refD, int job, double* d_rho,
+                 double* d_lam, double* d_mu) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(S->nev > 0 && S->X, NMG_ESTATE, "no eigenpairs: run nmg_cheblannr first");
+    NMG_CHECK(S->dscale.n >= S->n, NMG_ESTATE, "the operator was not set with nmg_set_amv_dad");
+    NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "np must be 4 (P1) or 10 (P2)");
+    NMG_CHECK(job == 1, NMG_ENOTIMPL, "sensitivities with reference gravity (JOB=2) are not available");
+    NMG_CHECK(n_elem > 0 && elems && detJ && invJ && refM && refD && d_rho && d_lam && d_mu, NMG_EINVAL,
+              "null argument");
+    const int64_t nn = S->n / 3;
+    for (int64_t k = 0; k < n_elem * np; ++k)
+      NMG_CHECK(elems[k] >= 0 && elems[k] < nn, NMG_EINVAL, "element node id out of range");
+    const int nev = S->nev;
+    DVec dE, dJ, dI, dM, dD, dlam, gr, gl, gm;
+    dE.ensure((n_elem * np + 1) / 2 + 1);
+    dJ.ensure(n_elem);
+    dI.ensure(9 * n_elem);
+    dM.ensure(np * np);
+    dD.ensure(3 * np * np);
+    dlam.ensure(nev);
+    gr.ensure(n_elem * nev);
+    gl.ensure(n_elem * nev);
+    gm.ensure(n_elem * nev);
+    NMG_HIP(hipMemcpyAsync(dE.p, elems, sizeof(int32_t) * n_elem * np, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(dJ.p, detJ, sizeof(double) * n_elem, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(dI.p, invJ, sizeof(double) * 9 * n_elem, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(dM.p, refM, sizeof(double) * np * np, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(dD.p, refD, sizeof(double) * 3 * np * np, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(dlam.p, S->lam.data(), sizeof(double) * nev, hipMemcpyHostToDevice, S->s));
+    k_adjoint(np, n_elem, nev, (const int32_t*)dE.p, dJ.p, dI.p, dM.p, dD.p, S->X, S->n, S->dscale.p, dlam.p,
+              gr.p, gl.p, gm.p, S->s);
+    NMG_HIP(hipMemcpyAsync(d_rho, gr.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipMemcpyAsync(d_lam, gl.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipMemcpyAsync(d_mu, gm.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
   });
 }
 

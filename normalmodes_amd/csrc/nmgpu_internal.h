@@ -135,6 +135,10 @@ void k_block_filter_step(int K, const DevMat& A, const double* x, const double* 
 void k_block_colscale(int64_t n, int K, const double* mu, const double* x, double* y, hipStream_t s);
 void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s);
 void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s);
+// per-element eigenvalue sensitivities (see nmgpu_kernels.hip)
+void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* detJ, const double* invJ,
+               const double* refM, const double* refD, const double* X, int64_t ldx, const double* d,
+               const double* lam, double* g_rho, double* g_lam, double* g_mu, hipStream_t s);
 size_t dot_work_size();
 size_t gemv_work_size(int64_t n, int k);
 

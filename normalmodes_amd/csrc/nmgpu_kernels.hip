@@ -997,3 +997,122 @@ bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStr
 }
 
 }  // namespace nmg
+
+// ==========================================================================
+// Adjoint: per element e and eigenpair j (x B-normalised, physical
+// coordinates), the derivatives of lambda_j with respect to a uniform
+// change of the nodal Lame parameters and density of element e, from the
+// element formulas of the elastic operator:
+//   d lambda / d lam_e = detJ d^T M d,           d = sum_i D_i u_i  (divergence)
+//   d lambda / d mu_e  = 2 detJ sum_ij e_ij^T M e_ij,  e_ij = (D_j u_i + D_i u_j)/2
+//   d lambda / d rho_e = -lambda_j detJ sum_i u_i^T M u_i
+// u_i: component i of x at the element's nodes, D_i = sum_r invJ(i,r) Dr_r.
+// One thread per (element, eigenpair); the reference-element matrices sit in
+// LDS.  NP = 10 (P2) or 4 (P1).
+namespace nmg {
+
+template <int NP>
+__global__ __launch_bounds__(kBlock) void adjoint_k(int64_t ne, int nev, const int32_t* __restrict__ elems,
+                                                    const double* __restrict__ detJ,
+                                                    const double* __restrict__ invJ,
+                                                    const double* __restrict__ refM,
+                                                    const double* __restrict__ refD,
+                                                    const double* __restrict__ X, int64_t ldx,
+                                                    const double* __restrict__ d,
+                                                    const double* __restrict__ lam,
+                                                    double* __restrict__ g_rho, double* __restrict__ g_lam,
+                                                    double* __restrict__ g_mu) {
+  __shared__ double Ms[NP * NP];
+  __shared__ double Ds[3 * NP * NP];
+  for (int t = threadIdx.x; t < NP * NP; t += kBlock) Ms[t] = refM[t];
+  for (int t = threadIdx.x; t < 3 * NP * NP; t += kBlock) Ds[t] = refD[t];
+  __syncthreads();
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= ne * nev) return;
+  const int j = (int)(gid / ne);
+  const int64_t e = gid - (int64_t)j * ne;
+  const double* xj = X + (int64_t)j * ldx;
+  double u[3][NP];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    const int64_t v = elems[e * NP + a];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[i][a] = xj[3 * v + i] * d[3 * v + i];
+  }
+  double J[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) J[t] = invJ[e * 9 + t];
+  const double dJ = detJ[e];
+  // grad[i][k][a] = (D_k u_i)_a
+  double gr[3][3][NP];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double ru[3][NP];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int a = 0; a < NP; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) s += Ds[r * NP * NP + a * NP + b] * u[i][b];
+        ru[r][a] = s;
+      }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int a = 0; a < NP; ++a) gr[i][k][a] = J[k * 3 + 0] * ru[0][a] + J[k * 3 + 1] * ru[1][a] + J[k * 3 + 2] * ru[2][a];
+  }
+  auto quad = [&](const double* p, const double* q) {
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+      double t = 0.0;
+#pragma unroll
+      for (int b = 0; b < NP; ++b) t += Ms[a * NP + b] * q[b];
+      s += p[a] * t;
+    }
+    return s;
+  };
+  double dv[NP];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) dv[a] = gr[0][0][a] + gr[1][1][a] + gr[2][2][a];
+  const double glam = dJ * quad(dv, dv);
+  double gmu = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double ek[NP];
+#pragma unroll
+      for (int a = 0; a < NP; ++a) ek[a] = 0.5 * (gr[i][k][a] + gr[k][i][a]);
+      gmu += quad(ek, ek);
+    }
+  gmu *= 2.0 * dJ;
+  double grho = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) grho += quad(u[i], u[i]);
+  grho *= -lam[j] * dJ;
+  const int64_t o = (int64_t)j * ne + e;
+  g_rho[o] = grho;
+  g_lam[o] = glam;
+  g_mu[o] = gmu;
+}
+
+void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* detJ, const double* invJ,
+               const double* refM, const double* refD, const double* X, int64_t ldx, const double* d,
+               const double* lam, double* g_rho, double* g_lam, double* g_mu, hipStream_t s) {
+  const int64_t total = ne * (int64_t)nev;
+  const int64_t grid = (total + kBlock - 1) / kBlock;
+  if (grid <= 0) return;
+  if (np == 10)
+    hipLaunchKernelGGL(adjoint_k<10>, dim3((unsigned)grid), dim3(kBlock), 0, s, ne, nev, elems, detJ, invJ, refM,
+                       refD, X, ldx, d, lam, g_rho, g_lam, g_mu);
+  else if (np == 4)
+    hipLaunchKernelGGL(adjoint_k<4>, dim3((unsigned)grid), dim3(kBlock), 0, s, ne, nev, elems, detJ, invJ, refM,
+                       refD, X, ldx, d, lam, g_rho, g_lam, g_mu);
+  else
+    throw Error{NMG_EINVAL, "adjoint supports P1 (4 nodes) and P2 (10 nodes) elements"};
+  NMG_HIP(hipGetLastError());
+}
+
+}  // namespace nmg

@@ -148,6 +148,7 @@ class EigResult:
     timings: dict
     lan_steps_bounds: int = 0
     probe: dict = field(default_factory=dict)
+    handle: int = 0   # solver handle kept alive with keep=True (caller finishes it)
 
     @property
     def freq_mhz(self):
@@ -162,7 +163,8 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
-                probe_every: int = 0, block: int = 1, resid_target: float | None = None) -> EigResult:
+                probe_every: int = 0, block: int = 1, resid_target: float | None = None,
+                keep: bool = False) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
@@ -226,10 +228,14 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         lam, X = pv.pevsl_copy_result(h, mv.n, vectors)       # :124-130
         t["copy_result"] = time.perf_counter() - t0
         pv.pevsl_freepol(pol)                                 # :215
-    finally:
+    except BaseException:
+        pv.pevsl_finish(h)
+        raise
+    if not keep:
         pv.pevsl_finish(h)                                    # :220
     order = np.argsort(lam, kind="stable")                    # ssort_real, :165
     return EigResult(lam=lam[order], X=None if X is None else X[order], res_ref=r1[order],
                      res2=r2[order], lmin=lmin, lmax=lmax, xintv=xintv, deg=pinfo.deg,
                      bar=pinfo.bar, lan_steps=st["lan_steps"], filter_applies=st["filter_applies"],
-                     ncand=st["ncand"], timings=t, lan_steps_bounds=lb_steps, probe=probe)
+                     ncand=st["ncand"], timings=t, lan_steps_bounds=lb_steps, probe=probe,
+                     handle=h if keep else 0)

@@ -1,0 +1,121 @@
+"""Adjoint sensitivities d lambda / d(rho, lam, mu) per element, on the device,
+against (i) x^T (dA_e - lambda dB_e) x formed from the FE producer's own
+element matrices on the CPU and (ii) central finite differences of the
+eigenvalues of the re-assembled pencil (shift-invert ARPACK)."""
+import numpy as np
+import pytest
+
+@pytest.fixture(scope="module")
+def solved(small_p2):
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    from normalmodes_amd import pevsl as pv
+    from normalmodes_amd.adjoint import backward
+    mv = setupmatvec(small_p2, bdeg="auto")
+    r = apply_pevsl(mv, interval=(1.0, 12.0), keep=True)
+    try:
+        g = backward(r.handle, small_p2.mesh)
+    finally:
+        pv.pevsl_finish(r.handle)
+    X = r.X * mv.d[None, :]      # physical, B-normalised
+    mv.free()
+    return r, g, X
+
+
+def _reference_gradients(P, X, lam, sel):
+    from normalmodes_amd.adjoint import element_matrices
+    npl = P.mesh.elems.shape[1]
+    one, zero = np.ones((sel.size, npl)), np.zeros((sel.size, npl))
+    K_lam, _ = element_matrices(P.mesh, sel, one, zero, zero)
+    K_mu, _ = element_matrices(P.mesh, sel, zero, one, zero)
+    _, M_rho = element_matrices(P.mesh, sel, zero, zero, one)
+    out = np.zeros((3, lam.size, sel.size))
+    for q, e in enumerate(sel):
+        nodes = P.mesh.elems[e]
+        dofs = (3 * nodes[None, :] + np.arange(3)[:, None]).ravel()   # (component, node) order of K
+        for j in range(lam.size):
+            xe = X[j, dofs]
+            u = xe.reshape(3, npl)
+            out[0, j, q] = -lam[j] * sum(u[p] @ M_rho[q] @ u[p] for p in range(3))
+            out[1, j, q] = xe @ K_lam[q] @ xe
+            out[2, j, q] = xe @ K_mu[q] @ xe
+    return out
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_element_matrices(small_p2, solved):
+    r, g, X = solved
+    sel = np.arange(small_p2.mesh.n_elem)
+    ref = _reference_gradients(small_p2, X, r.lam, sel)
+    for k in range(3):
+        scale = np.abs(ref[k]).max()
+        assert np.max(np.abs(g[k] - ref[k])) < 1e-11 * scale, k
+    # consistency: sum over elements of mu * dlam/dmu + lam * dlam/dlam = lambda
+    # (A is linear and homogeneous in (lam, mu) with rho fixed)
+    lam_e = small_p2.lam.mean(axis=1)   # homogeneous model: uniform per element
+    mu_e = small_p2.mu.mean(axis=1)
+    recon = g[1] @ lam_e + g[2] @ mu_e
+    assert np.max(np.abs(recon - r.lam) / r.lam) < 1e-9
+    # and sum_e rho_e dlam/drho_e = -lambda (B linear in rho)
+    rho_e = small_p2.rho.mean(axis=1)
+    assert np.max(np.abs(g[0] @ rho_e + r.lam) / r.lam) < 1e-9
+
+
+@pytest.mark.gpu
+def test_adjoint_finite_differences(small_p2, solved, dense_eigs):
+    _check_fd(small_p2, solved, dense_eigs)
+
+
+def test_reference_gradients_finite_differences(small_p2, dense_eigs):
+    """The CPU reference (element matrices of the FE producer, quadratic forms
+    in the dense eigenvectors) against finite differences, and the Euler
+    identities sum_e (lam_e dl/dlam_e + mu_e dl/dmu_e) = l, sum_e rho_e dl/drho_e = -l."""
+    import scipy.linalg as sl
+    P = small_p2
+    w, V = sl.eigh(P.A.to_scipy().toarray(), P.B.to_scipy().toarray())
+    sel = np.where((w > 1.0) & (w < 12.0))[0]
+    g = _reference_gradients(P, V[:, sel].T, w[sel], np.arange(P.mesh.n_elem))
+    lam_e, mu_e, rho_e = P.lam.mean(axis=1), P.mu.mean(axis=1), P.rho.mean(axis=1)
+    assert np.max(np.abs(g[1] @ lam_e + g[2] @ mu_e - w[sel]) / w[sel]) < 1e-12
+    assert np.max(np.abs(g[0] @ rho_e + w[sel]) / w[sel]) < 1e-12
+
+    class R:
+        lam = w[sel]
+    _check_fd(P, (R, g, None), dense_eigs)
+
+
+def _check_fd(small_p2, solved, dense_eigs):
+    import scipy.sparse.linalg as sla
+    from normalmodes_amd.assemble import assemble
+    r, g, X = solved
+    P = small_p2
+    # the best-separated computed eigenvalue: its derivative is well defined and
+    # the perturbation below moves it far less than its distance to the others
+    w = dense_eigs(P)
+    gaps = [np.min(np.abs(np.delete(w, np.argmin(np.abs(w - l))) - l)) / l for l in r.lam]
+    j = int(np.argmax(gaps))
+    lamj = r.lam[j]
+    assert gaps[j] > 1e-4
+    rho0, lam0, mu0 = P.rho.copy(), P.lam.copy(), P.mu.copy()
+    rng = np.random.default_rng(2018)
+    top = np.argsort(-np.abs(g[2][j]))[:3]
+    elems = np.concatenate([top, rng.choice(P.mesh.n_elem, size=1)])
+    rel = 1e-4
+
+    def eig_near(rho, lam, mu):
+        vs = np.sqrt(mu / rho)
+        vp = np.sqrt((lam + 2 * mu) / rho)
+        Q = assemble(P.mesh, rho=rho, vp=vp, vs=vs)
+        ev = sla.eigsh(Q.A.to_scipy().tocsc(), k=4, M=Q.B.to_scipy().tocsc(), sigma=lamj * (1 - 1e-6),
+                       which="LM", tol=1e-14, return_eigenvectors=False)
+        return ev[np.argmin(np.abs(ev - lamj))]
+
+    for e in elems:
+        for k, base in ((0, rho0), (1, lam0), (2, mu0)):
+            h = rel * float(np.abs(base[e]).mean())
+            vals = []
+            for sgn in (+1, -1):
+                arr = [rho0.copy(), lam0.copy(), mu0.copy()]
+                arr[k][e] += sgn * h
+                vals.append(eig_near(*arr))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - g[k][j, e]) <= 1e-4 * abs(g[k][j, e]) + 1e-7 * lamj, (int(e), k, fd, g[k][j, e])

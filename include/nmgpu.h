@@ -144,6 +144,32 @@ int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t *elems, con
                  const double *invJ, const double *refM, const double *refD, int job, double *d_rho,
                  double *d_lam, double *d_mu);
 
+/* Density of states by the kernel polynomial method, as pEVSL's kpmdos
+ * (the reference fixes NSLICES = 1, src/mod_pevsl.f90:47, 104-105, and never
+ * calls it; pEVSL's slicing drivers do).  intv = {a, b, lmin, lmax}.
+ * mu[k], k = 0..mdeg: Hutchinson estimates (nvec Rademacher vectors; nvec in
+ * {1, 2, 4, 8} or a multiple of 16) of tr T_k((A~B~^-1 - cc)/dd),
+ * cc = (lmax+lmin)/2, dd = (lmax-lmin)/2.  ecnt: Jackson-damped estimate of
+ * the number of eigenvalues in [a, b]. */
+int nmg_kpmdos(nmg_handle h, int mdeg, int nvec, const double *intv, double *mu, double *ecnt);
+/* Jackson-damped eigenvalue count in [intv[0], intv[1]] from the moments. */
+int nmg_dos_count(const double *mu, int mdeg, const double *intv, double *count);
+/* pEVSL spslicer: boundaries sli[0] = a < ... < sli[n_slices] = b with equal
+ * estimated counts (bisection of the closed-form count). */
+int nmg_spslicer(double *sli, const double *mu, int mdeg, const double *intv, int n_slices);
+/* Batched spectrum slices (no reference counterpart: NSLICES = 1 at
+ * src/mod_pevsl.f90:47): ChebLanNr on every [sli[i], sli[i+1]] (i <
+ * n_slices <= 16), each with its own FindPol(thr_int, thr_ext) filter on
+ * [lmin, lmax], its own recurrence and tests, advanced in lockstep so that
+ * every sparse product reads A~ and B~ once for all slices.  The union of
+ * the pairs (a pair found by two neighbours on their shared boundary kept
+ * once) is then what nmg_get_nev / nmg_copy_result / nmg_residuals /
+ * nmg_backward return, slice by slice. */
+int nmg_solve_slices(nmg_handle h, int n_slices, const double *sli, double lmin, double lmax, double thr_int,
+                     double thr_ext, int maxit, double tol);
+/* pairs per slice of the last nmg_solve_slices (int[n_slices]). */
+int nmg_slice_counts(nmg_handle h, int *counts);
+
 /* Tuning/diagnostics of the last nmg_cheblannr / nmg_lanbounds call:
  * Lanczos steps, filter-operator applications, candidate count, seconds. */
 int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *ncand,
@@ -182,21 +208,6 @@ int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *sam
 int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double *z, double *y);
 /* Operators of the handle on host vectors: which = 0 A, 1 B, 2 B-solve. */
 int nmg_apply(nmg_handle h, int which, const double *x, double *y);
-/* Residual target of nmg_cheblannr (default 5e-11; 0 = stop on the
- * reference's trace test alone).  Once the trace test and the filtered Ritz
- * residuals pass, the pairs are extracted and the iteration stops when every
- * ||A~x - lam B~x||_2/|lam| (x B~-normalised) is at or below the target, or
- * when that residual stops improving (its floor is set by the B-solve
- * accuracy). */
-int nmg_set_residual_target(nmg_handle h, double target);
-
-/* Lanczos block width (no reference counterpart; pEVSL's ChebLanNr is the
- * block width 1, the default).  With block > 1, nmg_cheblannr runs the block
- * filtered Lanczos: the same filter, B-solve and extraction, applied to
- * `block` vectors at once so that each sparse product streams the matrices
- * once per block (1, 2, 4, 8 or 16). */
-int nmg_set_block_size(nmg_handle h, int block);
-
 /* The same on K host columns at once (X, Y: K blocks of n, column c at c*n)
  * through the k-column kernels of the block filter; K in {1, 2, 4, 8, 16}. */
 int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);

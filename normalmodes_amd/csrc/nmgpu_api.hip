@@ -14,6 +14,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -459,9 +460,13 @@ struct Solver {
   int ncand = 0;
   double seconds = 0;
 
+  bool borrowed = false;  // stream and rocBLAS handle belong to a parent solver
+  std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
+
   ~Solver() {
     free_basis();
     dfree(X);
+    if (borrowed) return;
     if (blas) rocblas_destroy_handle(blas);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -569,36 +574,51 @@ static void setup_work(Solver& S) {
   S.red.ensure((int64_t)std::max(dot_work_size(), (size_t)16));
 }
 
-// one Lanczos step: Z(:,k+1) <- op(.) then three-term, CGS2, B-solve, normalise
-// op: 0 = A v_k (bounds), 1 = filter of z_k
-static void lanczos_step(Solver& S, int k, int op, const PolyFilter* P, double* hwork) {
+// Lanczos step k, split so that several independent recurrences can share
+// their sparse products: (1) op(.) into z_{k+1}; (2) three-term recurrence and
+// CGS2 against (Z, V)(:, 0:k); (3) v_{k+1} = B^-1 z_{k+1}; (4) B-normalise.
+static void lanczos_orth(Solver& S, int k, double* hwork) {
   const int64_t n = S.n;
   double* v = S.V + (size_t)k * n;
   double* z = S.Z + (size_t)k * n;
-  double* vn = S.V + (size_t)(k + 1) * n;
   double* zn = S.Z + (size_t)(k + 1) * n;
   double* alpha = S.ab.p + k;
   double* beta = S.ab.p + S.ab.n / 2 + k;
-  if (op == 0) k_spmv(S.A(), v, zn, S.s);
-  else filter_apply(S, *P, z, zn);
   if (k > 0) k_axpy_dev(n, beta - 1, S.Z + (size_t)(k - 1) * n, zn, S.s);
   k_dot(n, v, zn, alpha, false, S.red.p, S.s);
   k_axpy_dev(n, alpha, z, zn, S.s);
-  // classical Gram-Schmidt, twice, against (Z, V)(:, 0:k)
+  // classical Gram-Schmidt, twice
   double* h = hwork;
   double* gw = hwork + (k + 2);
   for (int pass = 0; pass < 2; ++pass) {
     k_gemv_t(n, k + 1, S.V, n, zn, h, gw, S.s);
     k_gemv_n_sub(n, k + 1, S.Z, n, h, zn, S.s);
   }
+}
+
+static void lanczos_normalise(Solver& S, int k) {
+  const int64_t n = S.n;
+  double* vn = S.V + (size_t)(k + 1) * n;
+  double* zn = S.Z + (size_t)(k + 1) * n;
+  double* beta = S.ab.p + S.ab.n / 2 + k;
   if (S.geneig) {
-    bsolve(S, zn, vn, S.w[4].p);
     k_dot(n, vn, zn, beta, true, S.red.p, S.s);
     k_invscale_dev(n, beta, vn, zn, S.s);
   } else {
     k_dot(n, zn, zn, beta, true, S.red.p, S.s);
     k_invscale_dev(n, beta, zn, nullptr, S.s);
   }
+}
+
+// op: 0 = A v_k (bounds), 1 = filter of z_k
+static void lanczos_step(Solver& S, int k, int op, const PolyFilter* P, double* hwork) {
+  const int64_t n = S.n;
+  double* zn = S.Z + (size_t)(k + 1) * n;
+  if (op == 0) k_spmv(S.A(), S.V + (size_t)k * n, zn, S.s);
+  else filter_apply(S, *P, S.Z + (size_t)k * n, zn);
+  lanczos_orth(S, k, hwork);
+  if (S.geneig) bsolve(S, zn, S.V + (size_t)(k + 1) * n, S.w[4].p);
+  lanczos_normalise(S, k);
 }
 
 static void lanczos_start(Solver& S) {
@@ -822,6 +842,57 @@ static bool residual_stop(Solver& S, int kdim, const double* dC, const std::vect
   return kdim - rt.best_k >= std::max(60, kdim / 4);
 }
 
+// ChebLanNr's test at Lanczos dimension kdim: on the reference cadence
+// (ntest, then every cycle steps) the trace of the Ritz values >= bar must
+// settle (|tr1 - tr0| < tol |tr1|), then every candidate's Ritz residual
+// |beta_k s_k| must be <= rtol, then (with a residual target) the extracted
+// pairs must meet it.  Returns 0 to go on, 1 to stop and extract, 2 when the
+// stop already left the extracted pairs in S.
+struct LanczosTest {
+  double tr0 = 0.0;
+  ResidualTracker rt;
+  std::vector<double> al, be, w, sv;
+};
+static int lanczos_test(Solver& S, int kdim, const PolyFilter& P, const double* xintv, double tol, LanczosTest& L) {
+  if (kdim < S.ntest || ((kdim - S.ntest) % S.cycle) != 0) return 0;
+  fetch_tridiag(S, kdim, L.al, L.be);
+  if (L.be[kdim - 1] == 0.0) return 1;
+  if (tridiag_eigvals(kdim, L.al.data(), L.be.data(), L.w) != NMG_OK)
+    throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
+  double tr1 = 0.0;
+  int cnt = 0;
+  for (int i = 0; i < kdim; ++i)
+    if (L.w[i] >= P.bar) {
+      tr1 += L.w[i];
+      cnt++;
+    }
+  const bool conv = cnt > 0 && std::fabs(tr1 - L.tr0) < tol * std::fabs(tr1);
+  L.tr0 = tr1;
+  if (!conv) return 0;
+  double rmax = 0.0;
+  for (int i = 0; i < kdim; ++i)
+    if (L.w[i] >= P.bar) {
+      tridiag_eigvec(kdim, L.al.data(), L.be.data(), L.w[i], L.sv);
+      rmax = std::max(rmax, std::fabs(L.be[kdim - 1] * L.sv[kdim - 1]));
+    }
+  if (rmax > S.rtol) return 0;
+  if (S.resid_target <= 0.0) return 1;
+  std::vector<double> th;
+  DVec dC;
+  tridiag_eig_device(S, kdim, L.al, L.be, dC, th);
+  return residual_stop(S, kdim, dC.p, th, P.bar, xintv, L.rt) ? 2 : 0;
+}
+
+// extraction after the loop: eigenvectors of T on the device (rocSOLVER
+// divide & conquer), Ritz vectors, Rayleigh-Ritz
+static void lanczos_extract(Solver& S, int kdim, const PolyFilter& P, const double* xintv) {
+  std::vector<double> al, be, theta;
+  fetch_tridiag(S, kdim, al, be);
+  DVec dC;
+  tridiag_eig_device(S, kdim, al, be, dC, theta);
+  extract_ritz(S, kdim, dC.p, theta, P.bar, xintv);
+}
+
 static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, const PolyFilter& P) {
   NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
   NMG_CHECK(xintv[1] > xintv[0], NMG_EINVAL, "empty interval");
@@ -835,62 +906,17 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
   S.filter_applies = 0;
   DVec hw;
   lanczos_start(S);
-  std::vector<double> al, be, w, sv;
-  double tr0 = 0.0;
-  ResidualTracker rt;
-  int kdim = 0;
-  bool stopped_with_result = false;
-  for (int k = 0; k < maxit; ++k) {
+  LanczosTest L;
+  int kdim = 0, stop = 0;
+  for (int k = 0; k < maxit && stop == 0; ++k) {
     ensure_basis(S, k + 2, k + 1, maxit + 1);
     hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
     lanczos_step(S, k, 1, &P, hw.p);
     kdim = k + 1;
-    if (kdim < S.ntest || ((kdim - S.ntest) % S.cycle) != 0) continue;
-    fetch_tridiag(S, kdim, al, be);
-    if (be[kdim - 1] == 0.0) break;
-    if (tridiag_eigvals(kdim, al.data(), be.data(), w) != NMG_OK)
-      throw Error{NMG_ENUM, "tridiagonal eigenvalues did not converge"};
-    double tr1 = 0.0;
-    int cnt = 0;
-    for (int i = 0; i < kdim; ++i)
-      if (w[i] >= P.bar) {
-        tr1 += w[i];
-        cnt++;
-      }
-    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
-    tr0 = tr1;
-    if (!conv) continue;
-    double rmax = 0.0;
-    for (int i = 0; i < kdim; ++i)
-      if (w[i] >= P.bar) {
-        tridiag_eigvec(kdim, al.data(), be.data(), w[i], sv);
-        rmax = std::max(rmax, std::fabs(be[kdim - 1] * sv[kdim - 1]));
-      }
-    if (rmax > S.rtol) continue;
-    if (S.resid_target <= 0.0) break;
-    {
-      std::vector<double> th;
-      DVec dC;
-      tridiag_eig_device(S, kdim, al, be, dC, th);
-      if (residual_stop(S, kdim, dC.p, th, P.bar, xintv, rt)) {
-        stopped_with_result = true;
-        break;
-      }
-    }
+    stop = lanczos_test(S, kdim, P, xintv, tol, L);
   }
   S.lan_steps = kdim;
-  if (stopped_with_result) {
-    S.free_basis();
-    S.seconds = now() - t0;
-    return;
-  }
-  fetch_tridiag(S, kdim, al, be);
-
-  // ---- extraction: eigenvectors of T on the device (rocSOLVER divide & conquer)
-  DVec dC;
-  std::vector<double> theta;
-  tridiag_eig_device(S, kdim, al, be, dC, theta);
-  extract_ritz(S, kdim, dC.p, theta, P.bar, xintv);
+  if (stop != 2) lanczos_extract(S, kdim, P, xintv);
   S.free_basis();
   S.seconds = now() - t0;
 }
@@ -1213,6 +1239,243 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
   extract_ritz(S, kdim, dT.p, w, P.bar, xintv);
   S.free_basis();
   S.seconds = now() - t0;
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum slices advanced in lockstep.  Slice s has its own interval
+// [sli[s], sli[s+1]], its own FindPol filter p_s and its own single-vector
+// ChebLanNr recurrence (basis, tridiagonal, tests, extraction), exactly as a
+// separate nmg_cheblannr call would run it.  What the slices share is every
+// sparse product: at each Lanczos step the K = (#slices rounded up to a block
+// width) current vectors go through one K-column filter (per-column Chebyshev
+// coefficients; a filter of lower degree has zero coefficients past its own
+// degree, so its polynomial is unchanged) and one K-column B-solve, so A~ and
+// B~ are read once per product for all slices.  All filters share cc and dd
+// (the centre and half-width of [lmin, lmax]).
+static int block_width_for(int ns) {
+  for (int K : {1, 2, 4, 8, 16})
+    if (K >= ns) return K;
+  return -1;
+}
+
+static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, double lmax, double thr_int,
+                            double thr_ext, int maxit, double tol) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  const int K = block_width_for(ns);
+  NMG_CHECK(ns >= 1 && K > 0, NMG_EINVAL, "between 1 and 16 slices");
+  for (int i = 0; i < ns; ++i) NMG_CHECK(sli[i + 1] > sli[i], NMG_EINVAL, "slice boundaries must increase");
+  const double t0 = now();
+  const int64_t n = S.n;
+  maxit = (int)std::min<int64_t>(maxit, n);
+  // filters
+  std::vector<PolyFilter> pol(ns);
+  std::vector<std::array<double, 4>> xi(ns);
+  int maxdeg = 0;
+  for (int i = 0; i < ns; ++i) {
+    xi[i] = {sli[i], sli[i + 1], lmin, lmax};
+    const int rc = find_pol(xi[i].data(), thr_int, thr_ext, 2, 10000, 2, pol[i]);
+    NMG_CHECK(rc == NMG_OK, NMG_ENUM, "no filter meets the thresholds for slice " + std::to_string(i));
+    maxdeg = std::max(maxdeg, pol[i].deg);
+  }
+  PolyFilter Pmax = pol[0];
+  Pmax.deg = maxdeg;
+  DVec mu_dev;
+  mu_dev.ensure((int64_t)(maxdeg + 1) * K);
+  {
+    std::vector<double> m((size_t)(maxdeg + 1) * K, 0.0);
+    for (int k = 0; k <= maxdeg; ++k)
+      for (int c = 0; c < ns; ++c) m[(size_t)k * K + c] = k <= pol[c].deg ? pol[c].mu[k] : 0.0;
+    NMG_HIP(hipMemcpyAsync(mu_dev.p, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, S.s));
+  }
+  // one recurrence per slice, sharing the operators, stream and rocBLAS handle
+  std::vector<std::unique_ptr<Solver>> sub(ns);
+  for (int i = 0; i < ns; ++i) {
+    Solver* C = new Solver();
+    sub[i].reset(C);
+    C->borrowed = true;
+    C->device = S.device;
+    C->s = S.s;
+    C->blas = S.blas;
+    C->n = n;
+    C->amat = const_cast<DevMat*>(&S.A());
+    C->bmat = S.bmat;
+    C->bsol = S.bsol;
+    C->geneig = S.geneig;
+    C->precision = S.precision;
+    C->seed = S.seed + 0x9E3779B97F4A7C15ULL * (uint64_t)i;
+    C->ntest = S.ntest;
+    C->cycle = S.cycle;
+    C->rtol = S.rtol;
+    C->resid_target = S.resid_target;
+    setup_work(*C);
+    C->ab.ensure(2 * (int64_t)(maxit + 2));
+    ensure_basis(*C, std::min(maxit + 1, 64), 0, maxit + 1);
+    lanczos_start(*C);
+  }
+  setup_work(S);
+  for (int i = 0; i < 6; ++i) S.bw[i].ensure(n * K);
+  DVec cm, hw;
+  cm.ensure(n * K);
+  S.filter_applies = 0;
+  std::vector<LanczosTest> L(ns);
+  std::vector<int> stop(ns, 0), kdim(ns, 0);
+  // gather column c of the K-column block from slice c's vector (zero for
+  // finished or padding columns), and back
+  auto gather = [&](auto vec_of) {
+    for (int c = 0; c < K; ++c) {
+      const double* src = (c < ns && stop[c] == 0) ? vec_of(c) : nullptr;
+      if (src) NMG_HIP(hipMemcpyAsync(cm.p + (size_t)c * n, src, sizeof(double) * n, hipMemcpyDeviceToDevice, S.s));
+      else k_fill(n, 0.0, cm.p + (size_t)c * n, S.s);
+    }
+    k_to_interleaved(n, K, cm.p, n, S.bw[4].p, S.s);
+  };
+  auto scatter = [&](auto vec_of) {
+    k_from_interleaved(n, K, S.bw[5].p, cm.p, n, S.s);
+    for (int c = 0; c < ns; ++c)
+      if (stop[c] == 0)
+        NMG_HIP(hipMemcpyAsync(vec_of(c), cm.p + (size_t)c * n, sizeof(double) * n, hipMemcpyDeviceToDevice, S.s));
+  };
+  int steps = 0;
+  for (int k = 0; k < maxit; ++k) {
+    int active = 0;
+    for (int i = 0; i < ns; ++i) active += stop[i] == 0;
+    if (active == 0) break;
+    for (int i = 0; i < ns; ++i)
+      if (stop[i] == 0) ensure_basis(*sub[i], k + 2, k + 1, maxit + 1);
+    // z_{k+1} = p_s(A~ B~^-1) z_k for every slice, one K-column filter
+    gather([&](int c) { return sub[c]->Z + (size_t)k * n; });
+    block_filter(S, K, Pmax, mu_dev.p, S.bw[4].p, S.bw[5].p);
+    scatter([&](int c) { return sub[c]->Z + (size_t)(k + 1) * n; });
+    hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
+    for (int i = 0; i < ns; ++i)
+      if (stop[i] == 0) lanczos_orth(*sub[i], k, hw.p);
+    // v_{k+1} = B~^-1 z_{k+1}, one K-column B-solve
+    if (S.geneig) {
+      gather([&](int c) { return sub[c]->Z + (size_t)(k + 1) * n; });
+      block_bsolve(S, K, S.bw[4].p, S.bw[5].p, S.bw[2].p);
+      scatter([&](int c) { return sub[c]->V + (size_t)(k + 1) * n; });
+    }
+    for (int i = 0; i < ns; ++i)
+      if (stop[i] == 0) lanczos_normalise(*sub[i], k);
+    steps = k + 1;
+    for (int i = 0; i < ns; ++i) {
+      if (stop[i] != 0) continue;
+      kdim[i] = k + 1;
+      sub[i]->filter_applies++;
+      stop[i] = lanczos_test(*sub[i], kdim[i], pol[i], xi[i].data(), tol, L[i]);
+    }
+  }
+  // extraction and the union of the slices' pairs
+  dfree(S.X);
+  S.lam.clear();
+  S.nev = 0;
+  S.slice_nev.assign(ns, 0);
+  S.ncand = 0;
+  int total = 0;
+  for (int i = 0; i < ns; ++i) {
+    Solver& C = *sub[i];
+    if (stop[i] != 2) lanczos_extract(C, kdim[i], pol[i], xi[i].data());
+    C.free_basis();
+    total += C.nev;
+    S.ncand += C.ncand;
+  }
+  // a pair on a shared boundary can be found by both neighbours: drop the
+  // second copy when the two are B~-parallel (|x_i^T B~ x_j| > 0.5)
+  std::vector<std::vector<char>> keep(ns);
+  for (int i = 0; i < ns; ++i) keep[i].assign(sub[i]->nev, 1);
+  DVec bx, dd;
+  bx.ensure(n);
+  dd.ensure(1);
+  for (int i = 0; i + 1 < ns; ++i) {
+    const double b = sli[i + 1], del = 1e-8 * std::max(1.0, std::fabs(b));
+    Solver &L0 = *sub[i], &L1 = *sub[i + 1];
+    for (int a = 0; a < L0.nev; ++a) {
+      if (std::fabs(L0.lam[a] - b) > del) continue;
+      apply_B(S, L0.X + (size_t)a * n, bx.p);
+      for (int c = 0; c < L1.nev; ++c) {
+        if (!keep[i + 1][c] || std::fabs(L1.lam[c] - L0.lam[a]) > 2 * del) continue;
+        k_dot(n, bx.p, L1.X + (size_t)c * n, dd.p, false, S.red.p, S.s);
+        double g = 0.0;
+        NMG_HIP(hipMemcpyAsync(&g, dd.p, sizeof(double), hipMemcpyDeviceToHost, S.s));
+        NMG_HIP(hipStreamSynchronize(S.s));
+        if (std::fabs(g) > 0.5) keep[i + 1][c] = 0;
+      }
+    }
+  }
+  total = 0;
+  for (int i = 0; i < ns; ++i)
+    for (char kf : keep[i]) total += kf;
+  if (total > 0) {
+    S.X = dmalloc<double>((size_t)n * total);
+    int j = 0;
+    for (int i = 0; i < ns; ++i)
+      for (int a = 0; a < sub[i]->nev; ++a) {
+        if (!keep[i][a]) continue;
+        NMG_HIP(hipMemcpyAsync(S.X + (size_t)j * n, sub[i]->X + (size_t)a * n, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, S.s));
+        S.lam.push_back(sub[i]->lam[a]);
+        S.slice_nev[i]++;
+        ++j;
+      }
+  }
+  S.nev = total;
+  NMG_HIP(hipStreamSynchronize(S.s));
+  S.filter_applies = 0;
+  for (int i = 0; i < ns; ++i) S.filter_applies += sub[i]->filter_applies;
+  S.lan_steps = steps;
+  S.seconds = now() - t0;
+}
+
+// Trace moments tau_k ~ tr T_k((A~ B~^-1 - cc)/dd), k = 0..mdeg, by
+// Hutchinson's estimator over nvec Rademacher vectors (tr M = E[v^T M v]),
+// the vectors advanced 16 (or nvec) at a time through the K-column Chebyshev
+// step and B-solve.  A~B~^-1 is similar to B~^-1 A~, so the trace is
+// sum_i T_k(t_i) over the pencil's eigenvalues.
+static void do_kpm(Solver& S, int mdeg, int nvec, double lmin, double lmax, std::vector<double>& tau) {
+  NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
+  NMG_CHECK(mdeg >= 1 && mdeg <= 100000, NMG_EINVAL, "mdeg must be in [1, 100000]");
+  NMG_CHECK(lmax > lmin, NMG_EINVAL, "lmax must exceed lmin");
+  const int K = nvec >= 16 ? 16 : block_width_for(nvec);
+  NMG_CHECK(nvec >= 1 && K > 0 && (nvec <= 16 ? K == nvec : nvec % 16 == 0), NMG_EINVAL,
+            "nvec must be 1, 2, 4, 8 or a multiple of 16");
+  const int64_t n = S.n;
+  const double cc = 0.5 * (lmax + lmin), dd = 0.5 * (lmax - lmin);
+  setup_work(S);
+  for (int i = 0; i < 6; ++i) S.bw[i].ensure(n * K);
+  DVec v0, mom, zmu, work;
+  v0.ensure(n * K);
+  mom.ensure((int64_t)(mdeg + 1) * K);
+  zmu.ensure(K);
+  work.ensure((int64_t)block_dot_work_size(K));
+  k_fill(K, 0.0, zmu.p, S.s);
+  tau.assign(mdeg + 1, 0.0);
+  std::vector<double> h((size_t)(mdeg + 1) * K), r((size_t)n * K);
+  const DevMat& A = S.A();
+  for (int batch = 0; batch < nvec / K; ++batch) {
+    start_vector(n * K, S.seed ^ (0xD1B54A32D192ED03ULL * (uint64_t)(batch + 1)), r.data());
+    for (auto& x : r) x = x < 0.0 ? -1.0 : 1.0;
+    NMG_HIP(hipMemcpyAsync(v0.p, r.data(), sizeof(double) * n * K, hipMemcpyHostToDevice, S.s));
+    double* vk = S.bw[0].p;
+    double* vkm = S.bw[1].p;
+    k_copy(n * K, v0.p, vk, S.s);
+    k_fill(n * K, 0.0, S.bw[5].p, S.s);
+    k_block_dots(n, K, v0.p, vk, mom.p, work.p, S.s);
+    for (int k = 1; k <= mdeg; ++k) {
+      const double t = (k == 1 ? 1.0 : 2.0) / dd;
+      const double* in = vk;
+      if (S.geneig) {
+        block_bsolve(S, K, vk, S.bw[2].p, S.bw[3].p);
+        in = S.bw[2].p;
+      }
+      k_block_filter_step(K, A, in, vk, vkm, S.bw[5].p, zmu.p, t, cc, k == 1, S.s, false);
+      std::swap(vk, vkm);
+      k_block_dots(n, K, v0.p, vk, mom.p + (size_t)k * K, work.p, S.s);
+    }
+    NMG_HIP(hipMemcpyAsync(h.data(), mom.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, S.s));
+    NMG_HIP(hipStreamSynchronize(S.s));
+    for (int k = 0; k <= mdeg; ++k)
+      for (int c = 0; c < K; ++c) tau[k] += h[(size_t)k * K + c] / nvec;
+  }
 }
 
 static void do_residuals(Solver& S, double* res_ref, double* res2) {
@@ -1651,6 +1914,57 @@ int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, con
     NMG_HIP(hipMemcpyAsync(d_lam, gl.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipMemcpyAsync(d_mu, gm.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_kpmdos(nmg_handle h, int mdeg, int nvec, const double* intv, double* mu, double* ecnt) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(intv && mu, NMG_EINVAL, "null argument");
+    std::vector<double> tau;
+    do_kpm(*S, mdeg, nvec, intv[2], intv[3], tau);
+    std::memcpy(mu, tau.data(), sizeof(double) * (mdeg + 1));
+    if (ecnt) *ecnt = dos_count(mdeg, tau.data(), intv[2], intv[3], intv[0], intv[1]);
+  });
+}
+
+int nmg_dos_count(const double* mu, int mdeg, const double* intv, double* count) {
+  return guarded([&] {
+    NMG_CHECK(mu && intv && count && mdeg >= 1, NMG_EINVAL, "bad argument");
+    NMG_CHECK(intv[3] > intv[2], NMG_EINVAL, "lmax must exceed lmin");
+    *count = dos_count(mdeg, mu, intv[2], intv[3], intv[0], intv[1]);
+  });
+}
+
+int nmg_spslicer(double* sli, const double* mu, int mdeg, const double* intv, int n_slices) {
+  return guarded([&] {
+    NMG_CHECK(sli && mu && intv && mdeg >= 1, NMG_EINVAL, "bad argument");
+    NMG_CHECK(intv[3] > intv[2] && intv[1] > intv[0], NMG_EINVAL, "bad interval");
+    NMG_CHECK(n_slices >= 1 && n_slices <= 16, NMG_EINVAL, "between 1 and 16 slices");
+    const int rc = dos_slicer(mdeg, mu, intv[2], intv[3], intv[0], intv[1], n_slices, sli);
+    NMG_CHECK(rc == NMG_OK, NMG_ENUM, "the density estimate gives no increasing slice boundaries");
+  });
+}
+
+int nmg_solve_slices(nmg_handle h, int n_slices, const double* sli, double lmin, double lmax, double thr_int,
+                     double thr_ext, int maxit, double tol) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(sli != nullptr, NMG_EINVAL, "null slice boundaries");
+    NMG_CHECK(maxit >= 2 && tol > 0.0, NMG_EINVAL, "bad maxit/tol");
+    NMG_CHECK(lmax > lmin, NMG_EINVAL, "lmax must exceed lmin");
+    NMG_CHECK(thr_int > 0 && thr_int < 1 && thr_ext > 0 && thr_ext < 1, NMG_EINVAL,
+              "thresholds must lie in (0, 1)");
+    (void)S->A();
+    do_solve_slices(*S, n_slices, sli, lmin, lmax, thr_int, thr_ext, maxit, tol);
+  });
+}
+
+int nmg_slice_counts(nmg_handle h, int* counts) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(counts != nullptr, NMG_EINVAL, "null output");
+    for (size_t i = 0; i < S->slice_nev.size(); ++i) counts[i] = S->slice_nev[i];
   });
 }
 

@@ -210,4 +210,44 @@ void start_vector(int64_t n, uint64_t seed, double* v) {
   }
 }
 
+// Kernel polynomial method.  With t = (x - cc)/dd on [-1, 1] and
+//   sum_i delta(t - t_i) = 1/(pi sqrt(1-t^2)) sum_k (2 - [k=0]) T_k(t) tau_k,
+// tau_k = sum_i T_k(t_i), the Jackson-damped count in [a, b] integrates in
+// closed form through t = cos(theta):
+//   count = 1/pi sum_k (2 - [k=0]) g_k tau_k I_k,
+//   I_0 = theta_a - theta_b,  I_k = (sin k theta_a - sin k theta_b)/k,
+// g_k = ((M+2-k) cos(k al) + sin(k al) cot(al))/(M+2), al = pi/(M+2)
+// (Jackson kernel for M+1 moments; Weisse et al., Rev. Mod. Phys. 78, 2006).
+double dos_count(int mdeg, const double* tau, double lmin, double lmax, double a, double b) {
+  const double cc = 0.5 * (lmax + lmin), dd = 0.5 * (lmax - lmin);
+  auto theta = [&](double x) { return std::acos(std::min(1.0, std::max(-1.0, (x - cc) / dd))); };
+  const double ta = theta(a), tb = theta(b);
+  const double al = M_PI / (mdeg + 2);
+  double sum = tau[0] * (ta - tb);  // k = 0: g_0 = 1
+  for (int k = 1; k <= mdeg; ++k) {
+    const double g = ((mdeg + 2 - k) * std::cos(k * al) + std::sin(k * al) / std::tan(al)) / (mdeg + 2);
+    sum += 2.0 * g * tau[k] * (std::sin(k * ta) - std::sin(k * tb)) / k;
+  }
+  return sum / M_PI;
+}
+
+int dos_slicer(int mdeg, const double* tau, double lmin, double lmax, double a, double b, int ns, double* sli) {
+  if (ns < 1 || !(b > a)) return NMG_EINVAL;
+  const double total = dos_count(mdeg, tau, lmin, lmax, a, b);
+  sli[0] = a;
+  sli[ns] = b;
+  for (int i = 1; i < ns; ++i) {
+    const double target = total * i / ns;
+    double lo = sli[i - 1], hi = b;
+    for (int it = 0; it < 100; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (dos_count(mdeg, tau, lmin, lmax, a, mid) < target) lo = mid;
+      else hi = mid;
+    }
+    sli[i] = 0.5 * (lo + hi);
+    if (!(sli[i] > sli[i - 1])) return NMG_ENUM;
+  }
+  return NMG_OK;
+}
+
 }  // namespace nmg

@@ -132,6 +132,9 @@ void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk
                        double ith, hipStream_t s, bool fp32 = false);
 void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
                          const double* mu, double t, double cc, bool first, hipStream_t s, bool fp32 = false);
+// out[c] = sum_i x[i*K+c] y[i*K+c] for K in {1,2,4,8,16}; work: block_dot_work_size(K)
+size_t block_dot_work_size(int K);
+void k_block_dots(int64_t n, int K, const double* x, const double* y, double* out, double* work, hipStream_t s);
 void k_block_colscale(int64_t n, int K, const double* mu, const double* x, double* y, hipStream_t s);
 void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s);
 void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s);
@@ -159,5 +162,11 @@ int tridiag_eigvals(int m, const double* d, const double* e, std::vector<double>
 void tridiag_eigvec(int m, const double* d, const double* e, double theta, std::vector<double>& s);
 // Lanczos start vector (splitmix64 of seed and index, uniform [-1,1))
 void start_vector(int64_t n, uint64_t seed, double* v);
+// kernel polynomial (KPM) density of states from trace moments
+// tau_k ~ tr T_k((M - cc)/dd), k = 0..mdeg, with Jackson damping:
+// estimated number of eigenvalues in [a, b]
+double dos_count(int mdeg, const double* tau, double lmin, double lmax, double a, double b);
+// slice boundaries a = sli[0] < ... < sli[ns] = b holding equal estimated counts
+int dos_slicer(int mdeg, const double* tau, double lmin, double lmax, double a, double b, int ns, double* sli);
 
 }  // namespace nmg

@@ -317,6 +317,40 @@ void k_dot(int64_t n, const double* x, const double* y, double* out, bool sqrt_o
   NMG_HIP(hipGetLastError());
 }
 
+// K column dot products of two interleaved blocks (i*K + c):
+// out[c] = sum_i x[i*K + c] y[i*K + c].  The grid stride is a multiple of K,
+// so every thread stays on the column threadIdx.x % K.
+__global__ __launch_bounds__(kBlock) void block_dot_part_k(int64_t n, int K, const double* __restrict__ x,
+                                                           const double* __restrict__ y, double* part) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  const int64_t nk = n * K;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nk; t += (int64_t)gridDim.x * kBlock)
+    s += x[t] * y[t];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    double q = 0.0;
+    for (int j = threadIdx.x; j < kBlock; j += K) q += sh[j];
+    part[(int64_t)threadIdx.x * kDotBlocks + blockIdx.x] = q;
+  }
+}
+__global__ __launch_bounds__(kBlock) void block_dot_final_k(const double* part, double* out) {
+  __shared__ double sh[kBlock / 64];
+  const double* p = part + (int64_t)blockIdx.x * kDotBlocks;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += kBlock) s += p[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+size_t block_dot_work_size(int K) { return (size_t)K * kDotBlocks; }
+void k_block_dots(int64_t n, int K, const double* x, const double* y, double* out, double* work, hipStream_t s) {
+  if (K < 1 || K > 16 || (kBlock % K) != 0) throw Error{NMG_EINVAL, "block dots: K must divide the block size"};
+  hipLaunchKernelGGL(block_dot_part_k, dim3(kDotBlocks), dim3(kBlock), 0, s, n, K, x, y, work);
+  hipLaunchKernelGGL(block_dot_final_k, dim3(K), dim3(kBlock), 0, s, work, out);
+  NMG_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(kBlock) void resid_part_k(int64_t n, const double* __restrict__ ax,
                                                        const double* __restrict__ bx,
                                                        const double* __restrict__ x, double lam,

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import pevsl as pv
 
-__all__ = ["MatvecSetup", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
@@ -149,6 +149,9 @@ class EigResult:
     lan_steps_bounds: int = 0
     probe: dict = field(default_factory=dict)
     handle: int = 0   # solver handle kept alive with keep=True (caller finishes it)
+    slices: np.ndarray | None = None        # slice boundaries (apply_slices)
+    slice_counts: np.ndarray | None = None  # pairs per slice
+    dos_count: float | None = None          # KPM estimate of the count in the interval
 
     @property
     def freq_mhz(self):
@@ -159,19 +162,8 @@ class EigResult:
         return None if self.X is None else self.X * d[None, :]
 
 
-def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float | None = None,
-                interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
-                lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
-                rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
-                probe_every: int = 0, block: int = 1, resid_target: float | None = None,
-                keep: bool = False) -> EigResult:
-    """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
-    if interval is None:
-        a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
-    else:
-        a, b = interval
-    t = {}
-    t0 = time.perf_counter()
+def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None):
+    """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
         pv.pevsl_setprobsizes(h, mv.n, mv.n, -1)              # :57
@@ -193,6 +185,28 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
             resid_target = 1e-5
         if resid_target is not None:
             pv.set_residual_target(h, resid_target)
+    except BaseException:
+        pv.pevsl_finish(h)
+        raise
+    return h
+
+
+def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float | None = None,
+                interval=None, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
+                lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
+                rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
+                probe_every: int = 0, block: int = 1, resid_target: float | None = None,
+                keep: bool = False) -> EigResult:
+    """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
+    if interval is None:
+        a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
+    else:
+        a, b = interval
+    t = {}
+    t0 = time.perf_counter()
+    h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every, block=block,
+                     resid_target=resid_target)
+    try:
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         lmin, lmax = pv.pevsl_lanbounds(h, *lanbounds)        # :84
@@ -239,3 +253,84 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                      bar=pinfo.bar, lan_steps=st["lan_steps"], filter_applies=st["filter_applies"],
                      ncand=st["ncand"], timings=t, lan_steps_bounds=lb_steps, probe=probe,
                      handle=h if keep else 0)
+
+
+def kpm_degree(xintv, n_slices: int, lo: int = 50, hi: int = 3000) -> int:
+    """Degree of the KPM expansion that resolves n_slices slices of [a, b]:
+    the Jackson kernel is ~pi/M wide in theta = arccos((x - cc)/dd), so M is
+    chosen to give about two kernel widths per slice."""
+    a, b, lmin, lmax = xintv
+    cc, dd = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    th = lambda x: math.acos(min(1.0, max(-1.0, (x - cc) / dd)))
+    dth = abs(th(a) - th(b))
+    return int(min(hi, max(lo, math.ceil(2.0 * math.pi * n_slices / max(dth, 1e-12)))))
+
+
+def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm", kpm_deg: int | None = None,
+                 kpm_nvec: int = 16, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
+                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7, rtol: float = 1e-9,
+                 vectors: bool = True, residual_check: bool = True, probe_every: int = 0,
+                 resid_target: float | None = None, keep: bool = False) -> EigResult:
+    """The interval split into n_slices spectrum slices, solved in lockstep.
+
+    Slice boundaries: ``kpm`` places them at equal counts of the kernel
+    polynomial density of states (pEVSL kpmdos + spslicer); ``uniform``
+    splits the interval evenly.  Each slice runs ChebLanNr with its own
+    FindPol filter; every sparse product is shared by all slices.
+    """
+    a, b = interval
+    t = {}
+    t0 = time.perf_counter()
+    h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every,
+                     resid_target=resid_target)
+    try:
+        t["setup"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lmin, lmax = pv.pevsl_lanbounds(h, *lanbounds)
+        t["lanbounds"] = time.perf_counter() - t0
+        lb_steps = pv.solve_stats(h)["lan_steps"]
+        if a < 1.0e-10:
+            a = lmin
+        xintv = np.array([a, b, lmin, lmax])
+        t0 = time.perf_counter()
+        ecnt = None
+        if slicer == "kpm":
+            mdeg = kpm_deg or kpm_degree(xintv, n_slices)
+            mu, ecnt = pv.kpmdos(h, mdeg, kpm_nvec, xintv)
+            sli = pv.spslicer(mu, xintv, n_slices)
+        elif slicer == "uniform":
+            sli = np.linspace(a, b, n_slices + 1)
+        else:
+            raise ValueError(slicer)
+        t["slicer"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nev = EVINT + 2
+        maxit = 3 * max(4 * nev, 2000)
+        pv.solve_slices(h, sli, lmin, lmax, thr_int, thr_ext, maxit, tol)
+        t["slices"] = time.perf_counter() - t0
+        st = pv.solve_stats(h)
+        counts = pv.slice_counts(h, n_slices)
+        probe = {}
+        if probe_every > 0:
+            for key, which in (("chebstep", 0), ("filterstep", 1)):
+                sec, ns, by = pv.kernel_probe(h, which)
+                probe[key] = dict(seconds=sec, samples=ns, bytes=by)
+        t0 = time.perf_counter()
+        if residual_check:
+            r1, r2 = pv.residuals(h)
+        else:
+            k = pv.pevsl_get_nev(h)
+            r1 = r2 = np.full(k, np.nan)
+        t["residual"] = time.perf_counter() - t0
+        lam, X = pv.pevsl_copy_result(h, mv.n, vectors)
+    except BaseException:
+        pv.pevsl_finish(h)
+        raise
+    if not keep:
+        pv.pevsl_finish(h)
+    order = np.argsort(lam, kind="stable")
+    return EigResult(lam=lam[order], X=None if X is None else X[order], res_ref=r1[order], res2=r2[order],
+                     lmin=lmin, lmax=lmax, xintv=xintv, deg=0, bar=float("nan"), lan_steps=st["lan_steps"],
+                     filter_applies=st["filter_applies"], ncand=st["ncand"], timings=t,
+                     lan_steps_bounds=lb_steps, probe=probe, handle=h if keep else 0, slices=sli,
+                     slice_counts=counts, dos_count=ecnt)

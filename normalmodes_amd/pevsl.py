@@ -21,7 +21,8 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
-           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "kpmdos", "dos_count", "spslicer",
+           "solve_slices", "slice_counts", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -269,6 +270,48 @@ def solve_stats(h: Handle):
     _check(_lib().nmg_solve_stats(_i64(h), ctypes.byref(ls), ctypes.byref(fa), ctypes.byref(nc),
                                   ctypes.byref(sec)))
     return dict(lan_steps=ls.value, filter_applies=fa.value, ncand=nc.value, seconds=sec.value)
+
+
+def kpmdos(h: Handle, mdeg: int, nvec: int, intv):
+    """Trace moments mu[0..mdeg] of T_k((A~B~^-1 - cc)/dd) and the estimated
+    eigenvalue count in [intv[0], intv[1]] (pEVSL kpmdos)."""
+    x = _f64(intv)
+    mu = np.empty(mdeg + 1)
+    ecnt = _d()
+    _check(_lib().nmg_kpmdos(_i64(h), ctypes.c_int(mdeg), ctypes.c_int(nvec), _p(x, _f64p), _p(mu, _f64p),
+                             ctypes.byref(ecnt)))
+    return mu, ecnt.value
+
+
+def dos_count(mu, intv) -> float:
+    m = _f64(mu)
+    x = _f64(intv)
+    c = _d()
+    _check(_lib().nmg_dos_count(_p(m, _f64p), ctypes.c_int(m.size - 1), _p(x, _f64p), ctypes.byref(c)))
+    return c.value
+
+
+def spslicer(mu, intv, n_slices: int) -> np.ndarray:
+    """Slice boundaries with equal estimated counts (pEVSL spslicer)."""
+    m = _f64(mu)
+    x = _f64(intv)
+    sli = np.empty(n_slices + 1)
+    _check(_lib().nmg_spslicer(_p(sli, _f64p), _p(m, _f64p), ctypes.c_int(m.size - 1), _p(x, _f64p),
+                               ctypes.c_int(n_slices)))
+    return sli
+
+
+def solve_slices(h: Handle, sli, lmin: float, lmax: float, thr_int: float = 0.8, thr_ext: float = 0.7,
+                 maxit: int = 9624, tol: float = 1e-5):
+    s = _f64(sli)
+    _check(_lib().nmg_solve_slices(_i64(h), ctypes.c_int(s.size - 1), _p(s, _f64p), _d(lmin), _d(lmax),
+                                   _d(thr_int), _d(thr_ext), ctypes.c_int(maxit), _d(tol)))
+
+
+def slice_counts(h: Handle, n_slices: int) -> np.ndarray:
+    c = np.zeros(n_slices, dtype=np.int32)
+    _check(_lib().nmg_slice_counts(_i64(h), _p(c, _i32p)))
+    return c
 
 
 def set_residual_target(h: Handle, target: float):

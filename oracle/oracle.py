@@ -16,6 +16,8 @@ Parity pins (tests/test_oracle_*.py):
   lanbounds            enclosure of the dense spectrum
   cheblannr            dense scipy.linalg.eigh(A, B) on the same pencil
                        (eigenvalues to 1e-12 relative, same count)
+  kpm_count            quadrature of the damped density vs the closed-form
+                       integral; exact moments of a known spectrum
   FindPol's exact degree rule, ChebLanNr's test cadence and LanBounds'
   stopping rule are restated from the published EVSL/pEVSL design and are
   "parity unpinned" against pEVSL itself (not vendored, cannot be run).
@@ -350,6 +352,56 @@ def solve_interval(A, B, xintv_ab, *, bdeg=45, thr_int=0.8, thr_ext=0.7, lb=(300
     if return_info:
         return lam, X, info
     return lam, X
+
+
+def jackson(mdeg):
+    """Jackson damping factors g_k, k = 0..mdeg (Weisse et al., Rev. Mod.
+    Phys. 78, 275 (2006), eq. 71, with N = mdeg + 1 moments)."""
+    N = mdeg + 1
+    k = np.arange(N)
+    a = np.pi / (N + 1)
+    return ((N - k + 1) * np.cos(k * a) + np.sin(k * a) / np.tan(a)) / (N + 1)
+
+
+def kpm_count(tau, lmin, lmax, a, b, npts=200001):
+    """Jackson-damped KPM eigenvalue count in [a, b] by trapezoidal quadrature
+    of the density in theta (t = cos theta), independent of a closed-form
+    integral:  count = 1/pi int_{theta_b}^{theta_a} sum_k (2-[k=0]) g_k tau_k cos(k theta) dtheta."""
+    tau = np.asarray(tau, dtype=float)
+    cc, dd = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    th = lambda x: np.arccos(np.clip((x - cc) / dd, -1.0, 1.0))
+    ta, tb = th(a), th(b)
+    t = np.linspace(tb, ta, npts)
+    g = jackson(tau.size - 1)
+    w = g * tau
+    w[1:] *= 2.0
+    dens = np.polynomial.chebyshev.chebval(np.cos(t), w)  # sum_k w_k cos(k t) = sum_k w_k T_k(cos t)
+    return float(np.trapezoid(dens, t) / np.pi)
+
+
+def rademacher_block(n, K, seed, batch):
+    """The K Rademacher vectors of batch `batch` used for the KPM trace
+    estimate (splitmix64 stream of the Lanczos start vector, signs), as an
+    (n, K) array."""
+    s = (seed ^ ((0xD1B54A32D192ED03 * (batch + 1)) & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+    r = start_vector(n * K, s)
+    return np.where(r < 0.0, -1.0, 1.0).reshape(n, K)
+
+
+def kpm_moments_dense(At, Bt, lmin, lmax, mdeg, V):
+    """tau_k = mean_c v_c^T T_k((At Bt^-1 - cc)/dd) v_c with a dense solve."""
+    import scipy.linalg as sl
+    cc, dd = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    C = sl.cho_factor(Bt)
+    op = lambda X: (At @ sl.cho_solve(C, X) - cc * X) / dd
+    tau = np.empty(mdeg + 1)
+    Tm, T = V, op(V)
+    tau[0] = np.sum(V * V) / V.shape[1]
+    tau[1] = np.sum(V * T) / V.shape[1]
+    for k in range(2, mdeg + 1):
+        Tm, T = T, 2.0 * op(T) - Tm
+        tau[k] = np.sum(V * T) / V.shape[1]
+    return tau
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")

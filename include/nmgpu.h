@@ -77,6 +77,16 @@ int nmg_csr_diagscale(nmg_handle mat, double sign, double *d);
 /* PEVSL_SETAMV_F90(h, sparseAV, mymatvec): w = D A D v  src/mod_pevsl.f90:77,
  * src/mod_matvec.f90:445-458.  A is used through a scaled copy D A D. */
 int nmg_set_amv_dad(nmg_handle h, nmg_handle matA, const double *d);
+
+/* sparsefsAV, src/mod_matvec.f90:498-520 (PEVSL_SETAMV_F90(pevslAB,
+ * sparsefsAV, mymatvec), src/mod_pevsl.f90:77-80): the fluid-solid operator
+ *   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))] = D (Ad - E Ap^-1 ET) D v,
+ * C_p = the Chebyshev solve chebAp (nmg_setup_chebiter on Ap~ = Dp (-Ap) Dp,
+ * the matrix scaled by nmg_csr_diagscale(matAp, -1, dp) as Apdiagscaling,
+ * src/mod_matvec.f90:345-441).  matAd: n x n; matE: n x np; matET: np x n
+ * (src/mod_matvec.f90:98-222).  d: the B scaling D (n), dp: Dp (np). */
+int nmg_set_amv_fs(nmg_handle h, nmg_handle matAd, nmg_handle matE, nmg_handle matET, nmg_handle chebAp,
+                   const double *d, const double *dp);
 /* PEVSL_SETAMV_F90(h, sparseBV|sparseApV, ...) with a plain matrix:
  * w = M v  (used for bounds of B~, src/mod_matvec.f90:80, and of Ap~ :157) */
 int nmg_set_amv(nmg_handle h, nmg_handle mat);

@@ -424,6 +424,20 @@ struct DVec {
   ~DVec() { dfree(p); }
 };
 
+// Fluid-solid operator of sparsefsAV (src/mod_matvec.f90:498-520):
+//   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))],  C_p ~ Ap~^-1 by Chebyshev
+// iteration, Ap~ = Dp (-Ap) Dp (Apdiagscaling, src/mod_matvec.f90:345-441);
+// i.e. D (Ad - E Ap^-1 ET) D, the displacement Schur complement.
+struct FsOp {
+  DevMat* Ad = nullptr;  // n x n
+  DevMat* E = nullptr;   // n x np
+  DevMat* ET = nullptr;  // np x n
+  Cheb* cp = nullptr;    // Chebyshev solve with Ap~ (np x np)
+  int64_t n = 0, np = 0;
+  DVec d, dp;
+  DVec t[3], p[3];  // work: n*K and np*K
+};
+
 struct Solver {
   int device = 0;
   hipStream_t s = nullptr;
@@ -460,6 +474,8 @@ struct Solver {
   int ncand = 0;
   double seconds = 0;
 
+  std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
+  DVec aw;                   // product A~ in of the unfused filter step (n*K)
   bool borrowed = false;  // stream and rocBLAS handle belong to a parent solver
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
 
@@ -482,8 +498,14 @@ struct Solver {
     NMG_CHECK(amat != nullptr, NMG_ESTATE, "no A operator set");
     return *amat;
   }
+  void check_A() const {
+    if (!fs) (void)A();
+  }
   bool fp32() const { return precision == NMG_FP32_FILTER; }
 };
+
+static void apply_A_block(Solver& S, int K, const double* x, double* y);
+static void apply_A(Solver& S, const double* x, double* y);
 
 // grow the Lanczos basis to hold `need` columns (copying the first `keep`)
 static void ensure_basis(Solver& S, int need, int keep, int maxcap) {
@@ -552,7 +574,7 @@ static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double
   double* b2 = S.w[3].p;
   k_copy(n, z, vk, S.s);
   k_scale_copy(n, P.mu[0], vk, y, S.s);
-  const DevMat& A = S.A();
+  if (S.fs) S.w[7].ensure(n);
   for (int k = 1; k <= P.deg; ++k) {
     const double t = (k == 1 ? 1.0 : 2.0) / P.dd;
     const double* in = vk;
@@ -560,10 +582,15 @@ static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double
       bsolve(S, vk, b1, b2);
       in = b1;
     }
-    const bool pr = S.probe[1].armed();
-    if (pr) S.probe[1].before(S.s);
-    k_filter_step(A, in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, false);
-    if (pr) S.probe[1].after(S.s);
+    if (S.fs) {
+      apply_A(S, in, S.w[7].p);
+      k_filter_update(n, 1, S.w[7].p, vk, vkm, y, nullptr, P.mu[k], t, P.cc, k == 1, S.s);
+    } else {
+      const bool pr = S.probe[1].armed();
+      if (pr) S.probe[1].before(S.s);
+      k_filter_step(S.A(), in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, false);
+      if (pr) S.probe[1].after(S.s);
+    }
     std::swap(vk, vkm);
   }
   S.filter_applies++;
@@ -614,7 +641,7 @@ static void lanczos_normalise(Solver& S, int k) {
 static void lanczos_step(Solver& S, int k, int op, const PolyFilter* P, double* hwork) {
   const int64_t n = S.n;
   double* zn = S.Z + (size_t)(k + 1) * n;
-  if (op == 0) k_spmv(S.A(), S.V + (size_t)k * n, zn, S.s);
+  if (op == 0) apply_A(S, S.V + (size_t)k * n, zn);
   else filter_apply(S, *P, S.Z + (size_t)k * n, zn);
   lanczos_orth(S, k, hwork);
   if (S.geneig) bsolve(S, zn, S.V + (size_t)(k + 1) * n, S.w[4].p);
@@ -736,7 +763,7 @@ static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vecto
     AY.ensure(n * nc);
     BY.ensure(n * nc);
     for (int j = 0; j < nc; ++j) {
-      k_spmv(S.A(), Y.p + (size_t)j * n, AY.p + (size_t)j * n, S.s);
+      apply_A(S, Y.p + (size_t)j * n, AY.p + (size_t)j * n);
       apply_B(S, Y.p + (size_t)j * n, BY.p + (size_t)j * n);
     }
     GA.ensure((int64_t)nc * nc);
@@ -933,27 +960,78 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
 // blocks R_{j+1}.  Candidates are the Ritz values of T >= bar; Ritz residual
 // of candidate s: ||R_{j+1} s_last||.
 
-// y (interleaved n*K) ~ B~^{-1} b, all K columns at once
-static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
-  const Cheb& C = *S.bsol;
-  const int64_t nk = S.n * K;
+// K interleaved columns of x ~ M^{-1} b by the Chebyshev semi-iteration of
+// cheb_solve, one K-column fused step per degree
+static void block_cheb(const Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
+                       KernelProbe* probe) {
+  const int64_t nk = C.mat->nrow * K;
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;
-  k_scale_copy(nk, 1.0 / th, b, xk, S.s);
-  k_fill(nk, 0.0, xm, S.s);
+  k_scale_copy(nk, 1.0 / th, b, xk, s);
+  k_fill(nk, 0.0, xm, s);
   double w = 2.0;
-  KernelProbe* probe = &S.probe[0];
   for (int k = 1; k <= C.deg; ++k) {
     w = 1.0 / (1.0 - w / (4.0 * sg * sg));
-    const bool pr = probe->armed();
-    if (pr) probe->before(S.s);
-    k_block_cheb_step(K, *C.mat, b, xk, xm, w, 1.0 / th, S.s, S.fp32());
-    if (pr) probe->after(S.s);
+    const bool pr = probe && probe->armed();
+    if (pr) probe->before(s);
+    k_block_cheb_step(K, *C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    if (pr) probe->after(s);
     std::swap(xk, xm);
   }
+}
+
+// y (interleaved n*K) ~ B~^{-1} b, all K columns at once
+static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
+  NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
+  block_cheb(*S.bsol, K, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
   S.bsol->ncalls += K;
-  S.bsol->nmatvec += (int64_t)C.deg * K;
+  S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
+}
+
+// A~ applied to K interleaved columns (K = 1: one vector): the sparse product,
+// or the fluid-solid Schur operator
+static void apply_A_block(Solver& S, int K, const double* x, double* y) {
+  if (!S.fs) {
+    if (K == 1) k_spmv(S.A(), x, y, S.s);
+    else k_block_spmv(K, S.A(), x, y, S.s);
+    return;
+  }
+  FsOp& F = *S.fs;
+  for (int i = 0; i < 3; ++i) {
+    F.t[i].ensure(F.n * K);
+    F.p[i].ensure(F.np * K);
+  }
+  double *v0 = F.t[0].p, *w0 = F.t[1].p, *w1 = F.t[2].p;
+  double *x0 = F.p[0].p, *y0 = F.p[1].p, *tp = F.p[2].p;
+  k_rowmul(F.n, K, F.d.p, x, v0, S.s);        // v0 = D v
+  k_block_spmv(K, *F.Ad, v0, w0, S.s);        // w0 = Ad v0
+  k_block_spmv(K, *F.ET, v0, x0, S.s);        // x0 = ET v0
+  k_rowmul(F.np, K, F.dp.p, x0, tp, S.s);     // x1 = Dp x0
+  block_cheb(*F.cp, K, tp, y0, x0, S.s, false, nullptr);  // y0 ~ Ap~^-1 x1
+  k_rowmul(F.np, K, F.dp.p, y0, tp, S.s);     // y1 = Dp y0
+  k_block_spmv(K, *F.E, tp, w1, S.s);         // w1 = E y1
+  k_add_rowmul(F.n, K, F.d.p, w0, w1, y, S.s);  // w = D (w0 + w1)
+  F.cp->ncalls += K;
+  F.cp->nmatvec += (int64_t)F.cp->deg * K;
+}
+static void apply_A(Solver& S, const double* x, double* y) { apply_A_block(S, 1, x, y); }
+
+// one Chebyshev filter step on K interleaved columns: vkm <- t (A~ in - cc vk)
+// - vkm, y += mu vkm (fused into the sparse product's epilogue, or after the
+// fluid-solid operator)
+static void block_filter_step(Solver& S, int K, const double* in, const double* vk, double* vkm, double* y,
+                              const double* mu_k, double t, double cc, bool first) {
+  if (S.fs) {
+    S.aw.ensure(S.n * K);
+    apply_A_block(S, K, in, S.aw.p);
+    k_filter_update(S.n, K, S.aw.p, vk, vkm, y, mu_k, 0.0, t, cc, first, S.s);
+    return;
+  }
+  const bool pr = S.probe[1].armed();
+  if (pr) S.probe[1].before(S.s);
+  k_block_filter_step(K, S.A(), in, vk, vkm, y, mu_k, t, cc, first, S.s, false);
+  if (pr) S.probe[1].after(S.s);
 }
 
 // y = p((A B^-1 - cc)/dd) z for K interleaved columns; mu_dev: (deg+1)*K
@@ -967,7 +1045,6 @@ static void block_filter(Solver& S, int K, const PolyFilter& P, const double* mu
   double* b2 = S.bw[3].p;
   k_copy(nk, z, vk, S.s);
   k_block_colscale(S.n, K, mu_dev, vk, y, S.s);
-  const DevMat& A = S.A();
   for (int k = 1; k <= P.deg; ++k) {
     const double t = (k == 1 ? 1.0 : 2.0) / P.dd;
     const double* in = vk;
@@ -975,10 +1052,7 @@ static void block_filter(Solver& S, int K, const PolyFilter& P, const double* mu
       block_bsolve(S, K, vk, b1, b2);
       in = b1;
     }
-    const bool pr = S.probe[1].armed();
-    if (pr) S.probe[1].before(S.s);
-    k_block_filter_step(K, A, in, vk, vkm, y, mu_dev + (size_t)k * K, t, P.cc, k == 1, S.s, false);
-    if (pr) S.probe[1].after(S.s);
+    block_filter_step(S, K, in, vk, vkm, y, mu_dev + (size_t)k * K, t, P.cc, k == 1);
     std::swap(vk, vkm);
   }
   S.filter_applies += K;
@@ -1297,7 +1371,8 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
     C->s = S.s;
     C->blas = S.blas;
     C->n = n;
-    C->amat = const_cast<DevMat*>(&S.A());
+    C->fs = S.fs;
+    C->amat = S.fs ? nullptr : const_cast<DevMat*>(&S.A());
     C->bmat = S.bmat;
     C->bsol = S.bsol;
     C->geneig = S.geneig;
@@ -1450,7 +1525,6 @@ static void do_kpm(Solver& S, int mdeg, int nvec, double lmin, double lmax, std:
   k_fill(K, 0.0, zmu.p, S.s);
   tau.assign(mdeg + 1, 0.0);
   std::vector<double> h((size_t)(mdeg + 1) * K), r((size_t)n * K);
-  const DevMat& A = S.A();
   for (int batch = 0; batch < nvec / K; ++batch) {
     start_vector(n * K, S.seed ^ (0xD1B54A32D192ED03ULL * (uint64_t)(batch + 1)), r.data());
     for (auto& x : r) x = x < 0.0 ? -1.0 : 1.0;
@@ -1467,7 +1541,7 @@ static void do_kpm(Solver& S, int mdeg, int nvec, double lmin, double lmax, std:
         block_bsolve(S, K, vk, S.bw[2].p, S.bw[3].p);
         in = S.bw[2].p;
       }
-      k_block_filter_step(K, A, in, vk, vkm, S.bw[5].p, zmu.p, t, cc, k == 1, S.s, false);
+      block_filter_step(S, K, in, vk, vkm, S.bw[5].p, zmu.p, t, cc, k == 1);
       std::swap(vk, vkm);
       k_block_dots(n, K, v0.p, vk, mom.p + (size_t)k * K, work.p, S.s);
     }
@@ -1485,7 +1559,7 @@ static void do_residuals(Solver& S, double* res_ref, double* res2) {
   out.ensure(2 * std::max(S.nev, 1));
   for (int j = 0; j < S.nev; ++j) {
     const double* x = S.X + (size_t)j * n;
-    k_spmv(S.A(), x, S.w[5].p, S.s);
+    apply_A(S, x, S.w[5].p);
     apply_B(S, x, S.w[6].p);
     k_resid(n, S.w[5].p, S.w[6].p, x, S.lam[j], out.p + 2 * j, S.red.p, S.s);
   }
@@ -1667,9 +1741,43 @@ int nmg_set_amv_dad(nmg_handle h, nmg_handle matA, const double* d) {
     NMG_HIP(hipStreamSynchronize(S->s));
     S->adad = std::move(C);
     S->amat = nullptr;
+    S->fs.reset();
     S->dscale.ensure(S->n);
     NMG_HIP(hipMemcpyAsync(S->dscale.p, d, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_set_amv_fs(nmg_handle h, nmg_handle matAd, nmg_handle matE, nmg_handle matET, nmg_handle chebAp,
+                   const double* d, const double* dp) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    auto F = std::make_shared<FsOp>();
+    F->Ad = get<DevMat>(matAd, K_MAT);
+    F->E = get<DevMat>(matE, K_MAT);
+    F->ET = get<DevMat>(matET, K_MAT);
+    F->cp = get<Cheb>(chebAp, K_CHEB);
+    const int64_t n = S->n, np = F->cp->mat->nrow;
+    NMG_CHECK(n > 0 && F->Ad->nrow == n && F->Ad->ncol == n, NMG_EINVAL, "Ad must be n x n");
+    NMG_CHECK(F->E->nrow == n && F->E->ncol == np, NMG_EINVAL, "E must be n x np (np = rows of Ap)");
+    NMG_CHECK(F->ET->nrow == np && F->ET->ncol == n, NMG_EINVAL, "ET must be np x n");
+    NMG_CHECK(d != nullptr && dp != nullptr, NMG_EINVAL, "null scaling vector");
+    for (int64_t i = 0; i < n; ++i)
+      NMG_CHECK(std::isfinite(d[i]) && d[i] > 0.0, NMG_EINVAL, "scaling vector must be positive");
+    for (int64_t i = 0; i < np; ++i)
+      NMG_CHECK(std::isfinite(dp[i]) && dp[i] > 0.0, NMG_EINVAL, "pressure scaling vector must be positive");
+    F->n = n;
+    F->np = np;
+    F->d.ensure(n);
+    F->dp.ensure(np);
+    NMG_HIP(hipMemcpyAsync(F->d.p, d, sizeof(double) * n, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipMemcpyAsync(F->dp.p, dp, sizeof(double) * np, hipMemcpyHostToDevice, S->s));
+    S->dscale.ensure(n);
+    NMG_HIP(hipMemcpyAsync(S->dscale.p, d, sizeof(double) * n, hipMemcpyHostToDevice, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
+    S->fs = std::move(F);
+    S->adad.reset();
+    S->amat = nullptr;
   });
 }
 
@@ -1680,6 +1788,7 @@ int nmg_set_amv(nmg_handle h, nmg_handle mat) {
     NMG_CHECK(S->n > 0 && M->nrow == S->n && M->ncol == S->n, NMG_EINVAL, "operator size mismatch");
     S->amat = M;
     S->adad.reset();
+    S->fs.reset();
   });
 }
 
@@ -1844,7 +1953,7 @@ int nmg_cheblannr(nmg_handle h, const double* xintv, int maxit, double tol, nmg_
     PolyFilter* P = get<PolyFilter>(pol, K_POL);
     NMG_CHECK(xintv != nullptr, NMG_EINVAL, "null interval");
     NMG_CHECK(maxit >= 2 && tol > 0.0, NMG_EINVAL, "bad maxit/tol");
-    (void)S->A();
+    S->check_A();
     if (S->block > 1) do_cheblannr_block(*S, xintv, maxit, tol, *P);
     else do_cheblannr(*S, xintv, maxit, tol, *P);
   });
@@ -1884,6 +1993,7 @@ int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, con
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(S->nev > 0 && S->X, NMG_ESTATE, "no eigenpairs: run nmg_cheblannr first");
     NMG_CHECK(S->dscale.n >= S->n, NMG_ESTATE, "the operator was not set with nmg_set_amv_dad");
+    NMG_CHECK(!S->fs, NMG_ENOTIMPL, "sensitivities of fluid-solid bodies are not available");
     NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "np must be 4 (P1) or 10 (P2)");
     NMG_CHECK(job == 1, NMG_ENOTIMPL, "sensitivities with reference gravity (JOB=2) are not available");
     NMG_CHECK(n_elem > 0 && elems && detJ && invJ && refM && refD && d_rho && d_lam && d_mu, NMG_EINVAL,
@@ -1955,7 +2065,7 @@ int nmg_solve_slices(nmg_handle h, int n_slices, const double* sli, double lmin,
     NMG_CHECK(lmax > lmin, NMG_EINVAL, "lmax must exceed lmin");
     NMG_CHECK(thr_int > 0 && thr_int < 1 && thr_ext > 0 && thr_ext < 1, NMG_EINVAL,
               "thresholds must lie in (0, 1)");
-    (void)S->A();
+    S->check_A();
     do_solve_slices(*S, n_slices, sli, lmin, lmax, thr_int, thr_ext, maxit, tol);
   });
 }
@@ -2023,7 +2133,8 @@ int nmg_kernel_probe(nmg_handle h, int which, double* mean_seconds, int64_t* sam
         NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
         *bytes_per_launch = S->bsol->mat->matrix_bytes(f32) + 4.0 * 8.0 * nk;
       } else {
-        *bytes_per_launch = S->A().matrix_bytes(false) + 6.0 * 8.0 * nk;
+        // the fluid-solid operator is not one fused launch: no figure
+        *bytes_per_launch = S->fs ? 0.0 : S->A().matrix_bytes(false) + 6.0 * 8.0 * nk;
       }
     }
   });
@@ -2055,7 +2166,10 @@ int nmg_apply(nmg_handle h, int which, const double* x, double* y) {
     dx.ensure(S->n);
     dy.ensure(S->n);
     NMG_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
-    if (which == 0) k_spmv(S->A(), dx.p, dy.p, S->s, S->fp32());
+    if (which == 0) {
+      if (S->fs) apply_A(*S, dx.p, dy.p);
+      else k_spmv(S->A(), dx.p, dy.p, S->s, S->fp32());
+    }
     else if (which == 1) apply_B(*S, dx.p, dy.p);
     else {
       NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
@@ -2082,7 +2196,10 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double* X, double* Y) 
     cm.ensure(n * K);
     NMG_HIP(hipMemcpyAsync(cm.p, X, sizeof(double) * n * K, hipMemcpyHostToDevice, S->s));
     k_to_interleaved(n, K, cm.p, n, S->bw[4].p, S->s);
-    if (which == 0) k_block_spmv(K, S->A(), S->bw[4].p, S->bw[5].p, S->s, S->fp32());
+    if (which == 0) {
+      if (S->fs) apply_A_block(*S, K, S->bw[4].p, S->bw[5].p);
+      else k_block_spmv(K, S->A(), S->bw[4].p, S->bw[5].p, S->s, S->fp32());
+    }
     else if (which == 1) {
       NMG_CHECK(S->bmat != nullptr, NMG_ESTATE, "no B set");
       k_block_spmv(K, *S->bmat, S->bw[4].p, S->bw[5].p, S->s);

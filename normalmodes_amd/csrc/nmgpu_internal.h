@@ -108,7 +108,15 @@ void k_filter_step(const DevMat& A, const double* x, const double* vk, double* v
 void k_fill(int64_t n, double v, double* y, hipStream_t s);
 void k_copy(int64_t n, const double* x, double* y, hipStream_t s);
 void k_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);  // y = a x
-void k_mul(int64_t n, const double* d, double* x, hipStream_t s);                  // x *= d
+void k_mul(int64_t n, const double* d, double* x, hipStream_t s);
+// interleaved K-column row scalings (K = 1: plain vectors)
+void k_rowmul(int64_t n, int K, const double* d, const double* x, double* y, hipStream_t s);        // y = d o x
+void k_add_rowmul(int64_t n, int K, const double* d, const double* a, const double* b, double* y,
+                  hipStream_t s);                                                                    // y = d o (a + b)
+// unfused Chebyshev filter update from a product w = A in: v = t (w - cc vk) - vkm,
+// vkm <- v, y += mu v (mu: K per-column coefficients, or null for the scalar mu0)
+void k_filter_update(int64_t n, int K, const double* w, const double* vk, double* vkm, double* y,
+                     const double* mu, double mu0, double t, double cc, bool first, hipStream_t s);                  // x *= d
 // y -= (*a) x  (a device scalar), optionally y -= (*b) z
 void k_axpy_dev(int64_t n, const double* a, const double* x, double* y, hipStream_t s);
 // x *= 1 / *beta ; z *= 1 / *beta (z may be null)

@@ -288,6 +288,46 @@ void k_invscale_dev(int64_t n, const double* beta, double* x, double* z, hipStre
   NMG_HIP(hipGetLastError());
 }
 
+// row scalings and the unfused filter update (operators that are not one
+// sparse product, e.g. the fluid-solid Schur operator) ----------------------
+// y[i*K + c] = d[i] * x[i*K + c]   (K = 1: y = d o x)
+__global__ void rowmul_k(int64_t n, int K, const double* __restrict__ d, const double* __restrict__ x,
+                         double* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock)
+    y[t] = d[t / K] * x[t];
+}
+// y[i*K + c] = d[i] * (a[i*K + c] + b[i*K + c])
+__global__ void add_rowmul_k(int64_t n, int K, const double* __restrict__ d, const double* __restrict__ a,
+                             const double* __restrict__ b, double* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock)
+    y[t] = d[t / K] * (a[t] + b[t]);
+}
+// v = t (w - cc vk) - [!first] vkm;  vkm <- v;  y += mu[c] v
+__global__ void filter_update_k(int64_t n, int K, const double* __restrict__ w, const double* __restrict__ vk,
+                                double* __restrict__ vkm, double* __restrict__ y, const double* __restrict__ mu,
+                                double mu0, double t, double cc, int first) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * K; i += (int64_t)gridDim.x * kBlock) {
+    const double v = t * (w[i] - cc * vk[i]) - (first ? 0.0 : vkm[i]);
+    vkm[i] = v;
+    y[i] += (mu ? mu[i % K] : mu0) * v;
+  }
+}
+void k_rowmul(int64_t n, int K, const double* d, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(rowmul_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, d, x, y);
+  NMG_HIP(hipGetLastError());
+}
+void k_add_rowmul(int64_t n, int K, const double* d, const double* a, const double* b, double* y,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(add_rowmul_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, d, a, b, y);
+  NMG_HIP(hipGetLastError());
+}
+void k_filter_update(int64_t n, int K, const double* w, const double* vk, double* vkm, double* y,
+                     const double* mu, double mu0, double t, double cc, bool first, hipStream_t s) {
+  hipLaunchKernelGGL(filter_update_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, w, vk, vkm, y, mu, mu0, t,
+                     cc, first ? 1 : 0);
+  NMG_HIP(hipGetLastError());
+}
+
 // deterministic two-stage reductions -----------------------------------------
 __global__ __launch_bounds__(kBlock) void dot_part_k(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ y, double* part) {

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import pevsl as pv
 
-__all__ = ["MatvecSetup", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
@@ -68,12 +68,96 @@ class MatvecSetup:
     porder: int
     timings: dict = field(default_factory=dict)
     tuned: dict = field(default_factory=dict)
+    fs: "FluidSetup | None" = None   # fluid-solid operator (A unused when set)
 
     def free(self):
         if self.chebB:
             pv.pevsl_chebiter_free(self.chebB)
-        pv.csr_free(self.matA)
+        if self.matA:
+            pv.csr_free(self.matA)
         pv.csr_free(self.matB)
+        if self.fs is not None:
+            self.fs.free()
+
+
+@dataclass
+class FluidSetup:
+    """The fluid-solid handles of ``mymatvec`` (sAdV, sEV, sETV, sApV, chebAp;
+    src/mod_cg_datatype.f90:92-96) and the pressure scaling Dp."""
+    matAd: int
+    matE: int
+    matET: int
+    matAp: int
+    dp: np.ndarray
+    aplmin: float = 0.0
+    aplmax: float = 0.0
+    apdeg: int = 0
+    chebAp: int = 0
+
+    def free(self):
+        if self.chebAp:
+            pv.pevsl_chebiter_free(self.chebAp)
+        for m in (self.matAd, self.matE, self.matET, self.matAp):
+            pv.csr_free(m)
+
+
+@dataclass
+class FluidProblem:
+    """A fluid-solid pencil in the reference's blocks (CGM%Ad, %E, %ET, %Ap, %B):
+    the operator is Ad - E Ap^-1 ET with Ap negative definite."""
+    Ad: object
+    E: object
+    ET: object
+    Ap: object
+    B: object
+    porder: int = 2
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return self.Ad.n_rows
+
+    @property
+    def n_p(self) -> int:
+        return self.Ap.n_rows
+
+
+def ap_chebiter_degree(porder: int, lmin: float, lmax: float) -> int:
+    """Degree of the Chebyshev solve with Ap~ (src/mod_matvec.f90:166-171):
+    P2: max(min(int(lmax/lmin), 15)*3, 100); P1: 25."""
+    if porder == 2:
+        return max(min(int(lmax / lmin), 15) * 3, 100)
+    return 25
+
+
+def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bounds=(1000, 2000, 1e-12),
+                   ap_bounds=(2000, 3000, 1e-12)) -> MatvecSetup:
+    """setupmatvec for a body with fluid regions (src/mod_matvec.f90:24-222):
+    B scaled and its Chebyshev solve as in the solid case; Ad, E, ET uploaded;
+    Ap negated and scaled (Apdiagscaling), its bounds by LanBounds and its
+    Chebyshev solve."""
+    t0 = time.perf_counter()
+    n = problem.n
+    matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
+    d = pv.csr_diagscale(matB, 1.0)                                   # :47-71
+    fs = FluidSetup(matAd=pv.csr_create(problem.Ad.rowptr, problem.Ad.col, problem.Ad.val),        # :98-119
+                    matE=pv.csr_create(problem.E.rowptr, problem.E.col, problem.E.val, n_cols=problem.n_p),
+                    matET=pv.csr_create(problem.ET.rowptr, problem.ET.col, problem.ET.val, n_cols=n),
+                    matAp=pv.csr_create(problem.Ap.rowptr, problem.Ap.col, problem.Ap.val), dp=None)
+    fs.dp = pv.csr_diagscale(fs.matAp, -1.0)                          # :137-140, 345-441
+    hp = pv.pevsl_start(0)                                            # :152
+    try:
+        pv.pevsl_setprobsizes(hp, problem.n_p, problem.n_p, -1)
+        pv.pevsl_set_seed(hp, seed)
+        pv.pevsl_setamv(hp, fs.matAp)                                 # :157
+        fs.aplmin, fs.aplmax = pv.pevsl_lanbounds(hp, *ap_bounds)     # :160-162
+    finally:
+        pv.pevsl_finish(hp)
+    fs.apdeg = ap_chebiter_degree(problem.porder, fs.aplmin, fs.aplmax)
+    fs.chebAp = pv.pevsl_setup_chebiter(fs.aplmin, fs.aplmax, fs.apdeg, fs.matAp)   # :174
+    mv = MatvecSetup(n=n, matA=0, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
+                     porder=problem.porder, timings=dict(upload=time.perf_counter() - t0), fs=fs)
+    return setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds)
 
 
 def upload(problem, *, porder: int | None = None, tune=(1,)) -> MatvecSetup:
@@ -170,7 +254,10 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
         pv.pevsl_set_seed(h, seed)
         pv.pevsl_setbmv(h, mv.matB)                           # :69
         pv.pevsl_setbsol_chebiter(h, 2, mv.chebB)             # :72-73
-        pv.pevsl_setamv_dad(h, mv.matA, mv.d)                 # :77
+        if mv.fs is not None:                                 # :77-80
+            pv.pevsl_setamv_fs(h, mv.fs.matAd, mv.fs.matE, mv.fs.matET, mv.fs.chebAp, mv.d, mv.fs.dp)
+        else:
+            pv.pevsl_setamv_dad(h, mv.matA, mv.d)
         pv.pevsl_set_geneig(h)                                # :82
         if precision != pv.FP64:
             pv.pevsl_set_precision(h, precision)

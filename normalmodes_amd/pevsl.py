@@ -15,7 +15,7 @@ from ._native import load_gpu
 
 __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprobsizes",
            "pevsl_parcsrcreate", "csr_create", "csr_free", "csr_info", "parcsr_matvec",
-           "csr_diagscale", "pevsl_setamv_dad", "pevsl_setamv", "pevsl_setbmv",
+           "csr_diagscale", "pevsl_setamv_dad", "pevsl_setamv_fs", "pevsl_setamv", "pevsl_setbmv",
            "pevsl_setbsol_chebiter", "pevsl_set_geneig", "pevsl_set_precision",
            "pevsl_set_seed", "pevsl_setup_chebiter", "pevsl_chebiter", "pevsl_chebiter_free",
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
@@ -141,6 +141,14 @@ def csr_diagscale(mat: Handle, sign: float = 1.0):
 def pevsl_setamv_dad(h: Handle, matA: Handle, d):
     d = _f64(d)
     _check(_lib().nmg_set_amv_dad(_i64(h), _i64(matA), _p(d, _f64p)))
+
+
+def pevsl_setamv_fs(h: Handle, matAd: Handle, matE: Handle, matET: Handle, chebAp: Handle, d, dp):
+    """sparsefsAV (src/mod_matvec.f90:498-520): D (Ad - E Ap^-1 ET) D."""
+    d = _f64(d)
+    dp = _f64(dp)
+    _check(_lib().nmg_set_amv_fs(_i64(h), _i64(matAd), _i64(matE), _i64(matET), _i64(chebAp), _p(d, _f64p),
+                                 _p(dp, _f64p)))
 
 
 def pevsl_setamv(h: Handle, mat: Handle):

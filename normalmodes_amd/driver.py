@@ -17,15 +17,19 @@ import numpy as np
 
 from . import pevsl as pv
 
-__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
 
 
 def freq_to_lambda(f_mhz: float) -> float:
-    """lambda = (2 pi f)^2 * 1e-6 for f in mHz (src/mod_pevsl.f90:93-94)."""
-    return (2.0 * math.pi * f_mhz) ** 2 * 1.0e-6
+    """lambda = (2 pi f)^2 * 1e-6 for f in mHz (src/mod_pevsl.f90:93-94), with
+    the reference's roundings: f is a default REAL (src/mod_para.f90:51-52)
+    and PI is assigned from a default-real literal (src/mod_pevsl.f90:43),
+    so both enter the double-precision product as float32 values."""
+    pi32 = float(np.float32(3.14159265359))
+    return (2.0 * pi32 * float(np.float32(f_mhz))) ** 2 * 1.0e-6
 
 
 def lambda_to_freq(lam):
@@ -421,3 +425,30 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
                      filter_applies=st["filter_applies"], ncand=st["ncand"], timings=t,
                      lan_steps_bounds=lb_steps, probe=probe, handle=h if keep else 0, slices=sli,
                      slice_counts=counts, dos_count=ecnt)
+
+
+def solve_input(inputdir, basename, *, porder: int = 1, job: int = 1, lowfreq: float = 0.2, upfreq: float = 2.0,
+                outputdir=None, bdeg="reference", tune=(1,), **kw):
+    """The reference's main program on its own input files (src/mainnm.f90):
+    read the mesh and model, assemble A and B (f1), setupmatvec,
+    pnm_apply_pevsl on [lowfreq, upfreq] mHz, and, with ``outputdir``, write
+    the mode files.  Returns (problem, setup, result, paths)."""
+    from .assemble import assemble
+    from .formats import read_input, write_modes
+    M = read_input(inputdir, basename, porder)
+    g0 = None
+    if job == 2:
+        if M.g0 is None:
+            raise ValueError("JOB=2 needs the potential-acceleration file")
+        g0 = M.g0 / 1.0e3                                  # m/s^2 -> km/s^2, src/mod_cg_create_matrix.f90:194
+    P = assemble(M.mesh, rho=M.rho, vp=M.vp, vs=M.vs, g0=g0, job=job, name=basename)
+    mv = setupmatvec(P, bdeg=bdeg, tune=tune)
+    try:
+        r = apply_pevsl(mv, lowfreq=lowfreq, upfreq=upfreq, **kw)
+        paths = []
+        if outputdir is not None and r.X is not None:
+            paths = write_modes(outputdir, basename, job=job, porder=porder, lowfreq=lowfreq, upfreq=upfreq,
+                                X_phys=r.physical_vectors(mv.d), vlist=M.mesh.vlist)
+    finally:
+        mv.free()
+    return P, mv, r, paths

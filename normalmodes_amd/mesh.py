@@ -20,7 +20,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["TetMesh", "ball_mesh", "ellipsoid_mesh", "h_for_tets"]
+__all__ = ["TetMesh", "ball_mesh", "ellipsoid_mesh", "h_for_tets", "mesh_from_vertices"]
 
 # (vertex a, vertex b, slot) for P2 edge nodes; vertices are 0-based here
 _P2_EDGES = ((0, 1, 1), (0, 2, 3), (1, 2, 4), (0, 3, 6), (1, 3, 7), (2, 3, 8))
@@ -36,6 +36,7 @@ class TetMesh:
     surface: np.ndarray      # (n_nodes,) bool: node lies on the outer surface
     seed: int
     h: float
+    vlist: np.ndarray | None = None  # (n_nodes,) input node id of each node (meshes read from files)
 
     @property
     def n_nodes(self) -> int:
@@ -130,29 +131,36 @@ def ball_mesh(n_tets: int | None = None, *, h: float | None = None, porder: int 
 
     ax = np.asarray(axes, dtype=np.float64)
     X = X * ax[None, :]
+    return _assemble_mesh(X, T, on_surf, porder, seed=seed, h=h, reorder_elems=True)
 
+
+def _promote(X, T, on_surf, porder):
+    """P1 or P2 node set of the tetrahedra T over vertices X: P2 adds one node
+    per unique edge at its midpoint (numbered after the vertices, in order of
+    the sorted vertex pairs), in the reference's local slots."""
     if porder == 1:
-        nodes = X
-        elems = T.copy()
-        surf = on_surf
-    elif porder == 2:
-        nv = X.shape[0]
-        pairs = np.concatenate([np.sort(T[:, [a, b]], axis=1) for a, b, _ in _P2_EDGES], axis=0)
-        key = pairs[:, 0] * nv + pairs[:, 1]
-        ukey, inv = np.unique(key, return_inverse=True)
-        ea, eb = ukey // nv, ukey % nv
-        mid = 0.5 * (X[ea] + X[eb])
-        nodes = np.concatenate([X, mid], axis=0)
-        surf = np.concatenate([on_surf, on_surf[ea] & on_surf[eb]])
-        elems = np.empty((T.shape[0], 10), dtype=np.int64)
-        for slot, v in zip(_P2_VERT_SLOTS, range(4)):
-            elems[:, slot] = T[:, v]
-        ne = T.shape[0]
-        for k, (_, _, slot) in enumerate(_P2_EDGES):
-            elems[:, slot] = nv + inv[k * ne:(k + 1) * ne]
-    else:
+        return X, T.copy(), on_surf
+    if porder != 2:
         raise ValueError("pOrder must be 1 or 2")
+    nv = X.shape[0]
+    pairs = np.concatenate([np.sort(T[:, [a, b]], axis=1) for a, b, _ in _P2_EDGES], axis=0)
+    key = pairs[:, 0] * nv + pairs[:, 1]
+    ukey, inv = np.unique(key, return_inverse=True)
+    ea, eb = ukey // nv, ukey % nv
+    mid = 0.5 * (X[ea] + X[eb])
+    nodes = np.concatenate([X, mid], axis=0)
+    surf = np.concatenate([on_surf, on_surf[ea] & on_surf[eb]])
+    elems = np.empty((T.shape[0], 10), dtype=np.int64)
+    for slot, v in zip(_P2_VERT_SLOTS, range(4)):
+        elems[:, slot] = T[:, v]
+    ne = T.shape[0]
+    for k, (_, _, slot) in enumerate(_P2_EDGES):
+        elems[:, slot] = nv + inv[k * ne:(k + 1) * ne]
+    return nodes, elems, surf
 
+
+def _assemble_mesh(X, T, on_surf, porder, *, seed, h, reorder_elems):
+    nodes, elems, surf = _promote(X, np.asarray(T, dtype=np.int64), on_surf, porder)
     order = _morton_order(nodes)
     rank = np.empty_like(order)
     rank[order] = np.arange(order.size)
@@ -160,13 +168,30 @@ def ball_mesh(n_tets: int | None = None, *, h: float | None = None, porder: int 
     surf = surf[order]
     elems = rank[elems]
     tets = elems[:, list(_P2_VERT_SLOTS if porder == 2 else (0, 1, 2, 3))]
-    # element order follows the Morton code of the first vertex
-    eorder = np.lexsort((elems[:, -1], tets.min(axis=1)))
-    elems = elems[eorder]
-    tets = tets[eorder]
+    if reorder_elems:
+        # element order follows the Morton code of the first vertex
+        eorder = np.lexsort((elems[:, -1], tets.min(axis=1)))
+        elems = elems[eorder]
+        tets = tets[eorder]
     return TetMesh(nodes=np.ascontiguousarray(nodes), elems=np.ascontiguousarray(elems.astype(np.int32)),
                    tets=np.ascontiguousarray(tets.astype(np.int32)), porder=porder, surface=surf,
-                   seed=seed, h=h)
+                   seed=seed, h=h, vlist=order.astype(np.int64))
+
+
+def mesh_from_vertices(X, T, porder: int, *, surface=None) -> TetMesh:
+    """A TetMesh over given vertices X (nv, 3) and tetrahedra T (ne, 4),
+    0-based, positively oriented.  Element order is kept (per-element model
+    data stays aligned); nodes are renumbered in Morton order and ``vlist``
+    records each node's id in the input numbering (vertices 0..nv-1, edge
+    nodes nv.. in the order of ``_promote``)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.asarray(T, dtype=np.int64)
+    vol = _tet_volumes(X, T)
+    if np.any(vol <= 0.0):
+        raise ValueError(f"{int(np.sum(vol <= 0))} tetrahedra are not positively oriented")
+    on_surf = np.zeros(X.shape[0], dtype=bool) if surface is None else np.asarray(surface, dtype=bool)
+    lrms = np.sqrt(np.mean([np.sum((X[T[:, b]] - X[T[:, a]]) ** 2, axis=1) for a, b, _ in _P2_EDGES], axis=0))
+    return _assemble_mesh(X, T, on_surf, porder, seed=-1, h=float(np.mean(lrms)), reorder_elems=False)
 
 
 def ellipsoid_mesh(n_tets: int, flattening: float = 1.0 / 300.0, **kw) -> TetMesh:

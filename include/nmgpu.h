@@ -114,6 +114,14 @@ int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
 /* pEVSL_chebiterstatsprint_f90(cheb)  src/mod_pevsl.f90:132-135 */
 int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
 int nmg_chebiter_free(nmg_handle cheb);
+/* How the solve runs (no reference counterpart): mode 1 (default) runs all
+ * deg steps in one persistent cooperative launch when the matrix is a
+ * scalar mass x I3 whose rows fit the workgroups' LDS (rows and values are
+ * read from HBM once per solve, steps are separated by grid barriers);
+ * mode 0 launches one fused kernel per step. */
+int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
+/* whether the persistent solve applies (and its grid and LDS bytes) */
+int nmg_chebiter_info(nmg_handle cheb, int fp32, int *persistent, int *grid, int *lds_bytes);
 
 /* --- eigensolver ------------------------------------------------------- */
 /* pEVSL_LANBOUNDS_F90(h, mlan, maxit, tol, lmin, lmax)
@@ -226,6 +234,9 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);
  * library's own stream) and the algorithmic bytes of one product. */
 int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
 /* Same for one fused Chebyshev B-solve step (one product + vector update). */
+/* Timing of one whole Chebyshev solve (deg steps) on device vectors, in the
+ * cheb's current mode; bytes: deg x the algorithmic bytes of one step. */
+int nmg_bench_bsolve(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double *seconds, double *bytes);
 /* Kernel mapping of a matrix (tuning): lanes per row of the single-column
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per

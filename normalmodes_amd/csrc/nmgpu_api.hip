@@ -363,6 +363,20 @@ struct Cheb {
   int deg = 0;
   int64_t ncalls = 0, nmatvec = 0;
   double seconds = 0;
+  // 1: one persistent launch per solve where planned (scalar mass x I3 whose
+  // rows fit the LDS budget), else one launch per step; 0: always per step
+  int mode = 1;
+  PersistPlan plan[2];  // FP64 / FP32 values
+  bool planned[2] = {false, false};
+  const PersistPlan* persistent(bool fp32) {
+    if (mode != 1) return nullptr;
+    const int i = fp32 ? 1 : 0;
+    if (!planned[i]) {
+      plan_cheb_persistent(*mat, fp32, plan[i]);
+      planned[i] = true;
+    }
+    return plan[i].ok ? &plan[i] : nullptr;
+  }
 };
 
 // Sampled kernel timing on the solver's stream: every `every`-th launch of a
@@ -529,9 +543,18 @@ static void ensure_basis(Solver& S, int need, int keep, int maxcap) {
 //   x_1 = b/th,  x_{k+1} = x_{k-1} + w_{k+1}(x_k - x_{k-1} + (b - B x_k)/th),
 //   w_1 = 2, w_{k+1} = 1/(1 - w_k/(4 s^2)), th = (lmax+lmin)/2, s = th/de.
 // The result lands in `out`; `tmp` is a second buffer.
-static void cheb_solve(const Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
+static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
                        bool fp32, KernelProbe* probe = nullptr) {
   const int64_t n = C.mat->nrow;
+  const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)n;
+  if (const PersistPlan* P = C.persistent(fp32)) {
+    const bool pr = probe && probe->armed();
+    if (pr) probe->before(s);
+    k_cheb_persistent(*C.mat, *P, b, out, tmp, C.deg, C.lmin, C.lmax, s);
+    if (pr) probe->after(s);
+    if (probe) probe->bytes = step_bytes * C.deg;
+    return;
+  }
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;
@@ -546,6 +569,7 @@ static void cheb_solve(const Cheb& C, const double* b, double* out, double* tmp,
     if (pr) probe->after(s);
     std::swap(xk, xm);
   }
+  if (probe) probe->bytes = step_bytes;
 }
 
 static void apply_B(Solver& S, const double* x, double* y) {
@@ -962,9 +986,11 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
 
 // K interleaved columns of x ~ M^{-1} b by the Chebyshev semi-iteration of
 // cheb_solve, one K-column fused step per degree
-static void block_cheb(const Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
+static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
                        KernelProbe* probe) {
+  if (K == 1) return cheb_solve(C, b, out, tmp, s, fp32, probe);
   const int64_t nk = C.mat->nrow * K;
+  if (probe) probe->bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;
@@ -2131,7 +2157,8 @@ int nmg_kernel_probe(nmg_handle h, int which, double* mean_seconds, int64_t* sam
       const double nk = (double)S->n * (double)S->block;
       if (which == 0) {
         NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set");
-        *bytes_per_launch = S->bsol->mat->matrix_bytes(f32) + 4.0 * 8.0 * nk;
+        // the launch that was sampled: one step, or a whole persistent solve
+        *bytes_per_launch = p.bytes > 0.0 ? p.bytes : S->bsol->mat->matrix_bytes(f32) + 4.0 * 8.0 * nk;
       } else {
         // the fluid-solid operator is not one fused launch: no figure
         *bytes_per_launch = S->fs ? 0.0 : S->A().matrix_bytes(false) + 6.0 * 8.0 * nk;
@@ -2428,6 +2455,44 @@ int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double* seconds, double* 
     (void)hipEventDestroy(e1);
     if (seconds) *seconds = ms * 1e-3 / reps;
     if (bytes) *bytes = M->algorithmic_bytes(fp32 != 0);
+  });
+}
+
+int nmg_chebiter_set_mode(nmg_handle cheb, int mode) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(mode == 0 || mode == 1, NMG_EINVAL, "mode must be 0 (one launch per step) or 1 (persistent)");
+    C->mode = mode;
+  });
+}
+
+int nmg_chebiter_info(nmg_handle cheb, int fp32, int* persistent, int* grid, int* lds_bytes) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    if (fp32 && !(C->mat->val32 || C->mat->bval32)) make_fp32(*C->mat, default_stream());
+    const PersistPlan* P = C->persistent(fp32 != 0);
+    if (persistent) *persistent = P ? 1 : 0;
+    if (grid) *grid = P ? P->grid : 0;
+    if (lds_bytes) *lds_bytes = P ? (int)P->lds_bytes : 0;
+  });
+}
+
+int nmg_bench_bsolve(nmg_handle cheb, int reps, int fp32, double* seconds, double* bytes) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
+    DevMat* M = C->mat;
+    hipStream_t s = default_stream();
+    if (fp32 && !(M->val32 || M->bval32)) make_fp32(*M, s);
+    const int64_t n = M->nrow;
+    DVec b, x, t;
+    b.ensure(n);
+    x.ensure(n);
+    t.ensure(n);
+    k_fill(n, 1.0, b.p, s);
+    const double sec = time_launches(s, reps, [&] { cheb_solve(*C, b.p, x.p, t.p, s, fp32 != 0); });
+    if (seconds) *seconds = sec;
+    if (bytes) *bytes = (M->matrix_bytes(fp32 != 0) + 4.0 * 8.0 * (double)n) * C->deg;
   });
 }
 

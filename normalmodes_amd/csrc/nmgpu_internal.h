@@ -82,6 +82,28 @@ struct DevMat {
   double algorithmic_bytes(bool fp32) const;  // one product: matrix + x + y
 };
 
+// Plan of the persistent Chebyshev solve (scalar mass x I3, one column):
+// cooperative grid, nnz-balanced row ranges per workgroup, LDS sizes.
+constexpr size_t kPersistLdsMax = 64 * 1024;
+struct PersistPlan {
+  bool ok = false, fp32 = false;
+  int grid = 0, rows_cap = 0, ent_cap = 0;
+  size_t lds_bytes = 0;
+  int64_t* part = nullptr;
+  void release() {
+    if (part) (void)hipFree(part);
+    part = nullptr;
+    ok = false;
+  }
+  ~PersistPlan() { release(); }
+};
+// plans the persistent solve for M (ok stays false when M is not MASS3, its
+// rows do not fit the LDS budget, or cooperative launch is unavailable)
+void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P);
+// x ~ M^-1 b by `deg` Chebyshev steps on [lmin, lmax] in one launch
+void k_cheb_persistent(const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp, int deg,
+                       double lmin, double lmax, hipStream_t s);
+
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);
 // creates a DevMat from host CSR (0-based); picks the layout

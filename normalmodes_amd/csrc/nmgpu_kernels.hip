@@ -12,6 +12,8 @@
 //          1417-1435): one scalar entry serves the three components.
 // Each product carries an epilogue functor so that the vector updates of the
 // Chebyshev B-solve and of the polynomial filter are fused into the same pass.
+#include <hip/hip_cooperative_groups.h>
+
 #include "nmgpu_internal.h"
 
 namespace nmg {
@@ -1187,6 +1189,165 @@ void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* 
   else
     throw Error{NMG_EINVAL, "adjoint supports P1 (4 nodes) and P2 (10 nodes) elements"};
   NMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Persistent Chebyshev B-solve (scalar mass x I3 layout, one column): all
+// `deg` steps of the semi-iteration in one cooperative launch.  Each
+// workgroup owns a contiguous, nnz-balanced range of node rows; it copies
+// its rows' column indices and values into LDS once, then every step only
+// gathers x_k (3 contiguous doubles per neighbour node) and updates its own
+// rows of x_{k+1}.  Steps are separated by a grid barrier
+// (cooperative_groups grid sync: device-scope fences, so x_{k+1} written by
+// one XCD is what every XCD reads in the next step).  Every workgroup runs
+// the same number of barriers; there is no early exit.
+constexpr int kPBlock = 512;  // threads per workgroup
+constexpr int kPG = 8;        // lanes per node row
+
+template <class VT>
+__global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci,
+                                                             const VT* __restrict__ va,
+                                                             const int64_t* __restrict__ part, int rows_cap,
+                                                             int ent_cap, const double* __restrict__ b,
+                                                             double* xa, double* xb, int deg, double ith,
+                                                             double sg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int64_t r0 = part[blockIdx.x], r1 = part[blockIdx.x + 1];
+  const int64_t e0 = rp[r0];
+  const int nr = (int)(r1 - r0);
+  VT* lv = (VT*)smem;                                    // ent_cap values
+  int32_t* lc = (int32_t*)(smem + sizeof(VT) * (size_t)ent_cap);  // ent_cap columns
+  int32_t* lr = lc + ent_cap;                            // rows_cap + 1 offsets
+  const int ne = (int)(rp[r1] - e0);
+  for (int t = threadIdx.x; t < ne; t += kPBlock) {
+    lv[t] = va[e0 + t];
+    lc[t] = ci[e0 + t];
+  }
+  for (int t = threadIdx.x; t <= nr; t += kPBlock) lr[t] = (int)(rp[r0 + t] - e0);
+  // x_1 = b / th (in xa), x_0 = 0 (in xb), own rows
+  for (int64_t i = 3 * r0 + threadIdx.x; i < 3 * r1; i += kPBlock) {
+    xa[i] = b[i] * ith;
+    xb[i] = 0.0;
+  }
+  __syncthreads();
+  grid.sync();
+  __threadfence();
+  const double* xk = xa;
+  double* xm = xb;
+  const int l = threadIdx.x % kPG;
+  double w = 2.0;
+  for (int k = 1; k <= deg; ++k) {
+    w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    for (int rb = 0; rb < nr; rb += kPBlock / kPG) {
+      const int r = rb + threadIdx.x / kPG;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      if (r < nr) {
+        const int end = lr[r + 1];
+        for (int j = lr[r] + l; j < end; j += kPG) {
+          const double v = (double)lv[j];
+          const double* xp = xk + 3 * (int64_t)lc[j];
+          s0 += v * xp[0];
+          s1 += v * xp[1];
+          s2 += v * xp[2];
+        }
+      }
+#pragma unroll
+      for (int off = kPG / 2; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, kPG);
+        s1 += __shfl_xor(s1, off, kPG);
+        s2 += __shfl_xor(s2, off, kPG);
+      }
+      if (r < nr && l < 3) {
+        const int64_t i = 3 * (r0 + r) + l;
+        const double sv = l == 0 ? s0 : (l == 1 ? s1 : s2);
+        const double xmi = xm[i], xki = xk[i];
+        xm[i] = xmi + w * (xki - xmi + (b[i] - sv) * ith);
+      }
+    }
+    __syncthreads();
+    grid.sync();
+    __threadfence();
+    double* t = const_cast<double*>(xk);
+    xk = xm;
+    xm = t;
+  }
+}
+
+template <class VT>
+static void* persistent_fn() {
+  return reinterpret_cast<void*>(&cheb_persistent_k<VT>);
+}
+
+void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P) {
+  P.release();
+  if (M.layout != L_MASS3 || M.h_browptr.empty() || (fp32 && !M.bval32)) return;
+  int dev = 0, ncu = 0;
+  NMG_HIP(hipGetDevice(&dev));
+  NMG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int coop = 0;
+  NMG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop || ncu <= 0) return;
+  const int64_t nb = M.nb;
+  const std::vector<int64_t>& rp = M.h_browptr;
+  // one workgroup per CU, fewer if the rows are few; nnz-balanced ranges
+  int G = (int)std::min<int64_t>(ncu, std::max<int64_t>(1, nb / 8));
+  std::vector<int64_t> part(G + 1, 0);
+  const double tot = (double)rp[nb] + 4.0 * (double)nb;  // entries plus a per-row cost
+  int64_t r = 0;
+  for (int g = 1; g < G; ++g) {
+    const double target = tot * g / G;
+    while (r < nb && (double)rp[r] + 4.0 * (double)r < target) ++r;
+    part[g] = r;
+  }
+  part[G] = nb;
+  int64_t max_rows = 0, max_ent = 0;
+  for (int g = 0; g < G; ++g) {
+    max_rows = std::max(max_rows, part[g + 1] - part[g]);
+    max_ent = std::max(max_ent, rp[part[g + 1]] - rp[part[g]]);
+  }
+  const size_t vsz = fp32 ? sizeof(float) : sizeof(double);
+  const size_t bytes = vsz * (size_t)max_ent + 4 * (size_t)max_ent + 4 * (size_t)(max_rows + 1) + 16;
+  if (bytes > kPersistLdsMax) return;
+  int occ = 0;
+  NMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fp32 ? persistent_fn<float>() : persistent_fn<double>(),
+                                                       kPBlock, bytes));
+  if (occ < 1 || (int64_t)occ * ncu < G) return;
+  P.part = nullptr;
+  NMG_HIP(hipMalloc(&P.part, sizeof(int64_t) * (G + 1)));
+  NMG_HIP(hipMemcpy(P.part, part.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice));
+  P.grid = G;
+  P.rows_cap = (int)max_rows;
+  P.ent_cap = (int)max_ent;
+  P.lds_bytes = bytes;
+  P.fp32 = fp32;
+  P.ok = true;
+}
+
+void k_cheb_persistent(const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp, int deg,
+                       double lmin, double lmax, hipStream_t s) {
+  NMG_CHECK(P.ok && deg >= 1, NMG_ESTATE, "persistent Chebyshev solve not planned");
+  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin);
+  double ith = 1.0 / th, sg = th / de;
+  // the result x_{deg+1} ends in xa for even deg, xb for odd
+  double* xa = (deg % 2 == 0) ? out : tmp;
+  double* xb = (deg % 2 == 0) ? tmp : out;
+  const int64_t* rp = M.browptr;
+  const int32_t* ci = M.bcol;
+  const int64_t* part = P.part;
+  int rows_cap = P.rows_cap, ent_cap = P.ent_cap;
+  void* args[] = {(void*)&rp, (void*)&ci, nullptr, (void*)&part, (void*)&rows_cap, (void*)&ent_cap,
+                  (void*)&b, (void*)&xa, (void*)&xb, (void*)&deg, (void*)&ith, (void*)&sg};
+  if (P.fp32) {
+    const float* va = M.bval32;
+    args[2] = (void*)&va;
+    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<float>(), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
+  } else {
+    const double* va = M.bval;
+    args[2] = (void*)&va;
+    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<double>(), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
+  }
 }
 
 }  // namespace nmg

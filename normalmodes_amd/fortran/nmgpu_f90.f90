@@ -167,6 +167,51 @@ module nmgpu_f90
       integer(c_int64_t), value :: h
       real(c_double), intent(out) :: res_ref(*), res2(*)
     end function
+    ! sparsefsAV (src/mod_matvec.f90:498-520): D (Ad - E Ap^-1 ET) D
+    integer(c_int) function nmg_set_amv_fs(h, matAd, matE, matET, chebAp, d, dp) bind(C, name="nmg_set_amv_fs")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h, matAd, matE, matET, chebAp
+      real(c_double), intent(in) :: d(*), dp(*)
+    end function
+    ! kernel polynomial density of states (pEVSL kpmdos); intv = (a, b, lmin, lmax)
+    integer(c_int) function nmg_kpmdos(h, mdeg, nvec, intv, mu, ecnt) bind(C, name="nmg_kpmdos")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      integer(c_int), value :: mdeg, nvec
+      real(c_double), intent(in) :: intv(4)
+      real(c_double), intent(out) :: mu(*), ecnt
+    end function
+    ! equal-count slice boundaries (pEVSL spslicer)
+    integer(c_int) function nmg_spslicer(sli, mu, mdeg, intv, n_slices) bind(C, name="nmg_spslicer")
+      import :: c_int, c_double
+      real(c_double), intent(out) :: sli(*)
+      real(c_double), intent(in) :: mu(*), intv(4)
+      integer(c_int), value :: mdeg, n_slices
+    end function
+    ! spectrum slices in lockstep (NSLICES = 1 in src/mod_pevsl.f90:47)
+    integer(c_int) function nmg_solve_slices(h, n_slices, sli, lmin, lmax, thr_int, thr_ext, maxit, tol) &
+        bind(C, name="nmg_solve_slices")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      integer(c_int), value :: n_slices, maxit
+      real(c_double), intent(in) :: sli(*)
+      real(c_double), value :: lmin, lmax, thr_int, thr_ext, tol
+    end function
+    integer(c_int) function nmg_slice_counts(h, counts) bind(C, name="nmg_slice_counts")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+      integer(c_int), intent(out) :: counts(*)
+    end function
+    ! adjoint: per-element d lambda / d (rho, lambda, mu), JOB = 1
+    integer(c_int) function nmg_backward(h, n_elem, np, elems, detJ, invJ, refM, refD, job, d_rho, d_lam, d_mu) &
+        bind(C, name="nmg_backward")
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int64_t), value :: h, n_elem
+      integer(c_int), value :: np, job
+      integer(c_int32_t), intent(in) :: elems(*)
+      real(c_double), intent(in) :: detJ(*), invJ(*), refM(*), refD(*)
+      real(c_double), intent(out) :: d_rho(*), d_lam(*), d_mu(*)
+    end function
   end interface
 
 contains

@@ -22,7 +22,8 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "kpmdos", "dos_count", "spslicer",
-           "solve_slices", "slice_counts", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
+           "bench_bsolve", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -366,6 +367,26 @@ def apply_block(h: Handle, which: int, X):
     _check(_lib().nmg_apply_block(_i64(h), ctypes.c_int(which), ctypes.c_int(X.shape[0]), _p(X, _f64p),
                                   _p(Y, _f64p)))
     return Y
+
+
+def chebiter_set_mode(cheb: Handle, mode: int):
+    """1: one persistent launch per solve where it applies; 0: one launch per step."""
+    _check(_lib().nmg_chebiter_set_mode(_i64(cheb), ctypes.c_int(mode)))
+
+
+def chebiter_info(cheb: Handle, fp32: bool = False):
+    p, g, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(_lib().nmg_chebiter_info(_i64(cheb), ctypes.c_int(int(fp32)), ctypes.byref(p), ctypes.byref(g),
+                                    ctypes.byref(l)))
+    return dict(persistent=bool(p.value), grid=g.value, lds_bytes=l.value)
+
+
+def bench_bsolve(cheb: Handle, reps: int = 10, fp32: bool = False):
+    """(seconds per whole solve, algorithmic bytes per solve)."""
+    s, b = _d(), _d()
+    _check(_lib().nmg_bench_bsolve(_i64(cheb), ctypes.c_int(reps), ctypes.c_int(int(fp32)), ctypes.byref(s),
+                                   ctypes.byref(b)))
+    return s.value, b.value
 
 
 def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):

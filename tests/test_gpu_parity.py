@@ -113,6 +113,46 @@ def test_chebiter(small_p2):
         pv.csr_free(h)
 
 
+def test_chebiter_persistent_matches_steps(small_p2, layered_p2):
+    """The one-launch persistent solve against one launch per step (same
+    recurrence, different summation order), FP64 and FP32 values, odd and
+    even degrees; generic CSR matrices are not planned for it."""
+    pv = _pv()
+    for P in (small_p2, layered_p2):
+        B = P.B
+        h = pv.csr_create(B.rowptr, B.col, B.val)
+        d_ref, val_ref = orc.diagscale(B, 1.0)
+        try:
+            pv.csr_diagscale(h, 1.0)
+            b = np.random.default_rng(9).standard_normal(B.n_rows)
+            for deg in (3, 58):
+                c = pv.pevsl_setup_chebiter(0.25, 4.6, deg, h)
+                try:
+                    info = pv.chebiter_info(c)
+                    assert info["persistent"] and info["grid"] >= 1, info
+                    x1 = pv.pevsl_chebiter(2, b, c)
+                    pv.chebiter_set_mode(c, 0)
+                    x0 = pv.pevsl_chebiter(2, b, c)
+                    assert _rel(x1, x0) < 1e-14, deg
+                    ref = orc.chebiter(orc.CSRView(B.rowptr, B.col, val_ref), 0.25, 4.6, deg, b)
+                    assert _rel(x1, ref) < 1e-13, deg
+                    pv.chebiter_set_mode(c, 1)
+                    t1, _ = pv.bench_bsolve(c, 3)
+                    assert t1 > 0
+                finally:
+                    pv.pevsl_chebiter_free(c)
+        finally:
+            pv.csr_free(h)
+    A = small_p2.A
+    g = pv.csr_create(A.rowptr, A.col, A.val)   # 3x3 blocks: not a scalar mass
+    try:
+        c = pv.pevsl_setup_chebiter(1.0, 2.0, 4, g)
+        assert not pv.chebiter_info(c)["persistent"]
+        pv.pevsl_chebiter_free(c)
+    finally:
+        pv.csr_free(g)
+
+
 def test_findpol_matches_oracle():
     pv = _pv()
     for xintv in ([0.5, 12.0, -0.1, 12600.0], [3000.0, 3200.0, 0.0, 12600.0], [11000, 12600, 0, 12600]):

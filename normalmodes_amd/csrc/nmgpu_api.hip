@@ -550,7 +550,7 @@ static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipSt
   if (const PersistPlan* P = C.persistent(fp32)) {
     const bool pr = probe && probe->armed();
     if (pr) probe->before(s);
-    k_cheb_persistent(*C.mat, *P, b, out, tmp, C.deg, C.lmin, C.lmax, s);
+    k_cheb_persistent(1, *C.mat, *P, b, out, tmp, C.deg, C.lmin, C.lmax, s);
     if (pr) probe->after(s);
     if (probe) probe->bytes = step_bytes * C.deg;
     return;
@@ -990,7 +990,16 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
                        KernelProbe* probe) {
   if (K == 1) return cheb_solve(C, b, out, tmp, s, fp32, probe);
   const int64_t nk = C.mat->nrow * K;
-  if (probe) probe->bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
+  const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
+  if (const PersistPlan* P = C.persistent(fp32)) {
+    const bool pr = probe && probe->armed();
+    if (pr) probe->before(s);
+    k_cheb_persistent(K, *C.mat, *P, b, out, tmp, C.deg, C.lmin, C.lmax, s);
+    if (pr) probe->after(s);
+    if (probe) probe->bytes = step_bytes * C.deg;
+    return;
+  }
+  if (probe) probe->bytes = step_bytes;
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;

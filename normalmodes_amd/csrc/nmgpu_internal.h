@@ -100,9 +100,10 @@ struct PersistPlan {
 // plans the persistent solve for M (ok stays false when M is not MASS3, its
 // rows do not fit the LDS budget, or cooperative launch is unavailable)
 void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P);
-// x ~ M^-1 b by `deg` Chebyshev steps on [lmin, lmax] in one launch
-void k_cheb_persistent(const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp, int deg,
-                       double lmin, double lmax, hipStream_t s);
+// x ~ M^-1 b by `deg` Chebyshev steps on [lmin, lmax] in one launch, for K
+// interleaved columns (K = 1, 2, 4, 8, 16)
+void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp,
+                       int deg, double lmin, double lmax, hipStream_t s);
 
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);

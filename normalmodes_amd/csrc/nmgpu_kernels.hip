@@ -1192,19 +1192,20 @@ void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* 
 }
 
 // ---------------------------------------------------------------------------
-// Persistent Chebyshev B-solve (scalar mass x I3 layout, one column): all
-// `deg` steps of the semi-iteration in one cooperative launch.  Each
-// workgroup owns a contiguous, nnz-balanced range of node rows; it copies
-// its rows' column indices and values into LDS once, then every step only
-// gathers x_k (3 contiguous doubles per neighbour node) and updates its own
-// rows of x_{k+1}.  Steps are separated by a grid barrier
+// Persistent Chebyshev B-solve (scalar mass x I3 layout, K interleaved
+// columns): all `deg` steps of the semi-iteration in one cooperative launch.
+// Each workgroup owns a contiguous, nnz-balanced range of node rows; it
+// copies its rows' column indices and values into LDS once, then every step
+// only gathers x_k (3K contiguous doubles per neighbour node) and updates
+// its own rows of x_{k+1}.  Steps are separated by a grid barrier
 // (cooperative_groups grid sync: device-scope fences, so x_{k+1} written by
 // one XCD is what every XCD reads in the next step).  Every workgroup runs
 // the same number of barriers; there is no early exit.
+// Lanes: L = max(K, 8) per node row; lane = slot * K + column, the S = L/K
+// slots split the row's entries and are summed by shuffles.
 constexpr int kPBlock = 512;  // threads per workgroup
-constexpr int kPG = 8;        // lanes per node row
 
-template <class VT>
+template <int K, class VT>
 __global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __restrict__ rp,
                                                              const int32_t* __restrict__ ci,
                                                              const VT* __restrict__ va,
@@ -1212,14 +1213,17 @@ __global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __re
                                                              int ent_cap, const double* __restrict__ b,
                                                              double* xa, double* xb, int deg, double ith,
                                                              double sg) {
+  constexpr int L = K > 8 ? K : 8;
+  constexpr int S = L / K;
+  constexpr int RPR = kPBlock / L;  // rows per round
   extern __shared__ __align__(16) unsigned char smem[];
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const int64_t r0 = part[blockIdx.x], r1 = part[blockIdx.x + 1];
   const int64_t e0 = rp[r0];
   const int nr = (int)(r1 - r0);
-  VT* lv = (VT*)smem;                                    // ent_cap values
+  VT* lv = (VT*)smem;                                             // ent_cap values
   int32_t* lc = (int32_t*)(smem + sizeof(VT) * (size_t)ent_cap);  // ent_cap columns
-  int32_t* lr = lc + ent_cap;                            // rows_cap + 1 offsets
+  int32_t* lr = lc + ent_cap;                                     // rows_cap + 1 offsets
   const int ne = (int)(rp[r1] - e0);
   for (int t = threadIdx.x; t < ne; t += kPBlock) {
     lv[t] = va[e0 + t];
@@ -1227,7 +1231,7 @@ __global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __re
   }
   for (int t = threadIdx.x; t <= nr; t += kPBlock) lr[t] = (int)(rp[r0 + t] - e0);
   // x_1 = b / th (in xa), x_0 = 0 (in xb), own rows
-  for (int64_t i = 3 * r0 + threadIdx.x; i < 3 * r1; i += kPBlock) {
+  for (int64_t i = 3 * K * r0 + threadIdx.x; i < 3 * K * r1; i += kPBlock) {
     xa[i] = b[i] * ith;
     xb[i] = 0.0;
   }
@@ -1236,34 +1240,39 @@ __global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __re
   __threadfence();
   const double* xk = xa;
   double* xm = xb;
-  const int l = threadIdx.x % kPG;
+  const int lane = threadIdx.x % L;
+  const int c = lane % K, slot = lane / K;
   double w = 2.0;
   for (int k = 1; k <= deg; ++k) {
     w = 1.0 / (1.0 - w / (4.0 * sg * sg));
-    for (int rb = 0; rb < nr; rb += kPBlock / kPG) {
-      const int r = rb + threadIdx.x / kPG;
+    for (int rb = 0; rb < nr; rb += RPR) {
+      const int r = rb + threadIdx.x / L;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
       if (r < nr) {
         const int end = lr[r + 1];
-        for (int j = lr[r] + l; j < end; j += kPG) {
+        for (int j = lr[r] + slot; j < end; j += S) {
           const double v = (double)lv[j];
-          const double* xp = xk + 3 * (int64_t)lc[j];
+          const double* xp = xk + (int64_t)3 * K * lc[j] + c;
           s0 += v * xp[0];
-          s1 += v * xp[1];
-          s2 += v * xp[2];
+          s1 += v * xp[K];
+          s2 += v * xp[2 * K];
         }
       }
 #pragma unroll
-      for (int off = kPG / 2; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, kPG);
-        s1 += __shfl_xor(s1, off, kPG);
-        s2 += __shfl_xor(s2, off, kPG);
+      for (int off = L / 2; off >= K; off >>= 1) {
+        s0 += __shfl_xor(s0, off, L);
+        s1 += __shfl_xor(s1, off, L);
+        s2 += __shfl_xor(s2, off, L);
       }
-      if (r < nr && l < 3) {
-        const int64_t i = 3 * (r0 + r) + l;
-        const double sv = l == 0 ? s0 : (l == 1 ? s1 : s2);
-        const double xmi = xm[i], xki = xk[i];
-        xm[i] = xmi + w * (xki - xmi + (b[i] - sv) * ith);
+      if (r < nr && slot == 0) {
+        const int64_t i0 = (int64_t)3 * K * (r0 + r) + c;
+        const double sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const int64_t i = i0 + p * K;
+          const double xmi = xm[i], xki = xk[i];
+          xm[i] = xmi + w * (xki - xmi + (b[i] - sv[p]) * ith);
+        }
       }
     }
     __syncthreads();
@@ -1276,8 +1285,15 @@ __global__ __launch_bounds__(kPBlock) void cheb_persistent_k(const int64_t* __re
 }
 
 template <class VT>
-static void* persistent_fn() {
-  return reinterpret_cast<void*>(&cheb_persistent_k<VT>);
+static void* persistent_fn(int K) {
+  switch (K) {
+    case 1: return reinterpret_cast<void*>(&cheb_persistent_k<1, VT>);
+    case 2: return reinterpret_cast<void*>(&cheb_persistent_k<2, VT>);
+    case 4: return reinterpret_cast<void*>(&cheb_persistent_k<4, VT>);
+    case 8: return reinterpret_cast<void*>(&cheb_persistent_k<8, VT>);
+    case 16: return reinterpret_cast<void*>(&cheb_persistent_k<16, VT>);
+    default: throw Error{NMG_EINVAL, "block width must be 1, 2, 4, 8 or 16"};
+  }
 }
 
 void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P) {
@@ -1311,9 +1327,11 @@ void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P) {
   const size_t bytes = vsz * (size_t)max_ent + 4 * (size_t)max_ent + 4 * (size_t)(max_rows + 1) + 16;
   if (bytes > kPersistLdsMax) return;
   int occ = 0;
-  NMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fp32 ? persistent_fn<float>() : persistent_fn<double>(),
-                                                       kPBlock, bytes));
-  if (occ < 1 || (int64_t)occ * ncu < G) return;
+  for (int K : {1, 2, 4, 8, 16}) {  // every width must fit the same grid
+    NMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fp32 ? persistent_fn<float>(K) : persistent_fn<double>(K),
+                                                         kPBlock, bytes));
+    if (occ < 1 || (int64_t)occ * ncu < G) return;
+  }
   P.part = nullptr;
   NMG_HIP(hipMalloc(&P.part, sizeof(int64_t) * (G + 1)));
   NMG_HIP(hipMemcpy(P.part, part.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice));
@@ -1325,8 +1343,8 @@ void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P) {
   P.ok = true;
 }
 
-void k_cheb_persistent(const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp, int deg,
-                       double lmin, double lmax, hipStream_t s) {
+void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp,
+                       int deg, double lmin, double lmax, hipStream_t s) {
   NMG_CHECK(P.ok && deg >= 1, NMG_ESTATE, "persistent Chebyshev solve not planned");
   const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin);
   double ith = 1.0 / th, sg = th / de;
@@ -1342,11 +1360,11 @@ void k_cheb_persistent(const DevMat& M, const PersistPlan& P, const double* b, d
   if (P.fp32) {
     const float* va = M.bval32;
     args[2] = (void*)&va;
-    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<float>(), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
+    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<float>(K), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
   } else {
     const double* va = M.bval;
     args[2] = (void*)&va;
-    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<double>(), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
+    NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<double>(K), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
   }
 }
 

@@ -139,6 +139,22 @@ def test_chebiter_persistent_matches_steps(small_p2, layered_p2):
                     pv.chebiter_set_mode(c, 1)
                     t1, _ = pv.bench_bsolve(c, 3)
                     assert t1 > 0
+                    # K interleaved columns (block Lanczos, slices, KPM)
+                    h2 = pv.pevsl_start(0)
+                    try:
+                        pv.pevsl_setprobsizes(h2, B.n_rows, B.n_rows, -1)
+                        pv.pevsl_setbsol_chebiter(h2, 2, c)
+                        for K in (2, 4, 16):
+                            X = np.random.default_rng(K).standard_normal((K, B.n_rows))
+                            Y1 = pv.apply_block(h2, 2, X)
+                            pv.chebiter_set_mode(c, 0)
+                            Y0 = pv.apply_block(h2, 2, X)
+                            pv.chebiter_set_mode(c, 1)
+                            assert _rel(Y1, Y0) < 1e-14, K
+                            for q in (0, K - 1):
+                                assert _rel(Y1[q], pv.pevsl_chebiter(2, X[q], c)) < 1e-14, (K, q)
+                    finally:
+                        pv.pevsl_finish(h2)
                 finally:
                     pv.pevsl_chebiter_free(c)
         finally:

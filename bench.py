@@ -20,7 +20,13 @@ Also reported on the same JSON line:
   spmv_hbm      sparse-product and Chebyshev-step rates at the 2M-DoF layered
                 ball size (``--spmv-workload c2s``; ``none`` to skip);
   cpu_baseline  the CPU restatement (oracle, C + OpenMP) timed on a bounded
-                sample of the same operator, scaled to a full solve.
+                sample of the same operator, scaled to a full solve;
+  fp32_filter   one solve with FP32 B-solve values: time, per-eigenvalue
+                difference from the FP64 solve, residuals.
+
+``--kernel-only`` runs no solve: a copy kernel of known size, then the
+dominant kernel alone (one B-solve launch per repetition), for hardware
+counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE).
 """
 from __future__ import annotations
 
@@ -50,6 +56,38 @@ def metric_name():
         return json.loads((ROOT / "BASELINE.json").read_text())["metric"]
     except Exception:
         return "s to all eigpairs in interval @resid<=1e-10, FP32-filter vs FP64; SpMV HBM GB/s"
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (a line 'traffic_bytes_per_launch: <kernel_tag> <bytes>')."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel_tag:
+                try:
+                    return float(parts[2]), f.name
+                except ValueError:
+                    pass
+    return None, None
+
+
+def kernel_only(args):
+    """Copy kernel of known size, then the dominant kernel alone."""
+    from normalmodes_amd import driver, pevsl as pv
+    from normalmodes_amd.workloads import build_problem
+    P = build_problem(args.workload)
+    mv = driver.upload(P, tune=(1,))
+    driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
+    n_copy = 1 << 24   # 16M doubles: 128 MiB read + 128 MiB written per launch
+    t_copy = pv.bench_copy(n_copy, 5)
+    info = pv.chebiter_info(mv.chebB)
+    t, b = pv.bench_bsolve(mv.chebB, args.steps)
+    print(json.dumps({"copy": {"doubles": n_copy, "bytes_per_launch": 16 * n_copy, "launches": 6,
+                               "us": round(t_copy * 1e6, 2)},
+                      "bsolve": {"workload": args.workload, "n_dof": P.n, "deg": mv.bdeg, "launches": args.steps + 1,
+                                 "us": round(t * 1e6, 2), "algorithmic_bytes": b, **info}}), flush=True)
+    mv.free()
 
 
 def solve_once(mv, interval, bdeg, precision, probe_every, block):
@@ -130,6 +168,8 @@ def main():
     ap.add_argument("--block", type=int, default=1, help="Lanczos block width (1 = ChebLanNr)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
+    ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
+    ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -137,6 +177,10 @@ def main():
             "(see DESIGN.md for how the path would shard); run with --gpus 1")
         sys.exit(2)
     bdeg = args.bdeg if args.bdeg in ("auto", "reference") else int(args.bdeg)
+    if args.kernel_only:
+        args.bdeg = bdeg
+        kernel_only(args)
+        return
 
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem, INTERVALS
@@ -165,6 +209,9 @@ def main():
     gate_ok = bool(r.res2.size > 0 and np.all(r.res2 <= GATE))
     chk = r.probe["chebstep"]
     achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
+    persistent = pv.chebiter_info(mv.chebB)["persistent"] and args.block in (1, 2, 4, 8, 16)
+    ktag = f"{'cheb_persistent_k' if persistent else 'mass3_chebstep'}:{args.workload}"
+    traffic, traffic_src = traffic_from_profiles(ktag)
     line = {
         "metric": metric_name(),
         "value": round(T, 4),
@@ -195,14 +242,31 @@ def main():
         "phases_s": {k: round(v, 4) for k, v in r.timings.items()},
         "setup_s": {"build": round(t_build, 2), "upload_and_autotune": round(t_upload, 3)},
         "roofline": {
-            "bound": "hbm", "kernel": "fused Chebyshev B-solve step (scalar mass x I3 product + update)",
+            "bound": "hbm",
+            "kernel": ("persistent Chebyshev B-solve (all steps in one cooperative launch; bytes = steps x "
+                       "one fused step's algorithmic bytes)" if persistent else
+                       "fused Chebyshev B-solve step (scalar mass x I3 product + update)"),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_launch": chk["bytes"], "mean_launch_us": round(chk["seconds"] * 1e6, 3),
             "samples": chk["samples"],
             "filter_step": r.probe["filterstep"],
         },
     }
+    if args.fp32_compare:
+        try:
+            el32, r32 = solve_once(mv, interval, bdeg, pv.FP32_FILTER, 0, args.block)
+            m = min(r32.lam.size, r.lam.size)
+            line["fp32_filter"] = {
+                "seconds": round(el32, 4), "nev": int(r32.lam.size),
+                "max_rel_eig_diff_vs_fp64": float(np.max(np.abs(r32.lam[:m] - r.lam[:m]) / np.abs(r.lam[:m])))
+                if m and r32.lam.size == r.lam.size else None,
+                "resid_max": float(np.max(r32.res2)) if r32.res2.size else None,
+                "note": "FP32 B-solve values; residual floor of the FP32 B-solve is ~1e-6 (not gated)"}
+            log(f"[bench] fp32 filter: {line['fp32_filter']}")
+        except Exception as e:
+            line["fp32_filter"] = {"error": str(e)}
     if args.spmv_workload and args.spmv_workload != "none":
         try:
             line["spmv_hbm"] = spmv_rates(args.spmv_workload)

@@ -234,6 +234,9 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);
  * library's own stream) and the algorithmic bytes of one product. */
 int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
 /* Same for one fused Chebyshev B-solve step (one product + vector update). */
+/* A plain copy kernel of n doubles, `reps` times (calibration of HBM byte
+ * counters: it moves 16 n bytes per launch); mean seconds per launch. */
+int nmg_bench_copy(int64_t n, int reps, double *seconds);
 /* Timing of one whole Chebyshev solve (deg steps) on device vectors, in the
  * cheb's current mode; bytes: deg x the algorithmic bytes of one step. */
 int nmg_bench_bsolve(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);

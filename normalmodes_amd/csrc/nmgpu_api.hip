@@ -2467,6 +2467,19 @@ int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double* seconds, double* 
   });
 }
 
+int nmg_bench_copy(int64_t n, int reps, double* seconds) {
+  return guarded([&] {
+    NMG_CHECK(n > 0 && reps >= 1, NMG_EINVAL, "n and reps must be positive");
+    hipStream_t s = default_stream();
+    DVec x, y;
+    x.ensure(n);
+    y.ensure(n);
+    k_fill(n, 1.0, x.p, s);
+    const double sec = time_launches(s, reps, [&] { k_copy_kernel(n, x.p, y.p, s); });
+    if (seconds) *seconds = sec;
+  });
+}
+
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);

@@ -130,6 +130,7 @@ void k_filter_step(const DevMat& A, const double* x, const double* vk, double* v
                    double t, double cc, double mu, bool first, hipStream_t s, bool fp32 = false);
 void k_fill(int64_t n, double v, double* y, hipStream_t s);
 void k_copy(int64_t n, const double* x, double* y, hipStream_t s);
+void k_copy_kernel(int64_t n, const double* x, double* y, hipStream_t s);  // a copy kernel (not a DMA)
 void k_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);  // y = a x
 void k_mul(int64_t n, const double* d, double* x, hipStream_t s);
 // interleaved K-column row scalings (K = 1: plain vectors)

@@ -330,6 +330,15 @@ void k_filter_update(int64_t n, int K, const double* w, const double* vk, double
   NMG_HIP(hipGetLastError());
 }
 
+// plain streaming copy (calibration of the HBM byte counters)
+__global__ void copy_k(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) y[i] = x[i];
+}
+void k_copy_kernel(int64_t n, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(copy_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y);
+  NMG_HIP(hipGetLastError());
+}
+
 // deterministic two-stage reductions -----------------------------------------
 __global__ __launch_bounds__(kBlock) void dot_part_k(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ y, double* part) {

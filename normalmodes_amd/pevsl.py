@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_bsolve", "bench_copy", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -387,6 +387,13 @@ def bench_bsolve(cheb: Handle, reps: int = 10, fp32: bool = False):
     _check(_lib().nmg_bench_bsolve(_i64(cheb), ctypes.c_int(reps), ctypes.c_int(int(fp32)), ctypes.byref(s),
                                    ctypes.byref(b)))
     return s.value, b.value
+
+
+def bench_copy(n: int, reps: int = 10) -> float:
+    """Mean seconds of a plain copy kernel of n doubles (16 n bytes)."""
+    sec = _d()
+    _check(_lib().nmg_bench_copy(_i64(n), ctypes.c_int(reps), ctypes.byref(sec)))
+    return sec.value
 
 
 def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):

@@ -209,7 +209,7 @@ def main():
     gate_ok = bool(r.res2.size > 0 and np.all(r.res2 <= GATE))
     chk = r.probe["chebstep"]
     achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
-    persistent = pv.chebiter_info(mv.chebB)["persistent"] and args.block in (1, 2, 4, 8, 16)
+    persistent = pv.chebiter_info(mv.chebB)["persistent"]
     ktag = f"{'cheb_persistent_k' if persistent else 'mass3_chebstep'}:{args.workload}"
     traffic, traffic_src = traffic_from_profiles(ktag)
     line = {

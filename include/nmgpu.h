@@ -114,13 +114,15 @@ int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
 /* pEVSL_chebiterstatsprint_f90(cheb)  src/mod_pevsl.f90:132-135 */
 int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
 int nmg_chebiter_free(nmg_handle cheb);
-/* How the solve runs (no reference counterpart): mode 1 (default) runs all
- * deg steps in one persistent cooperative launch when the matrix is a
- * scalar mass x I3 whose rows fit the workgroups' LDS (rows and values are
- * read from HBM once per solve, steps are separated by grid barriers);
- * mode 0 launches one fused kernel per step. */
+/* How the solve runs (no reference counterpart): mode 0 (default) launches
+ * one fused kernel per step; mode 1 runs all deg steps in one persistent
+ * cooperative launch when the matrix is a scalar mass x I3 whose rows fit
+ * the workgroups' LDS (rows and values are read from HBM once per solve,
+ * steps are separated by grid barriers; slower on MI355X at the sizes
+ * measured, because of the barrier cost). */
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
-/* whether the persistent solve applies (and its grid and LDS bytes) */
+/* persistent: 1 if the persistent solve is planned and selected, 2 if it
+ * could be selected (mode 0 now), 0 if it does not apply; grid, LDS bytes */
 int nmg_chebiter_info(nmg_handle cheb, int fp32, int *persistent, int *grid, int *lds_bytes);
 
 /* --- eigensolver ------------------------------------------------------- */

@@ -370,7 +370,7 @@ def apply_block(h: Handle, which: int, X):
 
 
 def chebiter_set_mode(cheb: Handle, mode: int):
-    """1: one persistent launch per solve where it applies; 0: one launch per step."""
+    """0 (default): one launch per step; 1: one persistent launch per solve where it applies."""
     _check(_lib().nmg_chebiter_set_mode(_i64(cheb), ctypes.c_int(mode)))
 
 
@@ -378,7 +378,7 @@ def chebiter_info(cheb: Handle, fp32: bool = False):
     p, g, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(_lib().nmg_chebiter_info(_i64(cheb), ctypes.c_int(int(fp32)), ctypes.byref(p), ctypes.byref(g),
                                     ctypes.byref(l)))
-    return dict(persistent=bool(p.value), grid=g.value, lds_bytes=l.value)
+    return dict(persistent=p.value == 1, plannable=p.value >= 1, grid=g.value, lds_bytes=l.value)
 
 
 def bench_bsolve(cheb: Handle, reps: int = 10, fp32: bool = False):

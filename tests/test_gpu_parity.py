@@ -129,7 +129,9 @@ def test_chebiter_persistent_matches_steps(small_p2, layered_p2):
                 c = pv.pevsl_setup_chebiter(0.25, 4.6, deg, h)
                 try:
                     info = pv.chebiter_info(c)
-                    assert info["persistent"] and info["grid"] >= 1, info
+                    assert not info["persistent"] and info["plannable"] and info["grid"] >= 1, info
+                    pv.chebiter_set_mode(c, 1)
+                    assert pv.chebiter_info(c)["persistent"]
                     x1 = pv.pevsl_chebiter(2, b, c)
                     pv.chebiter_set_mode(c, 0)
                     x0 = pv.pevsl_chebiter(2, b, c)
@@ -163,7 +165,7 @@ def test_chebiter_persistent_matches_steps(small_p2, layered_p2):
     g = pv.csr_create(A.rowptr, A.col, A.val)   # 3x3 blocks: not a scalar mass
     try:
         c = pv.pevsl_setup_chebiter(1.0, 2.0, 4, g)
-        assert not pv.chebiter_info(c)["persistent"]
+        assert not pv.chebiter_info(c)["plannable"]
         pv.pevsl_chebiter_free(c)
     finally:
         pv.csr_free(g)

@@ -900,8 +900,10 @@ static bool residual_stop(Solver& S, int kdim, const double* dC, const std::vect
 // (ntest, then every cycle steps) the trace of the Ritz values >= bar must
 // settle (|tr1 - tr0| < tol |tr1|), then every candidate's Ritz residual
 // |beta_k s_k| must be <= rtol, then (with a residual target) the extracted
-// pairs must meet it.  Returns 0 to go on, 1 to stop and extract, 2 when the
-// stop already left the extracted pairs in S.
+// pairs must meet it.  With no candidate, the stop comes when the largest
+// Ritz value has converged below bar (an empty interval).  Returns 0 to go
+// on, 1 to stop and extract, 2 when the stop already left the extracted
+// pairs in S.
 struct LanczosTest {
   double tr0 = 0.0;
   ResidualTracker rt;
@@ -920,7 +922,15 @@ static int lanczos_test(Solver& S, int kdim, const PolyFilter& P, const double* 
       tr1 += L.w[i];
       cnt++;
     }
-  const bool conv = cnt > 0 && std::fabs(tr1 - L.tr0) < tol * std::fabs(tr1);
+  if (cnt == 0) {
+    // no filtered Ritz value reaches bar: once the largest one has converged
+    // (|beta_k s_k| <= rtol) the top of p's spectrum lies below bar and the
+    // interval holds no eigenvalue
+    L.tr0 = 0.0;
+    tridiag_eigvec(kdim, L.al.data(), L.be.data(), L.w[kdim - 1], L.sv);
+    return std::fabs(L.be[kdim - 1] * L.sv[kdim - 1]) <= S.rtol ? 1 : 0;
+  }
+  const bool conv = std::fabs(tr1 - L.tr0) < tol * std::fabs(tr1);
   L.tr0 = tr1;
   if (!conv) return 0;
   double rmax = 0.0;

@@ -711,7 +711,21 @@ int or_cheblannr(const or_prob *p, int geneig, const double *xintv, int maxit, d
     int cnt = 0;
     for (int i = 0; i < kdim; ++i)
       if (w[i] >= bar) { tr1 += w[i]; cnt++; }
-    int conv = cnt > 0 && fabs(tr1 - tr0) < tol * fabs(tr1);
+    if (cnt == 0) {
+      /* empty interval: the largest Ritz value has converged below bar */
+      double *S0 = (double *)malloc(sizeof(double) * (size_t)kdim * kdim);
+      double *w0 = (double *)malloc(sizeof(double) * kdim);
+      or_tridiag_eig(kdim, al, be, w0, S0);
+      int imax = 0;
+      for (int i = 1; i < kdim; ++i)
+        if (w0[i] > w0[imax]) imax = i;
+      const double r0 = beta * fabs(S0[(size_t)imax * kdim + kdim - 1]);
+      free(S0); free(w0);
+      tr0 = 0.0;
+      if (r0 <= rtol) { k++; break; }
+      continue;
+    }
+    int conv = fabs(tr1 - tr0) < tol * fabs(tr1);
     tr0 = tr1;
     if (conv) {
       /* residual check of the candidates */

@@ -382,3 +382,19 @@ def test_fp32_bsolve_eigenvalues_match_fp64(small_p2, dense_eigs):
     assert r32.lam.size == r64.lam.size
     assert np.max(np.abs(r32.lam - r64.lam) / r64.lam) < 1e-12
     assert np.all(r32.res2 < 1e-4)
+
+
+def test_empty_interval_stops_early(small_p2, dense_eigs):
+    """An interval inside a spectral gap: no pair, and the stop comes from the
+    converged largest Ritz value, not from maxit."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    w = dense_eigs(small_p2)
+    nz = w[w > 1.0]
+    g = int(np.argmax(np.diff(nz[:30])))
+    a, b = nz[g] + 0.3 * (nz[g + 1] - nz[g]), nz[g] + 0.7 * (nz[g + 1] - nz[g])
+    mv = setupmatvec(small_p2, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(a, b), vectors=False)
+    finally:
+        mv.free()
+    assert r.lam.size == 0 and r.lan_steps < 200

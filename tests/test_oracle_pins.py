@@ -249,3 +249,18 @@ def test_start_vector_is_deterministic():
     assert np.array_equal(a, orc.start_vector(1000, 7))
     assert not np.array_equal(a, orc.start_vector(1000, 8))
     assert -1.0 <= a.min() and a.max() < 1.0 and abs(a.mean()) < 0.1
+
+
+def test_cheblannr_empty_interval_stops():
+    """An interval inside a spectral gap holds no eigenvalue: the oracle stops
+    once the largest filtered Ritz value has converged below the bar, long
+    before maxit, with no pair."""
+    import scipy.linalg as sl
+    from normalmodes_amd.workloads import build_problem
+    P = build_problem("tiny")
+    w = sl.eigh(P.A.to_scipy().toarray(), P.B.to_scipy().toarray(), eigvals_only=True)
+    nz = w[w > 1.0]
+    g = int(np.argmax(np.diff(nz[:30])))
+    a, b = nz[g] + 0.3 * (nz[g + 1] - nz[g]), nz[g] + 0.7 * (nz[g + 1] - nz[g])
+    lam, X, info = orc.solve_interval(P.A, P.B, (a, b), bdeg=58, return_info=True)
+    assert lam.size == 0 and info["nsteps"] < 200

@@ -1391,11 +1391,18 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
   maxit = (int)std::min<int64_t>(maxit, n);
   // filters
   std::vector<PolyFilter> pol(ns);
-  std::vector<std::array<double, 4>> xi(ns);
+  std::vector<std::array<double, 4>> xf(ns), xi(ns);
   int maxdeg = 0;
+  // a pair on an interior boundary can come out of Rayleigh-Ritz a rounding
+  // above (or below) it, while its filtered Ritz value sits at the bar of the
+  // neighbour's filter: extraction accepts pairs up to bnd_tol beyond an
+  // interior boundary, and the copies found by both neighbours are merged
+  auto bnd_tol = [](double x) { return 1e-9 * std::max(1.0, std::fabs(x)); };
   for (int i = 0; i < ns; ++i) {
-    xi[i] = {sli[i], sli[i + 1], lmin, lmax};
-    const int rc = find_pol(xi[i].data(), thr_int, thr_ext, 2, 10000, 2, pol[i]);
+    xf[i] = {sli[i], sli[i + 1], lmin, lmax};
+    xi[i] = {i > 0 ? sli[i] - bnd_tol(sli[i]) : sli[i], i + 1 < ns ? sli[i + 1] + bnd_tol(sli[i + 1]) : sli[i + 1],
+             lmin, lmax};
+    const int rc = find_pol(xf[i].data(), thr_int, thr_ext, 2, 10000, 2, pol[i]);
     NMG_CHECK(rc == NMG_OK, NMG_ENUM, "no filter meets the thresholds for slice " + std::to_string(i));
     maxdeg = std::max(maxdeg, pol[i].deg);
   }
@@ -1510,7 +1517,7 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
   bx.ensure(n);
   dd.ensure(1);
   for (int i = 0; i + 1 < ns; ++i) {
-    const double b = sli[i + 1], del = 1e-8 * std::max(1.0, std::fabs(b));
+    const double b = sli[i + 1], del = 2.0 * bnd_tol(b);
     Solver &L0 = *sub[i], &L1 = *sub[i + 1];
     for (int a = 0; a < L0.nev; ++a) {
       if (std::fabs(L0.lam[a] - b) > del) continue;

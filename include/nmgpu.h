@@ -115,9 +115,7 @@ int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
 int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
 int nmg_chebiter_free(nmg_handle cheb);
 /* How the solve runs (no reference counterpart): mode 0 (default) launches
- * one fused kernel per step; mode 2 advances two steps per launch on LDS
- * tiles (each workgroup stages x_k on its rows' 2-hop neighbourhood and
- * recomputes the 1-hop halo of x_{k+1}; scalar mass x I3 only); mode 1 runs all deg steps in one persistent
+ * one fused kernel per step; mode 1 runs all deg steps in one persistent
  * cooperative launch when the matrix is a scalar mass x I3 whose rows fit
  * the workgroups' LDS (rows and values are read from HBM once per solve,
  * steps are separated by grid barriers; slower on MI355X at the sizes
@@ -126,16 +124,6 @@ int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
 /* persistent: 1 if the persistent solve is planned and selected, 2 if it
  * could be selected (mode 0 now), 0 if it does not apply; grid, LDS bytes */
 int nmg_chebiter_info(nmg_handle cheb, int fp32, int *persistent, int *grid, int *lds_bytes);
-/* tiles of the two-step mode: tile_rows node rows per tile (0 keeps the
- * current size, default 64); ntiles = 0 when the mode does not apply. */
-int nmg_chebiter_tiles(nmg_handle cheb, int tile_rows, int *ntiles, int *max_n1, int *max_n2);
-/* The two-step tiling of a node pattern on the host (for checks): sizes =
- * {ntiles, nodes, row entries, local columns, max_n1, max_n2}; call with
- * NULL arrays to get the sizes, then again with arrays of those sizes
- * (tile_off, row_base: ntiles+1; tile_n: 3 ntiles). */
-int nmg_cheb2_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int tile_rows, int64_t *sizes,
-                   int32_t *tile_off, int32_t *row_base, int32_t *tile_n, int32_t *nodes, int64_t *row_g,
-                   int32_t *row_l, uint16_t *lcol);
 
 /* --- eigensolver ------------------------------------------------------- */
 /* pEVSL_LANBOUNDS_F90(h, mlan, maxit, tol, lmin, lmax)

@@ -357,18 +357,6 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-struct DVec {
-  double* p = nullptr;
-  int64_t n = 0;
-  void ensure(int64_t m) {
-    if (m <= n) return;
-    dfree(p);
-    p = dmalloc<double>(m);
-    n = m;
-  }
-  ~DVec() { dfree(p); }
-};
-
 struct Cheb {
   DevMat* mat = nullptr;
   double lmin = 0, lmax = 0;
@@ -383,19 +371,6 @@ struct Cheb {
   int mode = 0;
   PersistPlan plan[2];  // FP64 / FP32 values
   bool planned[2] = {false, false};
-  // mode 2: two steps per launch on LDS tiles (matrix powers) where planned
-  Cheb2Plan plan2;
-  bool planned2 = false;
-  int tile_rows = 64;
-  DVec w2[2];  // the two extra iterates of the two-step scheme
-  const Cheb2Plan* two_step() {
-    if (mode != 2) return nullptr;
-    if (!planned2) {
-      plan_cheb2(*mat, tile_rows, plan2);
-      planned2 = true;
-    }
-    return plan2.ok ? &plan2 : nullptr;
-  }
   const PersistPlan* persistent(bool fp32) {
     if (mode != 1) return nullptr;
     const int i = fp32 ? 1 : 0;
@@ -454,6 +429,17 @@ struct KernelProbe {
   }
 };
 
+struct DVec {
+  double* p = nullptr;
+  int64_t n = 0;
+  void ensure(int64_t m) {
+    if (m <= n) return;
+    dfree(p);
+    p = dmalloc<double>(m);
+    n = m;
+  }
+  ~DVec() { dfree(p); }
+};
 
 // Fluid-solid operator of sparsefsAV (src/mod_matvec.f90:498-520):
 //   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))],  C_p ~ Ap~^-1 by Chebyshev
@@ -573,40 +559,6 @@ static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipSt
     return;
   }
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
-  if (const Cheb2Plan* P2 = C.two_step()) {
-    // x_{k-1}, x_k -> x_{k+1}, x_{k+2} per launch over four rotating buffers
-    C.w2[0].ensure(n);
-    C.w2[1].ensure(n);
-    double* buf[4] = {tmp, out, C.w2[0].p, C.w2[1].p};
-    double* xm = buf[0];
-    double* xk = buf[1];
-    k_scale_copy(n, 1.0 / th, b, xk, s);
-    k_fill(n, 0.0, xm, s);
-    double w = 2.0;
-    int k = 1, rot = 0;
-    for (; k + 1 <= C.deg; k += 2) {
-      const double w1 = 1.0 / (1.0 - w / (4.0 * sg * sg));
-      const double w2 = 1.0 / (1.0 - w1 / (4.0 * sg * sg));
-      w = w2;
-      double* xc = buf[(rot + 2) % 4];
-      double* xd = buf[(rot + 3) % 4];
-      const bool pr = probe && probe->armed();
-      if (pr) probe->before(s);
-      k_cheb2(*C.mat, *P2, b, xm, xk, xc, xd, w1, w2, 1.0 / th, fp32, s);
-      if (pr) probe->after(s);
-      xm = xc;
-      xk = xd;
-      rot = (rot + 2) % 4;
-    }
-    if (k == C.deg) {  // odd degree: one more single step
-      w = 1.0 / (1.0 - w / (4.0 * sg * sg));
-      k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
-      std::swap(xk, xm);
-    }
-    if (xk != out) k_copy(n, xk, out, s);
-    if (probe) probe->bytes = 2.0 * step_bytes;
-    return;
-  }
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;
   k_scale_copy(n, 1.0 / th, b, xk, s);
@@ -2551,8 +2503,7 @@ int nmg_bench_copy(int64_t n, int reps, double* seconds) {
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);
-    NMG_CHECK(mode >= 0 && mode <= 2, NMG_EINVAL,
-              "mode must be 0 (one launch per step), 1 (persistent) or 2 (two steps per launch)");
+    NMG_CHECK(mode == 0 || mode == 1, NMG_EINVAL, "mode must be 0 (one launch per step) or 1 (persistent)");
     C->mode = mode;
   });
 }
@@ -2568,52 +2519,6 @@ int nmg_chebiter_info(nmg_handle cheb, int fp32, int* persistent, int* grid, int
     if (persistent) *persistent = (P && saved == 1) ? 1 : (P ? 2 : 0);
     if (grid) *grid = P ? P->grid : 0;
     if (lds_bytes) *lds_bytes = P ? (int)P->lds_bytes : 0;
-  });
-}
-
-int nmg_cheb2_host(int64_t nb, const int64_t* rowptr, const int32_t* col, int tile_rows, int64_t* sizes,
-                   int32_t* tile_off, int32_t* row_base, int32_t* tile_n, int32_t* nodes, int64_t* row_g,
-                   int32_t* row_l, uint16_t* lcol) {
-  return guarded([&] {
-    NMG_CHECK(nb > 0 && rowptr && col && sizes, NMG_EINVAL, "bad argument");
-    Cheb2Host H;
-    const int rc = build_cheb2_host(nb, rowptr, col, tile_rows, H);
-    NMG_CHECK(rc == NMG_OK, rc, "no two-step tiling within the LDS caps");
-    sizes[0] = (int64_t)H.tile_n.size() / 3;
-    sizes[1] = (int64_t)H.nodes.size();
-    sizes[2] = (int64_t)H.row_l.size();
-    sizes[3] = (int64_t)H.lcol.size();
-    sizes[4] = H.max_n1;
-    sizes[5] = H.max_n2;
-    auto cp = [](auto* dst, const auto& v) {
-      if (dst) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
-    };
-    cp(tile_off, H.tile_off);
-    cp(row_base, H.row_base);
-    cp(tile_n, H.tile_n);
-    cp(nodes, H.nodes);
-    cp(row_g, H.row_g);
-    cp(row_l, H.row_l);
-    cp(lcol, H.lcol);
-  });
-}
-
-int nmg_chebiter_tiles(nmg_handle cheb, int tile_rows, int* ntiles, int* max_n1, int* max_n2) {
-  return guarded([&] {
-    Cheb* C = get<Cheb>(cheb, K_CHEB);
-    NMG_CHECK(tile_rows >= 0, NMG_EINVAL, "tile_rows must be >= 0");
-    if (tile_rows > 0 && tile_rows != C->tile_rows) {
-      C->tile_rows = tile_rows;
-      C->planned2 = false;
-      C->plan2.release();
-    }
-    const int saved = C->mode;
-    C->mode = 2;
-    const Cheb2Plan* P = C->two_step();
-    C->mode = saved;
-    if (ntiles) *ntiles = P ? P->ntiles : 0;
-    if (max_n1) *max_n1 = P ? P->max_n1 : 0;
-    if (max_n2) *max_n2 = P ? P->max_n2 : 0;
   });
 }
 

@@ -105,40 +105,6 @@ void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P);
 void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp,
                        int deg, double lmin, double lmax, hipStream_t s);
 
-// Two-step Chebyshev tiles (matrix powers, scalar mass x I3, one column):
-// tile t owns node rows T; L1 = T and its neighbours, L2 = L1 and theirs.
-// Its node list is [T, L1 \ T, L2 \ L1] (n0 <= n1 <= n2 nodes); the rows of
-// L1 carry column indices local to that list.
-struct Cheb2Plan {
-  bool ok = false;
-  int ntiles = 0, max_n1 = 0, max_n2 = 0;
-  size_t lds_bytes = 0;
-  int32_t* tile_off = nullptr;   // ntiles+1 offsets into nodes and into rows
-  int32_t* row_base = nullptr;   // ntiles+1 offsets of each tile's L1 rows
-  int32_t* tile_n = nullptr;     // 3 per tile: n0, n1, n2
-  int32_t* nodes = nullptr;      // concatenated node lists (global node ids)
-  int64_t* row_g = nullptr;      // per L1 row: first entry in the matrix arrays
-  int32_t* row_l = nullptr;      // per L1 row (+1 per tile): offset into lcol
-  uint16_t* lcol = nullptr;      // local column indices
-  void release();
-  ~Cheb2Plan() { release(); }
-};
-// the same on the host (pure host code)
-struct Cheb2Host {
-  int max_n1 = 0, max_n2 = 0;
-  std::vector<int32_t> tile_off, row_base, tile_n, nodes, row_l;
-  std::vector<int64_t> row_g;
-  std::vector<uint16_t> lcol;
-};
-int build_cheb2_host(int64_t nb, const int64_t* rp, const int32_t* ci, int tile_rows, Cheb2Host& H);
-// plans tiles of about `tile_rows` node rows (ok stays false when M is not
-// MASS3 or a tile's closure exceeds the LDS caps)
-void plan_cheb2(const DevMat& M, int tile_rows, Cheb2Plan& P);
-// two Chebyshev steps in one launch: from x_{k-1} (xa), x_k (xb) to
-// x_{k+1} (xc) and x_{k+2} (xd) with the step weights w1 = w_{k+1}, w2 = w_{k+2}
-void k_cheb2(const DevMat& M, const Cheb2Plan& P, const double* b, const double* xa, const double* xb, double* xc,
-             double* xd, double w1, double w2, double ith, bool fp32, hipStream_t s);
-
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);
 // creates a DevMat from host CSR (0-based); picks the layout

@@ -1377,204 +1377,4 @@ void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const doubl
   }
 }
 
-// ---------------------------------------------------------------------------
-// Two Chebyshev steps per launch (matrix powers).  One workgroup per tile:
-// stage x_k on the tile's 2-hop node list in LDS, compute x_{k+1} on its
-// 1-hop list (the halo rows redundantly, from LDS only), then x_{k+2} on the
-// tile's own rows from the x_{k+1} values in LDS.  x_{k-1} and b are read
-// for the 1-hop rows; x_{k+1} and x_{k+2} of the own rows are written to
-// buffers that no workgroup reads in this launch.
-constexpr int kC2Block = 1024;
-constexpr int kC2G = 8;  // lanes per node row
-constexpr int kC2MaxN2 = 2048, kC2MaxN1 = 1024;
-
-void Cheb2Plan::release() {
-  for (void* p : {(void*)tile_off, (void*)row_base, (void*)tile_n, (void*)nodes, (void*)row_g, (void*)row_l,
-                  (void*)lcol})
-    if (p) (void)hipFree(p);
-  tile_off = row_base = tile_n = nodes = row_l = nullptr;
-  row_g = nullptr;
-  lcol = nullptr;
-  ok = false;
-}
-
-template <class T>
-static T* upload_vec(const std::vector<T>& v) {
-  T* d = nullptr;
-  NMG_HIP(hipMalloc(&d, sizeof(T) * std::max<size_t>(v.size(), 1)));
-  if (!v.empty()) NMG_HIP(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  return d;
-}
-
-int build_cheb2_host(int64_t nb, const int64_t* rp, const int32_t* ci, int tile_rows, Cheb2Host& H) {
-  H = Cheb2Host();
-  if (nb < 1 || tile_rows < 1) return NMG_EINVAL;
-  if (rp[nb] >= ((int64_t)1 << 31)) return NMG_ENOTIMPL;
-  std::vector<int32_t> mark(nb, -1);
-  H.tile_off.push_back(0);
-  H.row_base.push_back(0);
-  for (int64_t t0 = 0; t0 < nb; t0 += tile_rows) {
-    const int64_t t1 = std::min<int64_t>(nb, t0 + tile_rows);
-    const size_t base = H.nodes.size();
-    auto add = [&](int32_t v) {
-      if (mark[v] < 0) {
-        mark[v] = (int32_t)(H.nodes.size() - base);
-        H.nodes.push_back(v);
-      }
-    };
-    for (int64_t u = t0; u < t1; ++u) add((int32_t)u);
-    const int n0 = (int)(H.nodes.size() - base);
-    for (int i = 0; i < n0; ++i)
-      for (int64_t j = rp[H.nodes[base + i]]; j < rp[H.nodes[base + i] + 1]; ++j) add(ci[j]);
-    const int n1 = (int)(H.nodes.size() - base);
-    for (int i = n0; i < n1; ++i)
-      for (int64_t j = rp[H.nodes[base + i]]; j < rp[H.nodes[base + i] + 1]; ++j) add(ci[j]);
-    const int n2 = (int)(H.nodes.size() - base);
-    if (n2 > kC2MaxN2 || n1 > kC2MaxN1) {
-      for (size_t k = base; k < H.nodes.size(); ++k) mark[H.nodes[k]] = -1;
-      return NMG_ENOTIMPL;
-    }
-    // rows of L1 with local columns
-    for (int i = 0; i < n1; ++i) {
-      const int32_t u = H.nodes[base + i];
-      H.row_g.push_back(rp[u]);
-      H.row_l.push_back((int32_t)H.lcol.size());
-      for (int64_t j = rp[u]; j < rp[u + 1]; ++j) H.lcol.push_back((uint16_t)mark[ci[j]]);
-    }
-    H.row_l.push_back((int32_t)H.lcol.size());  // end of the tile's last row
-    H.row_g.push_back(0);                        // (keeps row_g indexed like row_l)
-    for (size_t k = base; k < H.nodes.size(); ++k) mark[H.nodes[k]] = -1;
-    H.tile_n.push_back(n0);
-    H.tile_n.push_back(n1);
-    H.tile_n.push_back(n2);
-    H.tile_off.push_back((int32_t)H.nodes.size());
-    H.row_base.push_back((int32_t)H.row_l.size());
-    H.max_n1 = std::max(H.max_n1, n1);
-    H.max_n2 = std::max(H.max_n2, n2);
-    if (H.lcol.size() >= ((size_t)1 << 31)) return NMG_ENOTIMPL;
-  }
-  return NMG_OK;
-}
-
-void plan_cheb2(const DevMat& M, int tile_rows, Cheb2Plan& P) {
-  P.release();
-  if (M.layout != L_MASS3 || M.h_browptr.empty() || tile_rows < 1) return;
-  Cheb2Host H;
-  if (build_cheb2_host(M.nb, M.h_browptr.data(), M.h_bcol.data(), tile_rows, H) != NMG_OK) return;
-  P.ntiles = (int)H.tile_n.size() / 3;
-  P.max_n1 = H.max_n1;
-  P.max_n2 = H.max_n2;
-  P.lds_bytes = sizeof(double) * 3 * ((size_t)H.max_n2 + (size_t)H.max_n1);
-  P.tile_off = upload_vec(H.tile_off);
-  P.row_base = upload_vec(H.row_base);
-  P.tile_n = upload_vec(H.tile_n);
-  P.nodes = upload_vec(H.nodes);
-  P.row_g = upload_vec(H.row_g);
-  P.row_l = upload_vec(H.row_l);
-  P.lcol = upload_vec(H.lcol);
-  P.ok = true;
-}
-
-template <class VT>
-__global__ __launch_bounds__(kC2Block) void cheb2_tile_k(const int32_t* __restrict__ tile_off,
-                                                         const int32_t* __restrict__ row_base,
-                                                         const int32_t* __restrict__ tile_n,
-                                                         const int32_t* __restrict__ nodes,
-                                                         const int64_t* __restrict__ row_g,
-                                                         const int32_t* __restrict__ row_l,
-                                                         const uint16_t* __restrict__ lcol,
-                                                         const VT* __restrict__ va, const double* __restrict__ b,
-                                                         const double* __restrict__ xa,
-                                                         const double* __restrict__ xb, double* __restrict__ xc,
-                                                         double* __restrict__ xd, int max_n2, double w1,
-                                                         double w2, double ith) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  double* xs = (double*)smem;       // x_k on L2: 3 * n2
-  double* xn = xs + 3 * max_n2;     // x_{k+1} on L1: 3 * n1
-  const int t = blockIdx.x;
-  const int n0 = tile_n[3 * t], n1 = tile_n[3 * t + 1], n2 = tile_n[3 * t + 2];
-  const int32_t* nd = nodes + tile_off[t];
-  const int rb0 = row_base[t];
-  for (int i = threadIdx.x; i < 3 * n2; i += kC2Block) xs[i] = xb[3 * (int64_t)nd[i / 3] + i % 3];
-  __syncthreads();
-  const int l = threadIdx.x % kC2G;
-  // step 1: x_{k+1} on L1
-  for (int r0 = 0; r0 < n1; r0 += kC2Block / kC2G) {
-    const int r = r0 + threadIdx.x / kC2G;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, pre_a = 0.0, pre_b = 0.0;
-    int64_t gi = 0;
-    if (r < n1) {
-      if (l < 3) {
-        gi = 3 * (int64_t)nd[r] + l;
-        pre_a = xa[gi];
-        pre_b = b[gi];
-      }
-      const int64_t g0 = row_g[rb0 + r];
-      const int ls = row_l[rb0 + r], le = row_l[rb0 + r + 1];
-      for (int j = ls + l; j < le; j += kC2G) {
-        const double v = (double)va[g0 + (j - ls)];
-        const double* xp = xs + 3 * (int)lcol[j];
-        s0 += v * xp[0];
-        s1 += v * xp[1];
-        s2 += v * xp[2];
-      }
-    }
-#pragma unroll
-    for (int off = kC2G / 2; off > 0; off >>= 1) {
-      s0 += __shfl_xor(s0, off, kC2G);
-      s1 += __shfl_xor(s1, off, kC2G);
-      s2 += __shfl_xor(s2, off, kC2G);
-    }
-    if (r < n1 && l < 3) {
-      const double sv = l == 0 ? s0 : (l == 1 ? s1 : s2);
-      const double xk = xs[3 * r + l];
-      xn[3 * r + l] = pre_a + w1 * (xk - pre_a + (pre_b - sv) * ith);
-    }
-  }
-  __syncthreads();
-  // step 2: x_{k+2} on the own rows
-  for (int r0 = 0; r0 < n0; r0 += kC2Block / kC2G) {
-    const int r = r0 + threadIdx.x / kC2G;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    if (r < n0) {
-      const int64_t g0 = row_g[rb0 + r];
-      const int ls = row_l[rb0 + r], le = row_l[rb0 + r + 1];
-      for (int j = ls + l; j < le; j += kC2G) {
-        const double v = (double)va[g0 + (j - ls)];
-        const double* xp = xn + 3 * (int)lcol[j];
-        s0 += v * xp[0];
-        s1 += v * xp[1];
-        s2 += v * xp[2];
-      }
-    }
-#pragma unroll
-    for (int off = kC2G / 2; off > 0; off >>= 1) {
-      s0 += __shfl_xor(s0, off, kC2G);
-      s1 += __shfl_xor(s1, off, kC2G);
-      s2 += __shfl_xor(s2, off, kC2G);
-    }
-    if (r < n0 && l < 3) {
-      const int64_t gi = 3 * (int64_t)nd[r] + l;
-      const double sv = l == 0 ? s0 : (l == 1 ? s1 : s2);
-      const double x1 = xn[3 * r + l], x0 = xs[3 * r + l];
-      xc[gi] = x1;
-      xd[gi] = x0 + w2 * (x1 - x0 + (b[gi] - sv) * ith);
-    }
-  }
-}
-
-void k_cheb2(const DevMat& M, const Cheb2Plan& P, const double* b, const double* xa, const double* xb, double* xc,
-             double* xd, double w1, double w2, double ith, bool fp32, hipStream_t s) {
-  NMG_CHECK(P.ok, NMG_ESTATE, "two-step Chebyshev tiles not planned");
-  if (fp32 && M.bval32)
-    hipLaunchKernelGGL(cheb2_tile_k<float>, dim3(P.ntiles), dim3(kC2Block), P.lds_bytes, s, P.tile_off, P.row_base,
-                       P.tile_n, P.nodes, P.row_g, P.row_l, P.lcol, (const float*)M.bval32, b, xa, xb, xc, xd,
-                       P.max_n2, w1, w2, ith);
-  else
-    hipLaunchKernelGGL(cheb2_tile_k<double>, dim3(P.ntiles), dim3(kC2Block), P.lds_bytes, s, P.tile_off,
-                       P.row_base, P.tile_n, P.nodes, P.row_g, P.row_l, P.lcol, (const double*)M.bval, b, xa, xb,
-                       xc, xd, P.max_n2, w1, w2, ith);
-  NMG_HIP(hipGetLastError());
-}
-
 }  // namespace nmg

@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_copy", "chebiter_tiles", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_bsolve", "bench_copy", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -370,7 +370,7 @@ def apply_block(h: Handle, which: int, X):
 
 
 def chebiter_set_mode(cheb: Handle, mode: int):
-    """0 (default): one launch per step; 1: one persistent launch per solve; 2: two steps per launch (LDS tiles)."""
+    """0 (default): one launch per step; 1: one persistent launch per solve where it applies."""
     _check(_lib().nmg_chebiter_set_mode(_i64(cheb), ctypes.c_int(mode)))
 
 
@@ -379,14 +379,6 @@ def chebiter_info(cheb: Handle, fp32: bool = False):
     _check(_lib().nmg_chebiter_info(_i64(cheb), ctypes.c_int(int(fp32)), ctypes.byref(p), ctypes.byref(g),
                                     ctypes.byref(l)))
     return dict(persistent=p.value == 1, plannable=p.value >= 1, grid=g.value, lds_bytes=l.value)
-
-
-def chebiter_tiles(cheb: Handle, tile_rows: int = 0):
-    """Tiles of the two-step mode (mode 2): ntiles (0: not applicable), max 1-hop and 2-hop nodes."""
-    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    _check(_lib().nmg_chebiter_tiles(_i64(cheb), ctypes.c_int(tile_rows), ctypes.byref(a), ctypes.byref(b),
-                                     ctypes.byref(c)))
-    return dict(ntiles=a.value, max_n1=b.value, max_n2=c.value)
 
 
 def bench_bsolve(cheb: Handle, reps: int = 10, fp32: bool = False):

@@ -171,36 +171,6 @@ def test_chebiter_persistent_matches_steps(small_p2, layered_p2):
         pv.csr_free(g)
 
 
-def test_chebiter_two_step_matches_steps(small_p2, layered_p2):
-    """Two steps per launch on LDS tiles (mode 2) against one launch per step:
-    odd and even degrees, several tile sizes, FP64 and FP32 values."""
-    pv = _pv()
-    for P in (small_p2, layered_p2):
-        B = P.B
-        h = pv.csr_create(B.rowptr, B.col, B.val)
-        d_ref, val_ref = orc.diagscale(B, 1.0)
-        try:
-            pv.csr_diagscale(h, 1.0)
-            b = np.random.default_rng(4).standard_normal(B.n_rows)
-            for deg in (1, 2, 3, 58):
-                c = pv.pevsl_setup_chebiter(0.25, 4.6, deg, h)
-                try:
-                    x0 = pv.pevsl_chebiter(2, b, c)
-                    for tile_rows in (16, 64, 200):
-                        info = pv.chebiter_tiles(c, tile_rows)
-                        assert info["ntiles"] > 0 and info["max_n2"] <= 2048, info
-                        pv.chebiter_set_mode(c, 2)
-                        x2 = pv.pevsl_chebiter(2, b, c)
-                        pv.chebiter_set_mode(c, 0)
-                        assert _rel(x2, x0) < 1e-14, (deg, tile_rows)
-                    ref = orc.chebiter(orc.CSRView(B.rowptr, B.col, val_ref), 0.25, 4.6, deg, b)
-                    assert _rel(x2, ref) < 1e-13, deg
-                finally:
-                    pv.pevsl_chebiter_free(c)
-        finally:
-            pv.csr_free(h)
-
-
 def test_findpol_matches_oracle():
     pv = _pv()
     for xintv in ([0.5, 12.0, -0.1, 12600.0], [3000.0, 3200.0, 0.0, 12600.0], [11000, 12600, 0, 12600]):

@@ -243,6 +243,11 @@ int nmg_bench_copy(int64_t n, int reps, double *seconds);
  * cheb's current mode; bytes: deg x the algorithmic bytes of one step. */
 int nmg_bench_bsolve(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double *seconds, double *bytes);
+/* Workgroup order of the sparse-product kernels of mat (no reference
+ * counterpart): 1 (default) gives each of the 8 XCDs one contiguous range
+ * of rows, so rows that gather the same x share an L2; 0 keeps the plain
+ * round-robin order. */
+int nmg_set_xcd_order(nmg_handle mat, int on);
 /* Kernel mapping of a matrix (tuning): lanes per row of the single-column
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */

@@ -351,6 +351,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
     cp(S.bval, M->bval, planes * S.nbnz);
   }
   M->single_blocked = S.single_blocked;
+  M->xcd = S.xcd;
   for (int i = 0; i < 5; ++i)
     if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave);
   return M.release();
@@ -2266,6 +2267,13 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double* X, double* Y) 
     k_from_interleaved(n, K, S->bw[5].p, cm.p, n, S->s);
     NMG_HIP(hipMemcpyAsync(Y, cm.p, sizeof(double) * n * K, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_set_xcd_order(nmg_handle mat, int on) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    M->xcd = on ? 1 : 0;
   });
 }
 

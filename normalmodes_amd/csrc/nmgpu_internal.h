@@ -74,6 +74,7 @@ struct DevMat {
   int block_map[5] = {0, 0, 0, 0, 0};
   Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
+  int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)
   std::vector<int64_t> h_browptr;  // host copy of the node pattern (tiling)
   std::vector<int32_t> h_bcol;
   ~DevMat();

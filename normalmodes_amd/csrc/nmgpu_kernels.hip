@@ -20,6 +20,19 @@ namespace nmg {
 
 constexpr int kBlock = 256;
 constexpr int kDotBlocks = 512;
+constexpr int kXcd = 8;  // accelerator complex dies (XCDs) of an MI355X
+
+// Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch).  With
+// remap != 0 the logical block index gives each XCD one contiguous range,
+// so neighbouring node rows -- which gather the same x entries -- share one
+// XCD's L2 instead of being spread over eight.  A bijection of [0, grid).
+__device__ __forceinline__ int64_t xcd_block(int remap) {
+  const int64_t b = blockIdx.x;
+  if (!remap) return b;
+  const int64_t n = gridDim.x, q = n / kXcd, r = n % kXcd;
+  const int64_t x = b % kXcd, i = b / kXcd;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
 
 // --------------------------------------------------------------------------
 template <int G>
@@ -88,8 +101,8 @@ template <int G, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void csr_rows(int64_t nrow, const int64_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci,
                                                    const VT* __restrict__ va,
-                                                   const double* __restrict__ x, Epi epi) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                   const double* __restrict__ x, Epi epi, int xcd) {
+  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -109,8 +122,8 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
                                                     const int64_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci,
                                                     const VT* __restrict__ bv,
-                                                    const double* __restrict__ x, Epi epi) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                    const double* __restrict__ x, Epi epi, int xcd) {
+  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -139,8 +152,8 @@ template <int G, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                      const int32_t* __restrict__ ci,
                                                      const VT* __restrict__ mv,
-                                                     const double* __restrict__ x, Epi epi) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                     const double* __restrict__ x, Epi epi, int xcd) {
+  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -177,17 +190,17 @@ static void launch_G(const DevMat& M, const VT* v, const double* x, Epi epi, hip
     const int64_t grid = (thr + kBlock - 1) / kBlock;
     if (grid > 0)
       hipLaunchKernelGGL((csr_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
-                         M.rowptr, M.col, v, x, epi);
+                         M.rowptr, M.col, v, x, epi, M.xcd);
   } else {
     const int64_t thr = M.nb * G;
     const int64_t grid = (thr + kBlock - 1) / kBlock;
     if (grid <= 0) return;
     if (M.layout == L_BSR3)
       hipLaunchKernelGGL((bsr3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                         M.nbnz, M.browptr, M.bcol, v, x, epi);
+                         M.nbnz, M.browptr, M.bcol, v, x, epi, M.xcd);
     else
       hipLaunchKernelGGL((mass3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                         M.browptr, M.bcol, v, x, epi);
+                         M.browptr, M.bcol, v, x, epi, M.xcd);
   }
 }
 
@@ -723,10 +736,10 @@ template <int K, int R, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                            const int32_t* __restrict__ ci,
                                                            const VT* __restrict__ mv,
-                                                           const double* __restrict__ x, Epi epi) {
+                                                           const double* __restrict__ x, Epi epi, int xcd) {
   constexpr int E = BlockMap<K, R>::E, U = BlockMap<K, R>::U;
   const int lane = threadIdx.x & 63;
-  const int64_t row = (((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int64_t row = ((xcd_block(xcd) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
   const int c = lane % K, e = (lane / K) % E;
   const bool live = row < nb;
   typename Epi::Pre pre[(3 + E - 1) / E];
@@ -776,11 +789,11 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
                                                           const int64_t* __restrict__ rp,
                                                           const int32_t* __restrict__ ci,
                                                           const VT* __restrict__ bv,
-                                                          const double* __restrict__ x, Epi epi) {
+                                                          const double* __restrict__ x, Epi epi, int xcd) {
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U;  // 9 values per entry
   const int lane = threadIdx.x & 63;
-  const int64_t row = (((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int64_t row = ((xcd_block(xcd) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
   const int c = lane % K, e = (lane / K) % E;
   const bool live = row < nb;
   typename Epi::Pre pre[(3 + E - 1) / E];
@@ -832,9 +845,9 @@ template <int K, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int64_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci,
                                                          const VT* __restrict__ va,
-                                                         const double* __restrict__ x, Epi epi) {
+                                                         const double* __restrict__ x, Epi epi, int xcd) {
   constexpr int E = 64 / K;
-  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t row = (xcd_block(xcd) * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int c = lane % K, e = lane / K;
   const bool wr = row < nrow && e == 0;
@@ -864,10 +877,10 @@ static void launch_block_R(const DevMat& M, const VT* v, const double* x, Epi ep
   if (grid <= 0) return;
   if (M.layout == L_BSR3)
     hipLaunchKernelGGL((bsr3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                       M.nbnz, M.browptr, M.bcol, v, x, epi);
+                       M.nbnz, M.browptr, M.bcol, v, x, epi, M.xcd);
   else
     hipLaunchKernelGGL((mass3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                       M.browptr, M.bcol, v, x, epi);
+                       M.browptr, M.bcol, v, x, epi, M.xcd);
 }
 
 template <int K, class VT, class Epi>
@@ -876,7 +889,7 @@ static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi,
     const int64_t grid = (M.nrow * 64 + kBlock - 1) / kBlock;
     if (grid > 0)
       hipLaunchKernelGGL((csr_block_rows<K, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
-                         M.rowptr, M.col, v, x, epi);
+                         M.rowptr, M.col, v, x, epi, M.xcd);
   } else {
     const int R = block_rows(M, K);
     if (R == 1) launch_block_R<K, 1>(M, v, x, epi, s);
@@ -993,12 +1006,12 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
                                                            const int64_t* __restrict__ rp,
                                                            const uint16_t* __restrict__ lcol,
                                                            const VT* __restrict__ val, int64_t nbnz,
-                                                           const double* __restrict__ x, Epi epi) {
+                                                           const double* __restrict__ x, Epi epi, int xcd) {
   extern __shared__ double xs[];
   constexpr int W3 = 3 * K;
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BSR ? (BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U) : BlockMap<K, R>::U;
-  const int64_t tile = blockIdx.x;
+  const int64_t tile = xcd_block(xcd);
   const int64_t r0 = trp[tile], r1 = trp[tile + 1];
   const int64_t c0 = tcp[tile];
   const int nu = (int)(tcp[tile + 1] - c0);
@@ -1060,10 +1073,10 @@ static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const 
   const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
   if (M.layout == L_BSR3)
     hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true>), dim3((unsigned)T.ntiles), dim3(kTileBlock), shmem,
-                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi);
+                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, M.xcd);
   else
     hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false>), dim3((unsigned)T.ntiles), dim3(kTileBlock),
-                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi);
+                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, M.xcd);
 }
 
 template <int K, class VT, class Epi>

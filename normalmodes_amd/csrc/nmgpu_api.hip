@@ -2342,9 +2342,10 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   auto consider = [&](double t, int kind, int group, int rows, int cap, int64_t nt) {
     if (t < best.seconds) best = TuneResult{t, kind, group, rows, cap, nt};
   };
-  // untiled layouts
+  // untiled layouts (the single-column route is K = 1's choice alone: tuning
+  // another width leaves it as it is)
   build_tiling(M, K, 0, 1);
-  M.single_blocked = false;
+  if (K == 1) M.single_blocked = false;
   if (K == 1) {
     for (int g : {8, 16, 32, 64}) {
       M.group = g;

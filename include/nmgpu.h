@@ -248,6 +248,10 @@ int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double 
  * of rows, so rows that gather the same x share an L2; 0 keeps the plain
  * round-robin order. */
 int nmg_set_xcd_order(nmg_handle mat, int on);
+/* Matrix values and indices of mat read with non-temporal (streaming)
+ * loads (1) or ordinary loads (0, default): a product streams them once, so
+ * streaming loads leave the caches to the gathered x. */
+int nmg_set_stream_loads(nmg_handle mat, int on);
 /* Kernel mapping of a matrix (tuning): lanes per row of the single-column
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */

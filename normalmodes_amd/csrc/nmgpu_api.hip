@@ -352,6 +352,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   }
   M->single_blocked = S.single_blocked;
   M->xcd = S.xcd;
+  M->nt = S.nt;
   for (int i = 0; i < 5; ++i)
     if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave);
   return M.release();
@@ -2274,6 +2275,13 @@ int nmg_set_xcd_order(nmg_handle mat, int on) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     M->xcd = on ? 1 : 0;
+  });
+}
+
+int nmg_set_stream_loads(nmg_handle mat, int on) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    M->nt = on ? 1 : 0;
   });
 }
 

@@ -75,6 +75,7 @@ struct DevMat {
   Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
   int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)
+  int nt = 0;                   // non-temporal matrix loads in the row kernels
   std::vector<int64_t> h_browptr;  // host copy of the node pattern (tiling)
   std::vector<int32_t> h_bcol;
   ~DevMat();
@@ -105,6 +106,9 @@ void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P);
 // interleaved columns (K = 1, 2, 4, 8, 16)
 void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp,
                        int deg, double lmin, double lmax, hipStream_t s);
+
+// kernel flags of M's sparse products (bit 0 XCD order, bit 1 streaming loads)
+inline int kflags(const DevMat& M) { return (M.xcd ? 1 : 0) | (M.nt ? 2 : 0); }
 
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);

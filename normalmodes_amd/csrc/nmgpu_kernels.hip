@@ -22,16 +22,24 @@ constexpr int kBlock = 256;
 constexpr int kDotBlocks = 512;
 constexpr int kXcd = 8;  // accelerator complex dies (XCDs) of an MI355X
 
+// Kernel flags of the sparse products: bit 0 XCD-contiguous workgroup order,
+// bit 1 non-temporal (streaming) loads of the matrix values and indices.
 // Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch).  With
-// remap != 0 the logical block index gives each XCD one contiguous range,
+// bit 0 set the logical block index gives each XCD one contiguous range,
 // so neighbouring node rows -- which gather the same x entries -- share one
 // XCD's L2 instead of being spread over eight.  A bijection of [0, grid).
-__device__ __forceinline__ int64_t xcd_block(int remap) {
+__device__ __forceinline__ int64_t xcd_block(int flags) {
   const int64_t b = blockIdx.x;
-  if (!remap) return b;
+  if (!(flags & 1)) return b;
   const int64_t n = gridDim.x, q = n / kXcd, r = n % kXcd;
   const int64_t x = b % kXcd, i = b / kXcd;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+// matrix loads: streamed once per product, so with bit 1 they are issued as
+// non-temporal loads that do not displace the gathered x from the caches
+template <class T>
+__device__ __forceinline__ T ldm(const T* p, int flags) {
+  return (flags & 2) ? __builtin_nontemporal_load(p) : *p;
 }
 
 // --------------------------------------------------------------------------
@@ -101,8 +109,8 @@ template <int G, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void csr_rows(int64_t nrow, const int64_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci,
                                                    const VT* __restrict__ va,
-                                                   const double* __restrict__ x, Epi epi, int xcd) {
-  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
+                                                   const double* __restrict__ x, Epi epi, int flags) {
+  const int64_t t = xcd_block(flags) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -111,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int64_t nrow, const int64_t* 
   double s = 0.0;
   if (row < nrow) {
     const int64_t e = rp[row + 1];
-    for (int64_t k = rp[row] + lane; k < e; k += G) s += (double)va[k] * x[ci[k]];
+    for (int64_t k = rp[row] + lane; k < e; k += G) s += (double)ldm(va + k, flags) * x[ldm(ci + k, flags)];
   }
   s = group_sum<G>(s);
   if (wr) epi.store(row, s, pre);
@@ -122,8 +130,8 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
                                                     const int64_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci,
                                                     const VT* __restrict__ bv,
-                                                    const double* __restrict__ x, Epi epi, int xcd) {
-  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
+                                                    const double* __restrict__ x, Epi epi, int flags) {
+  const int64_t t = xcd_block(flags) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -133,13 +141,14 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
   if (row < nb) {
     const int64_t e = rp[row + 1];
     for (int64_t k = rp[row] + lane; k < e; k += G) {
-      const int64_t c3 = 3 * (int64_t)ci[k];
+      const int64_t c3 = 3 * (int64_t)ldm(ci + k, flags);
+      double v[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[q] = (double)ldm(bv + q * nbnz + k, flags);
       const double x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
-      y0 += (double)bv[k] * x0 + (double)bv[nbnz + k] * x1 + (double)bv[2 * nbnz + k] * x2;
-      y1 += (double)bv[3 * nbnz + k] * x0 + (double)bv[4 * nbnz + k] * x1 +
-            (double)bv[5 * nbnz + k] * x2;
-      y2 += (double)bv[6 * nbnz + k] * x0 + (double)bv[7 * nbnz + k] * x1 +
-            (double)bv[8 * nbnz + k] * x2;
+      y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+      y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+      y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
     }
   }
   y0 = group_sum<G>(y0);
@@ -152,8 +161,8 @@ template <int G, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                      const int32_t* __restrict__ ci,
                                                      const VT* __restrict__ mv,
-                                                     const double* __restrict__ x, Epi epi, int xcd) {
-  const int64_t t = xcd_block(xcd) * kBlock + threadIdx.x;
+                                                     const double* __restrict__ x, Epi epi, int flags) {
+  const int64_t t = xcd_block(flags) * kBlock + threadIdx.x;
   const int64_t row = t / G;
   const int lane = threadIdx.x & (G - 1);
   typename Epi::Pre pre{};
@@ -163,8 +172,8 @@ __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* 
   if (row < nb) {
     const int64_t e = rp[row + 1];
     for (int64_t k = rp[row] + lane; k < e; k += G) {
-      const int64_t c3 = 3 * (int64_t)ci[k];
-      const double m = (double)mv[k];
+      const int64_t c3 = 3 * (int64_t)ldm(ci + k, flags);
+      const double m = (double)ldm(mv + k, flags);
       y0 += m * x[c3];
       y1 += m * x[c3 + 1];
       y2 += m * x[c3 + 2];
@@ -190,17 +199,17 @@ static void launch_G(const DevMat& M, const VT* v, const double* x, Epi epi, hip
     const int64_t grid = (thr + kBlock - 1) / kBlock;
     if (grid > 0)
       hipLaunchKernelGGL((csr_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
-                         M.rowptr, M.col, v, x, epi, M.xcd);
+                         M.rowptr, M.col, v, x, epi, kflags(M));
   } else {
     const int64_t thr = M.nb * G;
     const int64_t grid = (thr + kBlock - 1) / kBlock;
     if (grid <= 0) return;
     if (M.layout == L_BSR3)
       hipLaunchKernelGGL((bsr3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                         M.nbnz, M.browptr, M.bcol, v, x, epi, M.xcd);
+                         M.nbnz, M.browptr, M.bcol, v, x, epi, kflags(M));
     else
       hipLaunchKernelGGL((mass3_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                         M.browptr, M.bcol, v, x, epi, M.xcd);
+                         M.browptr, M.bcol, v, x, epi, kflags(M));
   }
 }
 
@@ -736,10 +745,10 @@ template <int K, int R, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                            const int32_t* __restrict__ ci,
                                                            const VT* __restrict__ mv,
-                                                           const double* __restrict__ x, Epi epi, int xcd) {
+                                                           const double* __restrict__ x, Epi epi, int flags) {
   constexpr int E = BlockMap<K, R>::E, U = BlockMap<K, R>::U;
   const int lane = threadIdx.x & 63;
-  const int64_t row = ((xcd_block(xcd) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int64_t row = ((xcd_block(flags) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
   const int c = lane % K, e = (lane / K) % E;
   const bool live = row < nb;
   typename Epi::Pre pre[(3 + E - 1) / E];
@@ -756,8 +765,8 @@ __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int
       for (int u = 0; u < U; ++u) {
         const int64_t k = k0 + (int64_t)u * E;
         const bool ok = k < end;
-        base[u] = ok ? 3 * (int64_t)ci[k] * K + c : (int64_t)c;
-        m[u] = ok ? (double)mv[k] : 0.0;
+        base[u] = ok ? 3 * (int64_t)ldm(ci + k, flags) * K + c : (int64_t)c;
+        m[u] = ok ? (double)ldm(mv + k, flags) : 0.0;
       }
       double xa[U], xb[U], xc[U];
 #pragma unroll
@@ -789,11 +798,11 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
                                                           const int64_t* __restrict__ rp,
                                                           const int32_t* __restrict__ ci,
                                                           const VT* __restrict__ bv,
-                                                          const double* __restrict__ x, Epi epi, int xcd) {
+                                                          const double* __restrict__ x, Epi epi, int flags) {
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U;  // 9 values per entry
   const int lane = threadIdx.x & 63;
-  const int64_t row = ((xcd_block(xcd) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
+  const int64_t row = ((xcd_block(flags) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
   const int c = lane % K, e = (lane / K) % E;
   const bool live = row < nb;
   typename Epi::Pre pre[(3 + E - 1) / E];
@@ -811,13 +820,13 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
         const int64_t k = k0 + (int64_t)u * E;
         ok[u] = k < end;
         kk[u] = ok[u] ? k : beg;
-        base[u] = ok[u] ? 3 * (int64_t)ci[k] * K + c : (int64_t)c;
+        base[u] = ok[u] ? 3 * (int64_t)ldm(ci + k, flags) * K + c : (int64_t)c;
       }
       double a[U][9], xa[U], xb[U], xc[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)bv[p * nbnz + kk[u]] : 0.0;
+        for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)ldm(bv + p * nbnz + kk[u], flags) : 0.0;
         xa[u] = x[base[u]];
         xb[u] = x[base[u] + K];
         xc[u] = x[base[u] + 2 * K];
@@ -845,9 +854,9 @@ template <int K, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int64_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci,
                                                          const VT* __restrict__ va,
-                                                         const double* __restrict__ x, Epi epi, int xcd) {
+                                                         const double* __restrict__ x, Epi epi, int flags) {
   constexpr int E = 64 / K;
-  const int64_t row = (xcd_block(xcd) * kBlock + threadIdx.x) >> 6;
+  const int64_t row = (xcd_block(flags) * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int c = lane % K, e = lane / K;
   const bool wr = row < nrow && e == 0;
@@ -856,7 +865,8 @@ __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int
   double s = 0.0;
   if (row < nrow) {
     const int64_t end = rp[row + 1];
-    for (int64_t k = rp[row] + e; k < end; k += E) s += (double)va[k] * x[(int64_t)ci[k] * K + c];
+    for (int64_t k = rp[row] + e; k < end; k += E)
+      s += (double)ldm(va + k, flags) * x[(int64_t)ldm(ci + k, flags) * K + c];
   }
   s = slot_sum<K>(s);
   if (wr) epi.store(row, c, s, pre);
@@ -877,10 +887,10 @@ static void launch_block_R(const DevMat& M, const VT* v, const double* x, Epi ep
   if (grid <= 0) return;
   if (M.layout == L_BSR3)
     hipLaunchKernelGGL((bsr3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                       M.nbnz, M.browptr, M.bcol, v, x, epi, M.xcd);
+                       M.nbnz, M.browptr, M.bcol, v, x, epi, kflags(M));
   else
     hipLaunchKernelGGL((mass3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
-                       M.browptr, M.bcol, v, x, epi, M.xcd);
+                       M.browptr, M.bcol, v, x, epi, kflags(M));
 }
 
 template <int K, class VT, class Epi>
@@ -889,7 +899,7 @@ static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi,
     const int64_t grid = (M.nrow * 64 + kBlock - 1) / kBlock;
     if (grid > 0)
       hipLaunchKernelGGL((csr_block_rows<K, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
-                         M.rowptr, M.col, v, x, epi, M.xcd);
+                         M.rowptr, M.col, v, x, epi, kflags(M));
   } else {
     const int R = block_rows(M, K);
     if (R == 1) launch_block_R<K, 1>(M, v, x, epi, s);
@@ -1006,12 +1016,12 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
                                                            const int64_t* __restrict__ rp,
                                                            const uint16_t* __restrict__ lcol,
                                                            const VT* __restrict__ val, int64_t nbnz,
-                                                           const double* __restrict__ x, Epi epi, int xcd) {
+                                                           const double* __restrict__ x, Epi epi, int flags) {
   extern __shared__ double xs[];
   constexpr int W3 = 3 * K;
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BSR ? (BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U) : BlockMap<K, R>::U;
-  const int64_t tile = xcd_block(xcd);
+  const int64_t tile = xcd_block(flags);
   const int64_t r0 = trp[tile], r1 = trp[tile + 1];
   const int64_t c0 = tcp[tile];
   const int nu = (int)(tcp[tile + 1] - c0);
@@ -1040,14 +1050,17 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
         for (int u = 0; u < U; ++u) {
           const int64_t k = k0 + (int64_t)u * E;
           if (k < end) {
-            const double* xn = xs + (int)lcol[k] * W3 + c;
+            const double* xn = xs + (int)ldm(lcol + k, flags) * W3 + c;
             const double xa = xn[0], xb = xn[K], xc = xn[2 * K];
             if (BSR) {
-              y0 += (double)val[k] * xa + (double)val[nbnz + k] * xb + (double)val[2 * nbnz + k] * xc;
-              y1 += (double)val[3 * nbnz + k] * xa + (double)val[4 * nbnz + k] * xb + (double)val[5 * nbnz + k] * xc;
-              y2 += (double)val[6 * nbnz + k] * xa + (double)val[7 * nbnz + k] * xb + (double)val[8 * nbnz + k] * xc;
+              double v[9];
+#pragma unroll
+              for (int q = 0; q < 9; ++q) v[q] = (double)ldm(val + q * nbnz + k, flags);
+              y0 += v[0] * xa + v[1] * xb + v[2] * xc;
+              y1 += v[3] * xa + v[4] * xb + v[5] * xc;
+              y2 += v[6] * xa + v[7] * xb + v[8] * xc;
             } else {
-              const double m = (double)val[k];
+              const double m = (double)ldm(val + k, flags);
               y0 += m * xa;
               y1 += m * xb;
               y2 += m * xc;
@@ -1073,10 +1086,10 @@ static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const 
   const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
   if (M.layout == L_BSR3)
     hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true>), dim3((unsigned)T.ntiles), dim3(kTileBlock), shmem,
-                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, M.xcd);
+                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
   else
     hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false>), dim3((unsigned)T.ntiles), dim3(kTileBlock),
-                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, M.xcd);
+                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
 }
 
 template <int K, class VT, class Epi>

@@ -72,12 +72,25 @@ def traffic_from_profiles(kernel_tag):
     return None, None
 
 
+def upload_with_layouts(P, layouts, tune):
+    """Upload A and B; autotune, or apply fixed single-column layouts."""
+    from normalmodes_amd import driver, pevsl as pv
+    if layouts == "auto":
+        return driver.upload(P, tune=tune)
+    mv = driver.upload(P, tune=())
+    for part in layouts.split(","):
+        key, g = part.split(":")
+        pv.set_kernel_map(mv.matA if key.strip() == "A" else mv.matB, group=int(g))
+        mv.tuned[f"{key.strip()}1"] = {"kind": "lanes_per_row", "rows": int(g), "fixed": True}
+    return mv
+
+
 def kernel_only(args):
     """Copy kernel of known size, then the dominant kernel alone."""
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem
     P = build_problem(args.workload)
-    mv = driver.upload(P, tune=(1,))
+    mv = upload_with_layouts(P, args.layouts, (1,))
     driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
     n_copy = 1 << 24   # 16M doubles: 128 MiB read + 128 MiB written per launch
     t_copy = pv.bench_copy(n_copy, 5)
@@ -170,6 +183,10 @@ def main():
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
+    ap.add_argument("--layouts", default="auto",
+                    help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
+                         "'A:<g>,B:<g>' (g in 8,16,32,64), e.g. for profiler runs, where timing-based tuning "
+                         "is perturbed")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -191,7 +208,7 @@ def main():
     interval = INTERVALS[args.workload]
     log(f"[bench] built {args.workload}: n={P.n} nnzA={P.A.nnz} nnzB={P.B.nnz} in {t_build:.1f}s")
     t0 = time.perf_counter()
-    mv = driver.upload(P, tune=sorted({1, args.block}))
+    mv = upload_with_layouts(P, args.layouts, sorted({1, args.block}))
     t_upload = time.perf_counter() - t0
     precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
 

@@ -214,10 +214,13 @@ int nmg_set_residual_target(nmg_handle h, double target);
 int nmg_set_block_size(nmg_handle h, int block);
 
 /* Kernel timing probe (no reference counterpart): with every > 0, every
- * `every`-th launch of the fused Chebyshev B-solve step (which = 0) and of the
- * fused filter step (which = 1) is bracketed by HIP events on the solver's
- * stream.  nmg_kernel_probe returns the mean sampled duration, the sample
- * count and the algorithmic bytes of one launch (matrix + vector streams). */
+ * `every`-th B-solve (which = 0: its deg back-to-back fused Chebyshev step
+ * launches, bracketed as a whole) and every `every`-th fused filter step
+ * launch (which = 1) are bracketed by HIP events on the solver's stream.
+ * nmg_kernel_probe returns the mean duration of one launch (bracket time /
+ * launches in it, so dispatch gaps between back-to-back launches are
+ * included), the sample count and the algorithmic bytes of one launch
+ * (matrix + vector streams). */
 int nmg_set_kernel_probe(nmg_handle h, int every);
 int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *samples,
                      double *bytes_per_launch);

@@ -73,15 +73,30 @@ def traffic_from_profiles(kernel_tag):
 
 
 def upload_with_layouts(P, layouts, tune):
-    """Upload A and B; autotune, or apply fixed single-column layouts."""
+    """Upload A and B; autotune, or apply fixed product layouts.
+
+    ``layouts``: 'auto', or comma-separated '<A|B><K>:<kind>:<params>' with
+    kind 'lanes:<g>' (K = 1, g lanes per node row), 'rows:<r>' (r node rows
+    per wavefront) or 'tiled:<cap>:<r>' (LDS tiles of at most cap nodes)."""
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
     mv = driver.upload(P, tune=())
     for part in layouts.split(","):
-        key, g = part.split(":")
-        pv.set_kernel_map(mv.matA if key.strip() == "A" else mv.matB, group=int(g))
-        mv.tuned[f"{key.strip()}1"] = {"kind": "lanes_per_row", "rows": int(g), "fixed": True}
+        f = part.strip().split(":")
+        mat = mv.matA if f[0][0] == "A" else mv.matB
+        K = int(f[0][1:] or 1)
+        if f[1] == "lanes":
+            pv.set_kernel_map(mat, group=int(f[2]))
+        elif f[1] == "rows":
+            pv.set_kernel_map(mat, 0, K, int(f[2]))
+            if K == 1:
+                pv.set_tiling(mat, 1, 0, 1)   # route single products through the row-block kernels
+        elif f[1] == "tiled":
+            pv.set_tiling(mat, K, int(f[2]), int(f[3]))
+        else:
+            raise ValueError(part)
+        mv.tuned[f[0]] = {"fixed": ":".join(f[1:])}
     return mv
 
 
@@ -90,16 +105,22 @@ def kernel_only(args):
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem
     P = build_problem(args.workload)
-    mv = upload_with_layouts(P, args.layouts, (1,))
+    mv = upload_with_layouts(P, args.layouts, sorted({1, args.block}))
     driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
     n_copy = 1 << 24   # 16M doubles: 128 MiB read + 128 MiB written per launch
     t_copy = pv.bench_copy(n_copy, 5)
     info = pv.chebiter_info(mv.chebB)
-    t, b = pv.bench_bsolve(mv.chebB, args.steps)
+    if args.block > 1:   # the k-column Chebyshev step, repeated
+        reps = args.steps * mv.bdeg
+        t, b = pv.bench_block_chebstep(mv.chebB, args.block, reps)
+        kern = {"kernel": f"k-column Chebyshev step, K={args.block}", "launches": reps + 1}
+    else:
+        t, b = pv.bench_bsolve(mv.chebB, args.steps)
+        kern = {"kernel": "B-solve (deg single-column steps)", "launches": args.steps + 1}
     print(json.dumps({"copy": {"doubles": n_copy, "bytes_per_launch": 16 * n_copy, "launches": 6,
                                "us": round(t_copy * 1e6, 2)},
-                      "bsolve": {"workload": args.workload, "n_dof": P.n, "deg": mv.bdeg, "launches": args.steps + 1,
-                                 "us": round(t * 1e6, 2), "algorithmic_bytes": b, **info}}), flush=True)
+                      "bsolve": {"workload": args.workload, "n_dof": P.n, "deg": mv.bdeg, "block": args.block,
+                                 "us": round(t * 1e6, 2), "algorithmic_bytes": b, **kern, **info}}), flush=True)
     mv.free()
 
 
@@ -178,7 +199,11 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--bdeg", default="auto", help="B-solve degree: auto | reference | <int>")
     ap.add_argument("--probe-every", type=int, default=64)
-    ap.add_argument("--block", type=int, default=1, help="Lanczos block width (1 = ChebLanNr)")
+    ap.add_argument("--block", type=int, default=8,
+                    help="Lanczos block width (1 = the reference's single-vector ChebLanNr; the block variant "
+                         "reorthogonalises with FP64 GEMMs and streams the matrices once per K columns)")
+    ap.add_argument("--single-compare", type=int, default=1,
+                    help="with --block > 1, also time one single-vector (block 1) solve (0 skips)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
@@ -227,7 +252,9 @@ def main():
     chk = r.probe["chebstep"]
     achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
     persistent = pv.chebiter_info(mv.chebB)["persistent"]
-    ktag = f"{'cheb_persistent_k' if persistent else 'mass3_chebstep'}:{args.workload}"
+    kname = ("cheb_persistent_k" if persistent else
+             ("mass3_chebstep" if args.block == 1 else f"block{args.block}_chebstep"))
+    ktag = f"{kname}:{args.workload}"
     traffic, traffic_src = traffic_from_profiles(ktag)
     line = {
         "metric": metric_name(),
@@ -262,7 +289,9 @@ def main():
             "bound": "hbm",
             "kernel": ("persistent Chebyshev B-solve (all steps in one cooperative launch; bytes = steps x "
                        "one fused step's algorithmic bytes)" if persistent else
-                       "fused Chebyshev B-solve step (scalar mass x I3 product + update)"),
+                       "fused Chebyshev B-solve step (scalar mass x I3 product + update)" if args.block == 1 else
+                       f"fused {args.block}-column Chebyshev B-solve step (scalar mass x I3 product + update on "
+                       f"{args.block} interleaved columns)"),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
@@ -271,6 +300,16 @@ def main():
             "filter_step": r.probe["filterstep"],
         },
     }
+    if args.single_compare and args.block > 1:
+        try:
+            el1, r1 = solve_once(mv, interval, bdeg, precision, 0, 1)
+            line["single_vector"] = {"seconds": round(el1, 4), "nev": int(r1.lam.size), "lanczos_steps": r1.lan_steps,
+                                     "resid_max": float(np.max(r1.res2)) if r1.res2.size else None,
+                                     "max_rel_eig_diff_vs_block": float(np.max(np.abs(r1.lam - r.lam) / r.lam))
+                                     if r1.lam.size == r.lam.size else None}
+            log(f"[bench] single vector: {line['single_vector']}")
+        except Exception as e:
+            line["single_vector"] = {"error": str(e)}
     if args.fp32_compare:
         try:
             el32, r32 = solve_once(mv, interval, bdeg, pv.FP32_FILTER, 0, args.block)
@@ -292,7 +331,10 @@ def main():
             line["spmv_hbm"] = {"error": str(e)}
     if args.cpu_seconds > 0:
         try:
-            line["cpu_baseline"] = cpu_baseline(P, mv, r, args.cpu_seconds)
+            # the CPU restatement runs the reference's single-vector ChebLanNr:
+            # scale it by a single-vector solve's filter work
+            r_ref = r if args.block == 1 else (r1 if "r1" in locals() else r)
+            line["cpu_baseline"] = cpu_baseline(P, mv, r_ref, args.cpu_seconds)
             log(f"[bench] cpu baseline: {line['cpu_baseline']}")
         except Exception as e:
             line["cpu_baseline"] = {"error": str(e)}

@@ -2326,8 +2326,9 @@ namespace nmg {
 // ---------------------------------------------------------------------------
 // Autotuner of the sparse-product mapping of one matrix for K columns: times
 // the candidate lane layouts on the matrix itself (untiled rows-per-wavefront
-// 1..8; LDS-tiled at three tile capacities x rows-per-wavefront 1..4; for
-// K = 1 also the lanes-per-row layout) and keeps the fastest.
+// 1..8; LDS-tiled at tile capacities on a geometric ladder x rows per
+// wavefront 1..8; for K = 1 also the lanes-per-row layout) and keeps the
+// fastest.
 template <class F>
 static double time_launches(hipStream_t s, int reps, F&& f) {
   f();  // warm-up
@@ -2381,12 +2382,17 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
     M.block_map[idx] = R;
     consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 1, 0, R, 0, 0);
   }
-  // tiled layouts (LDS budgets 64, 32, 16 KB) whose tiles hold the longest row
+  // tiled layouts: tile capacities on a geometric ladder from the longest
+  // row (a tile must hold one row) up to 64 KB of LDS per workgroup
   int64_t longest = 0;
   for (int64_t r = 0; r < M.nb; ++r) longest = std::max(longest, M.h_browptr[r + 1] - M.h_browptr[r]);
-  for (int kb : {64, 32, 16}) {
-    const int cap = std::min(65535, kb * 1024 / (24 * K));
-    if (cap < longest) continue;
+  const int limit = std::min(65535, 64 * 1024 / (24 * K));
+  std::vector<int> caps;
+  for (double f = 1.0; f * (double)longest <= (double)limit * 1.0001; f *= 1.28)
+    caps.push_back((int)std::ceil(f * (double)longest));
+  if (caps.empty() || caps.back() < limit) caps.push_back(limit);
+  for (int cap : caps) {
+    if (cap < longest || cap > limit) continue;
     for (int R : {1, 2, 4, 8}) {
       if (K * R > 64) continue;
       build_tiling(M, K, cap, R);

@@ -239,6 +239,10 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);
  * library's own stream) and the algorithmic bytes of one product. */
 int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
 /* Same for one fused Chebyshev B-solve step (one product + vector update). */
+/* The tall-skinny product of the block re-orthogonalisation on host
+ * arrays (for checks): C (m x kc) = A^T B, A: n x m, B: n x kc, all
+ * column-major. */
+int nmg_gemm_tn(int64_t n, int m, int kc, const double *A, const double *B, double *C);
 /* A plain copy kernel of n doubles, `reps` times (calibration of HBM byte
  * counters: it moves 16 n bytes per launch); mean seconds per launch. */
 int nmg_bench_copy(int64_t n, int reps, double *seconds);

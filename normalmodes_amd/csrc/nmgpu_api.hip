@@ -496,6 +496,7 @@ struct Solver {
 
   std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
+  DVec gw;                   // partial sums of the tall-skinny products
   bool borrowed = false;  // stream and rocBLAS handle belong to a parent solver
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
 
@@ -768,6 +769,19 @@ static void do_lanbounds(Solver& S, int mlan, int maxit, double tol, double* lmi
   S.seconds = now() - t0;
 }
 
+// C (m x kc) = A^T B for any kc, as tall-skinny products on column groups
+// of B of 16, 8, 4, 2 and 1
+static void gemm_tn_any(Solver& S, int64_t n, int m, int kc, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* C, int64_t ldc) {
+  for (int c0 = 0; c0 < kc;) {
+    int w = 16;
+    while (w > kc - c0) w >>= 1;
+    S.gw.ensure((int64_t)gemm_tn_work_size(n, m, w));
+    k_gemm_tn(n, m, w, A, lda, B + (int64_t)c0 * ldb, ldb, C + (int64_t)c0 * ldc, ldc, S.gw.p, S.s);
+    c0 += w;
+  }
+}
+
 // Ritz vectors of the candidates (theta >= bar) from the Lanczos basis V(:, 0:kdim)
 // and the eigenvectors dC (kdim x kdim, device) of the projected matrix, then
 // Rayleigh-Ritz of (A~, B~) on their span; keeps the pairs inside xintv[0..1].
@@ -806,12 +820,8 @@ static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vecto
     }
     GA.ensure((int64_t)nc * nc);
     GB.ensure((int64_t)nc * nc);
-    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, nc, nc,
-                            (rocblas_int)n, &one, Y.p, (rocblas_int)n, AY.p, (rocblas_int)n, &zero, GA.p,
-                            nc), "rocblas_dgemm (Y^T A Y) failed");
-    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, nc, nc,
-                            (rocblas_int)n, &one, Y.p, (rocblas_int)n, BY.p, (rocblas_int)n, &zero, GB.p,
-                            nc), "rocblas_dgemm (Y^T B Y) failed");
+    gemm_tn_any(S, n, nc, nc, Y.p, n, AY.p, n, GA.p, nc);  // Y^T A Y
+    gemm_tn_any(S, n, nc, nc, Y.p, n, BY.p, n, GB.p, nc);  // Y^T B Y
     // symmetrise (the products agree to rounding)
     {
       std::vector<double> ga((size_t)nc * nc), gb((size_t)nc * nc);
@@ -1166,8 +1176,8 @@ static void block_borth(Solver& S, int K, double* V, double* Z, std::vector<doub
   G.ensure((int64_t)K * K);
   Rinv.ensure((int64_t)K * K);
   T.ensure(n * K);
-  gemm(S, rocblas_operation_transpose, rocblas_operation_none, K, K, n, 1.0, V, n, Z, n, 0.0, G.p, K,
-       "block Gram V^T Z");
+  S.gw.ensure((int64_t)gemm_tn_work_size(n, K, K));
+  k_gemm_tn(n, K, K, V, n, Z, n, G.p, K, S.gw.p, S.s);  // block Gram V^T Z
   std::vector<double> g((size_t)K * K), Ri;
   NMG_HIP(hipMemcpyAsync(g.data(), G.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
   NMG_HIP(hipStreamSynchronize(S.s));
@@ -1273,8 +1283,8 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
       gemm(S, rocblas_operation_none, rocblas_operation_transpose, n, K, K, -1.0,
            S.Z + (size_t)(j - 1) * K * n, n, Rd.p, K, 1.0, W, n, "W -= Z_{j-1} R_j^T");
     }
-    gemm(S, rocblas_operation_transpose, rocblas_operation_none, K, K, n, 1.0, Vj, n, W, n, 0.0, Ad.p, K,
-         "A_j = V_j^T W");
+    S.gw.ensure((int64_t)gemm_tn_work_size(n, K, K));
+    k_gemm_tn(n, K, K, Vj, n, W, n, Ad.p, K, S.gw.p, S.s);  // A_j = V_j^T W
     std::vector<double> a((size_t)K * K);
     NMG_HIP(hipMemcpyAsync(a.data(), Ad.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
     gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, -1.0, Zj, n, Ad.p, K, 1.0, W, n,
@@ -1283,8 +1293,8 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
     const int64_t m = (int64_t)(j + 1) * K;
     Hd.ensure(m * K);
     for (int pass = 0; pass < 2; ++pass) {
-      gemm(S, rocblas_operation_transpose, rocblas_operation_none, m, K, n, 1.0, S.V, n, W, n, 0.0, Hd.p, m,
-           "H = V^T W");
+      S.gw.ensure((int64_t)gemm_tn_work_size(n, (int)m, K));
+      k_gemm_tn(n, (int)m, K, S.V, n, W, n, Hd.p, m, S.gw.p, S.s);  // H = V^T W
       gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, m, -1.0, S.Z, n, Hd.p, m, 1.0, W, n,
            "W -= Z H");
     }
@@ -2522,6 +2532,24 @@ int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double* seconds, double* 
     (void)hipEventDestroy(e1);
     if (seconds) *seconds = ms * 1e-3 / reps;
     if (bytes) *bytes = M->algorithmic_bytes(fp32 != 0);
+  });
+}
+
+int nmg_gemm_tn(int64_t n, int m, int kc, const double* A, const double* B, double* C) {
+  return guarded([&] {
+    NMG_CHECK(n > 0 && m > 0 && kc > 0 && A && B && C, NMG_EINVAL, "bad argument");
+    Solver S;
+    S.s = default_stream();
+    S.borrowed = true;
+    DVec dA, dB, dC;
+    dA.ensure(n * m);
+    dB.ensure(n * kc);
+    dC.ensure((int64_t)m * kc);
+    NMG_HIP(hipMemcpyAsync(dA.p, A, sizeof(double) * n * m, hipMemcpyHostToDevice, S.s));
+    NMG_HIP(hipMemcpyAsync(dB.p, B, sizeof(double) * n * kc, hipMemcpyHostToDevice, S.s));
+    gemm_tn_any(S, n, m, kc, dA.p, n, dB.p, n, dC.p, m);
+    NMG_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * m * kc, hipMemcpyDeviceToHost, S.s));
+    NMG_HIP(hipStreamSynchronize(S.s));
   });
 }
 

@@ -180,6 +180,11 @@ void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* 
                const double* refM, const double* refD, const double* X, int64_t ldx, const double* d,
                const double* lam, double* g_rho, double* g_lam, double* g_mu, hipStream_t s);
 size_t dot_work_size();
+// tall-skinny C (m x K, ldc) = A^T B, A: n x m (lda), B: n x K (ldb), K in
+// {1, 2, 4, 8, 16}; work: gemm_tn_work_size doubles
+size_t gemm_tn_work_size(int64_t n, int m, int K);
+void k_gemm_tn(int64_t n, int m, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+               int64_t ldc, double* work, hipStream_t s);
 size_t gemv_work_size(int64_t n, int k);
 
 // ---------------------------------------------------------------------------

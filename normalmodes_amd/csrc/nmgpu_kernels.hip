@@ -522,6 +522,103 @@ void k_gemv_t(int64_t n, int k, const double* V, int64_t ld, const double* z, do
   NMG_HIP(hipGetLastError());
 }
 
+// Tall-skinny C = A^T B (A: n x m, B: n x K, both column-major; C: m x K):
+// the reduction runs over the long dimension n, so it is split into chunks
+// (grid.x) and groups of CB columns of A (grid.y); each thread accumulates
+// CB x K products over its rows and the workgroup reduces them into one
+// partial per chunk; a second kernel sums the chunks.  Memory-bound
+// (2K/8 flop per byte of A), so plain FP64 FMAs stream A at full rate.
+
+template <int K, int CB>
+__global__ __launch_bounds__(kBlock) void gemm_tn_part_k(int64_t n, int m, const double* __restrict__ A,
+                                                         int64_t lda, const double* __restrict__ B,
+                                                         int64_t ldb, double* __restrict__ part, int nch) {
+  __shared__ double sh[kBlock / 64][CB * K];
+  const int j0 = blockIdx.y * CB;
+  const int64_t per = ((n + nch - 1) / nch + kBlock - 1) / kBlock * kBlock;
+  const int64_t start = (int64_t)blockIdx.x * per;
+  const int64_t end = start + per < n ? start + per : n;
+  const int ncol = (m - j0) < CB ? (m - j0) : CB;
+  double acc[CB][K];
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[c][k] = 0.0;
+  for (int64_t i = start + threadIdx.x; i < end; i += kBlock) {
+    double b[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] = B[(int64_t)k * ldb + i];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      if (c < ncol) {
+        const double a = A[(int64_t)(j0 + c) * lda + i];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[c][k] += a * b[k];
+      }
+    }
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double v = group_sum<64>(acc[c][k]);
+      if (l == 0) sh[w][c * K + k] = v;
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < CB * K; t += kBlock) {
+    const int c = t / K, k = t % K;
+    if (c < ncol) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlock / 64; ++q) v += sh[q][t];
+      part[((int64_t)k * m + j0 + c) * nch + blockIdx.x] = v;
+    }
+  }
+}
+
+__global__ void gemm_tn_final_k(int m, int K, const double* __restrict__ part, int nch, double* __restrict__ C,
+                                int64_t ldc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * K) return;
+  const int k = t / m, j = t % m;
+  const double* p = part + (int64_t)t * nch;
+  double v = 0.0;
+  for (int q = 0; q < nch; ++q) v += p[q];
+  C[(int64_t)k * ldc + j] = v;
+}
+
+static int gemm_tn_cb(int K) { return K >= 16 ? 4 : (K >= 8 ? 8 : (K >= 4 ? 16 : 32)); }
+
+static int gemm_tn_chunks(int64_t n, int m, int K) {
+  const int groups = (m + gemm_tn_cb(K) - 1) / gemm_tn_cb(K);
+  int64_t ch = 1024 / (groups > 0 ? groups : 1);
+  const int64_t maxch = (n + 4 * kBlock - 1) / (4 * kBlock);
+  if (ch > maxch) ch = maxch;
+  if (ch < 1) ch = 1;
+  return (int)ch;
+}
+
+size_t gemm_tn_work_size(int64_t n, int m, int K) { return (size_t)gemm_tn_chunks(n, m, K) * m * K + 16; }
+
+void k_gemm_tn(int64_t n, int m, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+               int64_t ldc, double* work, hipStream_t s) {
+  if (m <= 0) return;
+  const int nch = gemm_tn_chunks(n, m, K);
+  const int cb = gemm_tn_cb(K);
+  dim3 grid(nch, (m + cb - 1) / cb);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((gemm_tn_part_k<1, 32>), grid, dim3(kBlock), 0, s, n, m, A, lda, B, ldb, work, nch); break;
+    case 2: hipLaunchKernelGGL((gemm_tn_part_k<2, 32>), grid, dim3(kBlock), 0, s, n, m, A, lda, B, ldb, work, nch); break;
+    case 4: hipLaunchKernelGGL((gemm_tn_part_k<4, 16>), grid, dim3(kBlock), 0, s, n, m, A, lda, B, ldb, work, nch); break;
+    case 8: hipLaunchKernelGGL((gemm_tn_part_k<8, 8>), grid, dim3(kBlock), 0, s, n, m, A, lda, B, ldb, work, nch); break;
+    case 16: hipLaunchKernelGGL((gemm_tn_part_k<16, 4>), grid, dim3(kBlock), 0, s, n, m, A, lda, B, ldb, work, nch); break;
+    default: throw Error{NMG_EINVAL, "tall-skinny product: K must be 1, 2, 4, 8 or 16"};
+  }
+  hipLaunchKernelGGL(gemm_tn_final_k, dim3((m * K + 255) / 256), dim3(256), 0, s, m, K, work, nch, C, ldc);
+  NMG_HIP(hipGetLastError());
+}
+
 constexpr int kHT = 256;  // h tile staged in LDS
 
 __global__ __launch_bounds__(kBlock) void gemv_n_sub_k(int64_t n, int k, const double* __restrict__ Z,

@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_copy", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -387,6 +387,18 @@ def bench_bsolve(cheb: Handle, reps: int = 10, fp32: bool = False):
     _check(_lib().nmg_bench_bsolve(_i64(cheb), ctypes.c_int(reps), ctypes.c_int(int(fp32)), ctypes.byref(s),
                                    ctypes.byref(b)))
     return s.value, b.value
+
+
+def gemm_tn(A, B):
+    """A^T B on the device by the tall-skinny kernels (A: n x m, B: n x k)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    n, m = A.shape
+    k = B.shape[1]
+    C = np.empty((m, k), order="F")
+    _check(_lib().nmg_gemm_tn(_i64(n), ctypes.c_int(m), ctypes.c_int(k), A.ctypes.data_as(_f64p),
+                              B.ctypes.data_as(_f64p), C.ctypes.data_as(_f64p)))
+    return C
 
 
 def bench_copy(n: int, reps: int = 10) -> float:

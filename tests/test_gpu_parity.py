@@ -398,3 +398,17 @@ def test_empty_interval_stops_early(small_p2, dense_eigs):
     finally:
         mv.free()
     assert r.lam.size == 0 and r.lan_steps < 200
+
+
+def test_tall_skinny_products():
+    """The split-reduction A^T B of the block re-orthogonalisation and the
+    Rayleigh-Ritz Gram matrices against numpy, for every column width and
+    odd sizes."""
+    pv = _pv()
+    rng = np.random.default_rng(12)
+    for n, m, k in ((1, 1, 1), (1000, 3, 5), (51540, 424, 8), (20001, 37, 16), (777, 129, 37)):
+        A = rng.standard_normal((n, m))
+        B = rng.standard_normal((n, k))
+        C = pv.gemm_tn(A, B)
+        ref = A.T @ B
+        assert np.max(np.abs(C - ref)) <= 1e-12 * np.sqrt(n) * np.abs(ref).max() + 1e-14, (n, m, k)

@@ -264,21 +264,23 @@ int nmg_set_stream_loads(nmg_handle mat, int on);
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */
 int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave);
 /* LDS-tiled K-column products: rows cut into tiles whose referenced nodes
- * (at most max_nodes, <= 64 KB of x in LDS) are staged once per tile;
- * max_nodes = 0 removes the tiling. */
-int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave);
+ * (at most max_nodes, <= 64 KB of x in LDS) are staged once per tile by a
+ * workgroup of `threads` (256 or 1024; 0 = 1024); max_nodes = 0 removes the
+ * tiling. */
+int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int threads);
 /* The tiling of a node pattern computed on the host only (no device use):
  * call once with null arrays for ntiles and the list length (tcp[ntiles]),
  * then with trp[ntiles+1], tcp[ntiles+1], tcols[tcp[ntiles]], lcol[nnz]. */
 int nmg_tiling_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int max_nodes, int64_t *ntiles,
                     int *max_used, int64_t *trp, int64_t *tcp, int32_t *tcols, uint16_t *lcol);
 /* Autotuner (no reference counterpart): times the product layouts of mat for
- * K columns on the device (lanes per row; rows per wavefront; LDS tiles at
- * 64/32/16 KB) and keeps the fastest.  kind: 0 lanes-per-row, 1 rows per
+ * K columns on the device (lanes per row; rows per wavefront; LDS tiles with
+ * capacities on a geometric ladder up to 64 KB, workgroups of 1024 or 256
+ * threads) and keeps the fastest.  kind: 0 lanes-per-row, 1 rows per
  * wavefront, 2 tiled; rows: lanes per row (kind 0) or rows per wavefront;
- * cap: tile node capacity (kind 2). */
+ * cap: tile node capacity and threads: tile workgroup size (kind 2). */
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double *seconds, int *kind, int *rows,
-                     int *cap);
+                     int *cap, int *threads);
 int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 
 #ifdef __cplusplus

@@ -294,7 +294,7 @@ static void compute_tiling(int64_t nb, const int64_t* rp, const int32_t* ci, int
   if (row > start) close_tile();
 }
 
-void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave) {
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block) {
   int idx = -1;
   for (int i = 0; i < 5; ++i)
     if ((1 << i) == K) idx = i;
@@ -310,6 +310,8 @@ void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave) {
   T->max_nodes = H.max_nodes;
   T->cap = max_nodes;
   T->rows_per_wave = rows_per_wave > 0 ? rows_per_wave : 1;
+  NMG_CHECK(block == 256 || block == 1024, NMG_EINVAL, "tile workgroups have 256 or 1024 threads");
+  T->block = block;
   T->staged_nodes = (double)H.tcols.size();
   T->trp = dmalloc<int64_t>(H.trp.size());
   T->tcp = dmalloc<int64_t>(H.tcp.size());
@@ -354,7 +356,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->xcd = S.xcd;
   M->nt = S.nt;
   for (int i = 0; i < 5; ++i)
-    if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave);
+    if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave, S.tiling[i]->block);
   return M.release();
 }
 
@@ -2361,6 +2363,7 @@ struct TuneResult {
   int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block
   int group = 0, rows = 1, cap = 0;
   int64_t ntiles = 0;
+  int block = 1024;  // tile workgroup threads
 };
 
 static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t s) {
@@ -2373,8 +2376,8 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   y.ensure(nk);
   k_fill(nk, 1.0, x.p, s);
   TuneResult best;
-  auto consider = [&](double t, int kind, int group, int rows, int cap, int64_t nt) {
-    if (t < best.seconds) best = TuneResult{t, kind, group, rows, cap, nt};
+  auto consider = [&](double t, int kind, int group, int rows, int cap, int64_t nt, int block = 1024) {
+    if (t < best.seconds) best = TuneResult{t, kind, group, rows, cap, nt, block};
   };
   // untiled layouts (the single-column route is K = 1's choice alone: tuning
   // another width leaves it as it is)
@@ -2405,9 +2408,11 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
     if (cap < longest || cap > limit) continue;
     for (int R : {1, 2, 4, 8}) {
       if (K * R > 64) continue;
-      build_tiling(M, K, cap, R);
-      consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 2, 0, R, cap,
-               M.tiling[idx] ? M.tiling[idx]->ntiles : 0);
+      for (int tb : {1024, 256}) {
+        build_tiling(M, K, cap, R, tb);
+        consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 2, 0, R, cap,
+                 M.tiling[idx] ? M.tiling[idx]->ntiles : 0, tb);
+      }
     }
   }
   // apply the winner
@@ -2419,7 +2424,7 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
     M.block_map[idx] = best.rows;
     if (K == 1) M.single_blocked = true;
   } else {
-    build_tiling(M, K, best.cap, best.rows);
+    build_tiling(M, K, best.cap, best.rows, best.block);
     if (K == 1) M.single_blocked = true;
   }
   NMG_HIP(hipStreamSynchronize(s));
@@ -2430,7 +2435,7 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
 extern "C" {
 
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds, int* kind, int* rows,
-                     int* cap) {
+                     int* cap, int* threads) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
@@ -2442,6 +2447,7 @@ int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds,
     if (kind) *kind = r.kind;
     if (rows) *rows = r.kind == 0 ? r.group : r.rows;
     if (cap) *cap = r.cap;
+    if (threads) *threads = r.kind == 2 ? r.block : 0;
   });
 }
 
@@ -2497,13 +2503,13 @@ int nmg_tiling_host(int64_t nb, const int64_t* rowptr, const int32_t* col, int m
   });
 }
 
-int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave) {
+int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int threads) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     NMG_CHECK(rows_per_wave >= 1 && rows_per_wave <= 8 && K * rows_per_wave <= 64, NMG_EINVAL,
               "bad rows per wavefront");
     NMG_CHECK(max_nodes * 24 * K <= 64 * 1024, NMG_EINVAL, "tile exceeds 64 KB of LDS");
-    build_tiling(*M, K, max_nodes, rows_per_wave);
+    build_tiling(*M, K, max_nodes, rows_per_wave, threads > 0 ? threads : 1024);
     if (K == 1) M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
   });
 }

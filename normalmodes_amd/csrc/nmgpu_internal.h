@@ -44,6 +44,7 @@ struct Tiling {
   int max_nodes = 0;      // largest node list (sets the LDS size)
   int cap = 0;            // node capacity the tiles were cut for
   int rows_per_wave = 1;  // R of the lane layout
+  int block = 1024;       // threads of the workgroup sharing one tile (1024 or 256)
   int64_t* trp = nullptr;
   int64_t* tcp = nullptr;
   int32_t* tcols = nullptr;
@@ -111,7 +112,7 @@ void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const doubl
 inline int kflags(const DevMat& M) { return (M.xcd ? 1 : 0) | (M.nt ? 2 : 0); }
 
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
-void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave);
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024);
 // creates a DevMat from host CSR (0-based); picks the layout
 DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rowptr, const int32_t* col,
                     const double* val, bool allow_blocked = true);

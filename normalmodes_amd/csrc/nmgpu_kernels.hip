@@ -1104,10 +1104,9 @@ void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t 
 // tile-local column indices.  Epilogues work on global memory as before.
 namespace nmg {
 
-constexpr int kTileBlock = 1024;  // 16 wavefronts share one tile's LDS
-
-template <int K, int R, class VT, class Epi, bool BSR>
-__global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __restrict__ trp,
+// TB threads (TB/64 wavefronts) share one tile's LDS: 1024 or 256
+template <int K, int R, class VT, class Epi, bool BSR, int TB>
+__global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict__ trp,
                                                            const int64_t* __restrict__ tcp,
                                                            const int32_t* __restrict__ tcols,
                                                            const int64_t* __restrict__ rp,
@@ -1123,7 +1122,7 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
   const int64_t c0 = tcp[tile];
   const int nu = (int)(tcp[tile + 1] - c0);
   // stage the tile's x blocks
-  for (int t = threadIdx.x; t < nu * W3; t += kTileBlock) {
+  for (int t = threadIdx.x; t < nu * W3; t += TB) {
     const int node = t / W3, j = t - node * W3;
     xs[t] = x[(int64_t)tcols[c0 + node] * W3 + j];
   }
@@ -1132,7 +1131,7 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
   const int wave = threadIdx.x >> 6;
   const int c = lane % K, e = (lane / K) % E;
   const int rsub = lane / (K * E);
-  for (int64_t rb = r0 + (int64_t)wave * R; rb < r1; rb += (int64_t)(kTileBlock / 64) * R) {
+  for (int64_t rb = r0 + (int64_t)wave * R; rb < r1; rb += (int64_t)(TB / 64) * R) {
     const int64_t row = rb + rsub;
     const bool live = row < r1;
     typename Epi::Pre pre[(3 + E - 1) / E];
@@ -1177,16 +1176,23 @@ __global__ __launch_bounds__(kTileBlock) void tiled_block_rows(const int64_t* __
   }
 }
 
+template <int K, int R, class VT, class Epi, int TB>
+static void launch_tiled_RB(const DevMat& M, const Tiling& T, const VT* v, const double* x, Epi epi,
+                            hipStream_t s) {
+  const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
+  if (M.layout == L_BSR3)
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
+                       T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
+  else
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
+                       T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
+}
+
 template <int K, int R, class VT, class Epi>
 static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const double* x, Epi epi,
                            hipStream_t s) {
-  const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
-  if (M.layout == L_BSR3)
-    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true>), dim3((unsigned)T.ntiles), dim3(kTileBlock), shmem,
-                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
-  else
-    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false>), dim3((unsigned)T.ntiles), dim3(kTileBlock),
-                       shmem, s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
+  if (T.block == 256) launch_tiled_RB<K, R, VT, Epi, 256>(M, T, v, x, epi, s);
+  else launch_tiled_RB<K, R, VT, Epi, 1024>(M, T, v, x, epi, s);
 }
 
 template <int K, class VT, class Epi>

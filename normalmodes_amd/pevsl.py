@@ -434,17 +434,19 @@ def set_stream_loads(mat: Handle, on: bool = True):
     _check(_lib().nmg_set_stream_loads(_i64(mat), ctypes.c_int(int(on))))
 
 
-def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1):
-    _check(_lib().nmg_set_tiling(_i64(mat), ctypes.c_int(K), ctypes.c_int(max_nodes), ctypes.c_int(rows_per_wave)))
+def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1, threads: int = 1024):
+    _check(_lib().nmg_set_tiling(_i64(mat), ctypes.c_int(K), ctypes.c_int(max_nodes), ctypes.c_int(rows_per_wave),
+                                 ctypes.c_int(threads)))
 
 
 def csr_autotune(mat: Handle, K: int, reps: int = 10, fp32: bool = False):
     s = _d()
-    kind, rows, cap = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    kind, rows, cap, thr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(_lib().nmg_csr_autotune(_i64(mat), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
-                                   ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap)))
+                                   ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap),
+                                   ctypes.byref(thr)))
     return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled"][kind.value],
-                rows=rows.value, cap=cap.value)
+                rows=rows.value, cap=cap.value, threads=thr.value)
 
 
 def bench_chebstep(cheb: Handle, reps: int = 20, fp32: bool = False):

@@ -412,3 +412,44 @@ def test_tall_skinny_products():
         C = pv.gemm_tn(A, B)
         ref = A.T @ B
         assert np.max(np.abs(C - ref)) <= 1e-12 * np.sqrt(n) * np.abs(ref).max() + 1e-14, (n, m, k)
+
+
+def test_tiled_products_match_untiled(small_p2):
+    """LDS-tiled k-column products (scalar mass and 3x3 blocks; store and
+    Chebyshev epilogues) for every width, rows per wavefront, tile capacity
+    and workgroup size, against the untiled kernels."""
+    from normalmodes_amd.driver import setupmatvec, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg=20, tune=())
+    M = small_p2.B.to_scipy()[0::3, 0::3].tocsr()
+    longest = int(np.max(np.diff(M.indptr)))
+    rng = np.random.default_rng(21)
+
+    def products(X):
+        # the solver clones A (scaled) with the layout A has when it is set
+        h = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
+        try:
+            return [pv.apply_block(h, which, X) for which in (0, 1, 2)]
+        finally:
+            pv.pevsl_finish(h)
+
+    try:
+        for K in (1, 2, 4, 8, 16):
+            X = rng.standard_normal((K, small_p2.n))
+            for mat in (mv.matA, mv.matB):
+                pv.set_tiling(mat, K, 0, 1)
+            ref = products(X)
+            limit = 64 * 1024 // (24 * K)
+            for R in (1, 2, 4):
+                if K * R > 64:
+                    continue
+                for cap in sorted({longest, min(2 * longest, limit), limit}):
+                    for threads in (256, 1024):
+                        for mat in (mv.matA, mv.matB):
+                            pv.set_tiling(mat, K, cap, R, threads)
+                        for which, (Y, Yr) in enumerate(zip(products(X), ref)):
+                            assert _rel(Y, Yr) < 1e-14, (K, R, cap, threads, which)
+            for mat in (mv.matA, mv.matB):
+                pv.set_tiling(mat, K, 0, 1)
+    finally:
+        mv.free()

@@ -1117,79 +1117,49 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
   constexpr int W3 = 3 * K;
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BSR ? (BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U) : BlockMap<K, R>::U;
-  constexpr int NV = BSR ? 9 : 1;          // values per entry
-  constexpr int NP = (3 + E - 1) / E;      // epilogue slots per lane
   const int64_t tile = xcd_block(flags);
   const int64_t r0 = trp[tile], r1 = trp[tile + 1];
   const int64_t c0 = tcp[tile];
   const int nu = (int)(tcp[tile + 1] - c0);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int c = lane % K, e = (lane / K) % E;
-  const int rsub = lane / (K * E);
-  // The first row group's row offsets, first entries and epilogue inputs do
-  // not depend on x: they are loaded before the staging, so their latency
-  // overlaps it.
-  const int64_t rowf = r0 + (int64_t)wave * R + rsub;
-  const bool livef = rowf < r1;
-  int64_t begf = 0, endf = 0;
-  if (livef) {
-    begf = rp[rowf];
-    endf = rp[rowf + 1];
-  }
-  int pl[U];
-  double pm[U][NV];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int64_t k = begf + e + (int64_t)u * E;
-    const bool ok = livef && k < endf;
-    pl[u] = ok ? (int)ldm(lcol + k, flags) : 0;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) pm[u][q] = ok ? (double)ldm(val + q * nbnz + k, flags) : 0.0;
-  }
-  typename Epi::Pre pref[NP];
-#pragma unroll
-  for (int q = 0; q < NP; ++q)
-    if (livef && e + q * E < 3) pref[q] = epi.load(3 * rowf + e + q * E, c);
   // stage the tile's x blocks
   for (int t = threadIdx.x; t < nu * W3; t += TB) {
     const int node = t / W3, j = t - node * W3;
     xs[t] = x[(int64_t)tcols[c0 + node] * W3 + j];
   }
   __syncthreads();
-  bool first = true;
-  for (int64_t rb = r0 + (int64_t)wave * R; rb < r1; rb += (int64_t)(TB / 64) * R, first = false) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int c = lane % K, e = (lane / K) % E;
+  const int rsub = lane / (K * E);
+  for (int64_t rb = r0 + (int64_t)wave * R; rb < r1; rb += (int64_t)(TB / 64) * R) {
     const int64_t row = rb + rsub;
     const bool live = row < r1;
-    typename Epi::Pre pre[NP];
+    typename Epi::Pre pre[(3 + E - 1) / E];
 #pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      if (first) pre[q] = pref[q];
-      else if (live && e + q * E < 3) pre[q] = epi.load(3 * row + e + q * E, c);
-    }
+    for (int q = 0; q < (3 + E - 1) / E; ++q)
+      if (live && e + q * E < 3) pre[q] = epi.load(3 * row + e + q * E, c);
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
     if (live) {
-      const int64_t beg = first ? begf : rp[row], end = first ? endf : rp[row + 1];
+      const int64_t beg = rp[row], end = rp[row + 1];
       for (int64_t k0 = beg + e; k0 < end; k0 += (int64_t)U * E) {
-        const bool pf = first && k0 == beg + e;  // the prefetched chunk
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int64_t k = k0 + (int64_t)u * E;
           if (k < end) {
-            const int lc = pf ? pl[u] : (int)ldm(lcol + k, flags);
-            const double* xn = xs + lc * W3 + c;
+            const double* xn = xs + (int)ldm(lcol + k, flags) * W3 + c;
             const double xa = xn[0], xb = xn[K], xc = xn[2 * K];
-            double v[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) v[q] = pf ? pm[u][q] : (double)ldm(val + q * nbnz + k, flags);
             if (BSR) {
+              double v[9];
+#pragma unroll
+              for (int q = 0; q < 9; ++q) v[q] = (double)ldm(val + q * nbnz + k, flags);
               y0 += v[0] * xa + v[1] * xb + v[2] * xc;
-              y1 += v[3 % NV] * xa + v[4 % NV] * xb + v[5 % NV] * xc;
-              y2 += v[6 % NV] * xa + v[7 % NV] * xb + v[8 % NV] * xc;
+              y1 += v[3] * xa + v[4] * xb + v[5] * xc;
+              y2 += v[6] * xa + v[7] * xb + v[8] * xc;
             } else {
-              y0 += v[0] * xa;
-              y1 += v[0] * xb;
-              y2 += v[0] * xc;
+              const double m = (double)ldm(val + k, flags);
+              y0 += m * xa;
+              y1 += m * xb;
+              y2 += m * xc;
             }
           }
         }
@@ -1199,7 +1169,7 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
     y1 = slots_sum<K, E>(y1);
     y2 = slots_sum<K, E>(y2);
 #pragma unroll
-    for (int q = 0; q < NP; ++q) {
+    for (int q = 0; q < (3 + E - 1) / E; ++q) {
       const int p = e + q * E;
       if (live && p < 3) epi.store(3 * row + p, c, p == 0 ? y0 : (p == 1 ? y1 : y2), pre[q]);
     }

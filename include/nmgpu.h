@@ -259,6 +259,11 @@ int nmg_set_xcd_order(nmg_handle mat, int on);
  * loads (1) or ordinary loads (0, default): a product streams them once, so
  * streaming loads leave the caches to the gathered x. */
 int nmg_set_stream_loads(nmg_handle mat, int on);
+/* Value order of a 3x3-block matrix (no reference counterpart): 0 planes
+ * (the nine values of each block in nine arrays), 1 tiles of 64 blocks
+ * (nine runs of 64 values per tile), so that one run of blocks reads
+ * contiguous memory.  Products, scaling and diagonals follow the order. */
+int nmg_set_value_order(nmg_handle mat, int tiled);
 /* Kernel mapping of a matrix (tuning): lanes per row of the single-column
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */

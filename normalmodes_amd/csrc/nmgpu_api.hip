@@ -347,14 +347,14 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
     cp(S.col, M->col, S.nnz);
     cp(S.val, M->val, S.nnz);
   } else {
-    const size_t planes = S.layout == L_BSR3 ? 9 : 1;
     cp(S.browptr, M->browptr, S.nb + 1);
     cp(S.bcol, M->bcol, S.nbnz);
-    cp(S.bval, M->bval, planes * S.nbnz);
+    cp(S.bval, M->bval, S.value_count());
   }
   M->single_blocked = S.single_blocked;
   M->xcd = S.xcd;
   M->nt = S.nt;
+  M->tiled_vals = S.tiled_vals;
   for (int i = 0; i < 5; ++i)
     if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave, S.tiling[i]->block);
   return M.release();
@@ -2302,6 +2302,13 @@ int nmg_set_xcd_order(nmg_handle mat, int on) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     M->xcd = on ? 1 : 0;
+  });
+}
+
+int nmg_set_value_order(nmg_handle mat, int tiled) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    set_value_order(*M, tiled, default_stream());
   });
 }
 

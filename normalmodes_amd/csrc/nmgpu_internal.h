@@ -77,6 +77,15 @@ struct DevMat {
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
   int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)
   int nt = 0;                   // non-temporal matrix loads in the row kernels
+  // BSR3 value order: 0 planes (value q of entry k at q*nbnz + k); 1 tiles of
+  // 64 entries (at (k/64)*576 + q*64 + k%64), so the nine values of a run of
+  // entries share DRAM pages while each plane load stays contiguous
+  int tiled_vals = 0;
+  int64_t value_count() const {
+    if (layout == L_CSR) return nnz;
+    if (layout == L_MASS3) return nbnz;
+    return tiled_vals ? 9 * ((nbnz + 63) / 64 * 64) : 9 * nbnz;
+  }
   std::vector<int64_t> h_browptr;  // host copy of the node pattern (tiling)
   std::vector<int32_t> h_bcol;
   ~DevMat();
@@ -108,8 +117,12 @@ void plan_cheb_persistent(const DevMat& M, bool fp32, PersistPlan& P);
 void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const double* b, double* out, double* tmp,
                        int deg, double lmin, double lmax, hipStream_t s);
 
-// kernel flags of M's sparse products (bit 0 XCD order, bit 1 streaming loads)
-inline int kflags(const DevMat& M) { return (M.xcd ? 1 : 0) | (M.nt ? 2 : 0); }
+// kernel flags of M's sparse products (bit 0 XCD order, bit 1 streaming
+// loads, bit 2 tiled BSR3 value order)
+inline int kflags(const DevMat& M) { return (M.xcd ? 1 : 0) | (M.nt ? 2 : 0) | (M.tiled_vals ? 4 : 0); }
+// converts the BSR3 values of M (and their FP32 copy) to the tiled order (1)
+// or back to planes (0)
+void set_value_order(DevMat& M, int tiled, hipStream_t s);
 
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024);

@@ -35,6 +35,10 @@ __device__ __forceinline__ int64_t xcd_block(int flags) {
   const int64_t x = b % kXcd, i = b / kXcd;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
+// position of value q of BSR3 entry k (planes, or tiles of 64 entries)
+__device__ __forceinline__ int64_t bidx(int64_t k, int q, int64_t nbnz, int flags) {
+  return (flags & 4) ? ((k >> 6) * 576 + (int64_t)q * 64 + (k & 63)) : ((int64_t)q * nbnz + k);
+}
 // matrix loads: streamed once per product, so with bit 1 they are issued as
 // non-temporal loads that do not displace the gathered x from the caches
 template <class T>
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
       const int64_t c3 = 3 * (int64_t)ldm(ci + k, flags);
       double v[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) v[q] = (double)ldm(bv + q * nbnz + k, flags);
+      for (int q = 0; q < 9; ++q) v[q] = (double)ldm(bv + bidx(k, q, nbnz, flags), flags);
       const double x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
       y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
       y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
@@ -656,14 +660,15 @@ __global__ void csr_scale_k(int64_t nrow, const int64_t* rp, const int32_t* ci, 
   for (int64_t k = rp[r]; k < rp[r + 1]; ++k) v[k] = a * v[k] * dr[ci[k]];
 }
 __global__ void bsr3_scale_k(int64_t nb, int64_t nbnz, const int64_t* rp, const int32_t* ci, double* bv,
-                             const double* dl, const double* dr, double sign) {
+                             const double* dl, const double* dr, double sign, int flags) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= nb) return;
   for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
     const int64_t c = ci[k];
     for (int p = 0; p < 9; ++p) {
       const int rr = p / 3, cc = p % 3;
-      bv[p * nbnz + k] = sign * dl[3 * r + rr] * bv[p * nbnz + k] * dr[3 * c + cc];
+      const int64_t i = bidx(k, p, nbnz, flags);
+      bv[i] = sign * dl[3 * r + rr] * bv[i] * dr[3 * c + cc];
     }
   }
 }
@@ -680,7 +685,7 @@ void scale_devmat(DevMat& M, const double* dl, const double* dr, double sign, hi
                        M.nrow, M.rowptr, M.col, M.val, dl, dr, sign);
   } else if (M.layout == L_BSR3) {
     hipLaunchKernelGGL(bsr3_scale_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, dl, dr, sign);
+                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, dl, dr, sign, kflags(M));
   } else {
     hipLaunchKernelGGL(mass3_scale_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        M.nb, M.browptr, M.bcol, M.bval, dl, dr, sign);
@@ -698,15 +703,15 @@ __global__ void csr_diag_k(int64_t nrow, const int64_t* rp, const int32_t* ci, c
   d[r] = x;
 }
 __global__ void bsr3_diag_k(int64_t nb, int64_t nbnz, const int64_t* rp, const int32_t* ci,
-                            const double* bv, double* d) {
+                            const double* bv, double* d, int flags) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= nb) return;
   double x0 = 0, x1 = 0, x2 = 0;
   for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
     if (ci[k] == r) {
-      x0 = bv[0 * nbnz + k];
-      x1 = bv[4 * nbnz + k];
-      x2 = bv[8 * nbnz + k];
+      x0 = bv[bidx(k, 0, nbnz, flags)];
+      x1 = bv[bidx(k, 4, nbnz, flags)];
+      x2 = bv[bidx(k, 8, nbnz, flags)];
     }
   d[3 * r] = x0;
   d[3 * r + 1] = x1;
@@ -727,7 +732,7 @@ void diag_devmat(const DevMat& M, double* d, hipStream_t s) {
                        M.nrow, M.rowptr, M.col, M.val, d);
   else if (M.layout == L_BSR3)
     hipLaunchKernelGGL(bsr3_diag_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, d);
+                       M.nb, M.nbnz, M.browptr, M.bcol, M.bval, d, kflags(M));
   else
     hipLaunchKernelGGL(mass3_diag_k, dim3((unsigned)((M.nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        M.nb, M.browptr, M.bcol, M.bval, d);
@@ -744,7 +749,7 @@ void make_fp32(DevMat& M, hipStream_t s) {
     if (!M.val32) NMG_HIP(hipMalloc(&M.val32, sizeof(float) * (M.nnz > 0 ? M.nnz : 1)));
     hipLaunchKernelGGL(to_f32_k, dim3(grid_for(M.nnz)), dim3(kBlock), 0, s, M.nnz, M.val, M.val32);
   } else {
-    const int64_t nv = (M.layout == L_BSR3 ? 9 : 1) * M.nbnz;
+    const int64_t nv = M.value_count();
     if (!M.bval32) NMG_HIP(hipMalloc(&M.bval32, sizeof(float) * (nv > 0 ? nv : 1)));
     hipLaunchKernelGGL(to_f32_k, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, M.bval, M.bval32);
   }
@@ -923,7 +928,7 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)ldm(bv + p * nbnz + kk[u], flags) : 0.0;
+        for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)ldm(bv + bidx(kk[u], p, nbnz, flags), flags) : 0.0;
         xa[u] = x[base[u]];
         xb[u] = x[base[u] + K];
         xc[u] = x[base[u] + 2 * K];
@@ -1151,7 +1156,7 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
             if (BSR) {
               double v[9];
 #pragma unroll
-              for (int q = 0; q < 9; ++q) v[q] = (double)ldm(val + q * nbnz + k, flags);
+              for (int q = 0; q < 9; ++q) v[q] = (double)ldm(val + bidx(k, q, nbnz, flags), flags);
               y0 += v[0] * xa + v[1] * xb + v[2] * xc;
               y1 += v[3] * xa + v[4] * xb + v[5] * xc;
               y2 += v[6] * xa + v[7] * xb + v[8] * xc;
@@ -1503,6 +1508,38 @@ void k_cheb_persistent(int K, const DevMat& M, const PersistPlan& P, const doubl
     const double* va = M.bval;
     args[2] = (void*)&va;
     NMG_HIP(hipLaunchCooperativeKernel(persistent_fn<double>(K), dim3(P.grid), dim3(kPBlock), args, P.lds_bytes, s));
+  }
+}
+
+// BSR3 values between the plane order and the 64-entry tile order
+__global__ void bsr3_reorder_k(int64_t nbnz, int64_t ntot, const double* __restrict__ in, double* __restrict__ out,
+                               int to_tiles) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < ntot; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = (t / 576) * 64 + (t % 64);
+    const int q = (int)((t % 576) / 64);
+    if (to_tiles) out[t] = k < nbnz ? in[(int64_t)q * nbnz + k] : 0.0;
+    else if (k < nbnz) out[(int64_t)q * nbnz + k] = in[t];
+  }
+}
+
+void set_value_order(DevMat& M, int tiled, hipStream_t s) {
+  NMG_CHECK(M.layout == L_BSR3, NMG_EINVAL, "the value order applies to 3x3-block matrices");
+  tiled = tiled ? 1 : 0;
+  if (tiled == M.tiled_vals) return;
+  const int64_t ntiles = 9 * ((M.nbnz + 63) / 64 * 64);
+  const int64_t nplanes = 9 * M.nbnz;
+  double* out = nullptr;
+  NMG_HIP(hipMalloc(&out, sizeof(double) * (tiled ? ntiles : nplanes)));
+  hipLaunchKernelGGL(bsr3_reorder_k, dim3(grid_for(ntiles)), dim3(kBlock), 0, s, M.nbnz, ntiles, M.bval, out, tiled);
+  NMG_HIP(hipGetLastError());
+  NMG_HIP(hipStreamSynchronize(s));
+  NMG_HIP(hipFree(M.bval));
+  M.bval = out;
+  M.tiled_vals = tiled;
+  if (M.bval32) {  // rebuild the FP32 copy in the new order and size
+    NMG_HIP(hipFree(M.bval32));
+    M.bval32 = nullptr;
+    make_fp32(M, s);
   }
 }
 

@@ -21,7 +21,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_chebiter_stats", "pevsl_lanbounds", "pevsl_findpol", "pol_info",
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
-           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "csr_autotune", "kpmdos", "dos_count", "spslicer",
+           "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
            "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "LAYOUTS"]
 
@@ -432,6 +432,10 @@ def set_xcd_order(mat: Handle, on: bool = True):
 
 def set_stream_loads(mat: Handle, on: bool = True):
     _check(_lib().nmg_set_stream_loads(_i64(mat), ctypes.c_int(int(on))))
+
+
+def set_value_order(mat: Handle, tiled: bool = True):
+    _check(_lib().nmg_set_value_order(_i64(mat), ctypes.c_int(int(tiled))))
 
 
 def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1, threads: int = 1024):

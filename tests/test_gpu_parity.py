@@ -453,3 +453,45 @@ def test_tiled_products_match_untiled(small_p2):
                 pv.set_tiling(mat, K, 0, 1)
     finally:
         mv.free()
+
+
+def test_bsr3_value_order():
+    """3x3-block values in 64-entry tiles give the same products (single and
+    k-column, untiled and LDS-tiled) and the same diagonal scaling as the
+    plane order, bit for bit."""
+    from normalmodes_amd.workloads import build_problem
+    pv = _pv()
+    P = build_problem("tiny")
+    A = P.A
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal(A.n_rows)
+    X = rng.standard_normal((8, A.n_rows))
+    ha = pv.csr_create(A.rowptr, A.col, A.val)
+    hb = pv.csr_create(A.rowptr, A.col, A.val)
+    try:
+        assert pv.csr_info(ha)["layout"] == "bsr3"
+        pv.set_value_order(hb, True)
+        assert np.array_equal(pv.parcsr_matvec(x, ha), pv.parcsr_matvec(x, hb))
+        da = pv.csr_diagscale(ha, 1.0)
+        db = pv.csr_diagscale(hb, 1.0)
+        assert np.array_equal(da, db)
+        assert np.array_equal(pv.parcsr_matvec(x, ha), pv.parcsr_matvec(x, hb))
+        for cap in (0, 120):
+            for K in (1, 8):
+                for h in (ha, hb):
+                    pv.set_tiling(h, K, cap, 1)
+                outs = []
+                for h in (ha, hb):
+                    s = pv.pevsl_start(0)
+                    try:
+                        pv.pevsl_setprobsizes(s, A.n_rows, A.n_rows, -1)
+                        pv.pevsl_setamv(s, h)
+                        outs.append(pv.apply_block(s, 0, X[:K]))
+                    finally:
+                        pv.pevsl_finish(s)
+                assert np.array_equal(outs[0], outs[1]), (K, cap)
+        pv.set_value_order(hb, False)
+        assert np.array_equal(pv.parcsr_matvec(x, ha), pv.parcsr_matvec(x, hb))
+    finally:
+        pv.csr_free(ha)
+        pv.csr_free(hb)

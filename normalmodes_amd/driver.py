@@ -176,6 +176,8 @@ def upload(problem, *, porder: int | None = None, tune=(1,)) -> MatvecSetup:
     matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
     matA = pv.csr_create(problem.A.rowptr, problem.A.col, problem.A.val)
     d = pv.csr_diagscale(matB, 1.0)                      # Bdiagscaling, :252-342
+    if pv.csr_info(matA)["layout"] == "bsr3":
+        pv.set_value_order(matA, True)                   # 64-block value tiles (measured ~1.5% faster)
     t = dict(upload=time.perf_counter() - t0)
     t0 = time.perf_counter()
     tuned = {}

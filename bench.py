@@ -209,6 +209,7 @@ def main():
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
+    ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint of the solved pairs (0 skips)")
     ap.add_argument("--layouts", default="auto",
                     help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
                          "'A:<g>,B:<g>' (g in 8,16,32,64), e.g. for profiler runs, where timing-based tuning "
@@ -324,6 +325,24 @@ def main():
             log(f"[bench] fp32 filter: {line['fp32_filter']}")
         except Exception as e:
             line["fp32_filter"] = {"error": str(e)}
+    if args.adjoint:
+        try:
+            from normalmodes_amd.adjoint import backward
+            ra = driver.apply_pevsl(mv, interval=interval, vectors=False, block=args.block, keep=True)
+            try:
+                backward(ra.handle, P.mesh)   # first call: element data upload and kernel load
+                t0 = time.perf_counter()
+                g = backward(ra.handle, P.mesh)
+                ta = time.perf_counter() - t0
+            finally:
+                pv.pevsl_finish(ra.handle)
+            line["adjoint"] = {"seconds": round(ta, 4), "elements": P.mesh.n_elem, "pairs": int(ra.lam.size),
+                               "element_pairs_per_s": round(P.mesh.n_elem * ra.lam.size / ta, 0),
+                               "note": "d lambda / d(rho, lambda, mu) per element for every pair, host<->device "
+                                       "copies of the element data and the three gradients included"}
+            log(f"[bench] adjoint: {line['adjoint']}")
+        except Exception as e:
+            line["adjoint"] = {"error": str(e)}
     if args.spmv_workload and args.spmv_workload != "none":
         try:
             line["spmv_hbm"] = spmv_rates(args.spmv_workload)

@@ -119,3 +119,64 @@ def _check_fd(small_p2, solved, dense_eigs):
                 vals.append(eig_near(*arr))
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[k][j, e]) <= 1e-4 * abs(g[k][j, e]) + 1e-7 * lamj, (int(e), k, fd, g[k][j, e])
+
+
+@pytest.mark.gpu
+def test_adjoint_heterogeneous_ellipsoid():
+    """The adjoint on a small version of the heterogeneous-ellipsoid case:
+    flattening 1/300, layered profile with a seeded 2% Gaussian random field
+    (correlation length 0.1) on rho, vp, vs; the 50 lowest non-rigid modes;
+    central finite differences with relative step 1e-6 on 20 seeded random
+    elements for the best-separated eigenvalue."""
+    import scipy.linalg as sl
+    import scipy.sparse.linalg as sla
+    from normalmodes_amd.adjoint import backward
+    from normalmodes_amd.assemble import assemble
+    from normalmodes_amd.driver import apply_pevsl, setupmatvec
+    from normalmodes_amd import pevsl as pv
+    from normalmodes_amd.mesh import ellipsoid_mesh
+    from normalmodes_amd.models import element_node_values, gaussian_random_field, three_layer_solid
+    mesh = ellipsoid_mesh(500, flattening=1.0 / 300.0, porder=2, seed=1019)
+    rho, vp, vs = element_node_values(mesh, three_layer_solid(), perturb=gaussian_random_field(2018, 0.02, 0.1))
+    P = assemble(mesh, rho=rho, vp=vp, vs=vs)
+    w = sl.eigh(P.A.to_scipy().toarray(), P.B.to_scipy().toarray(), eigvals_only=True)
+    a, b = 0.5 * w[6], 0.5 * (w[55] + w[56])
+    ref = w[(w >= a) & (w <= b)]
+    assert ref.size == 50
+    mv = setupmatvec(P, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(a, b), keep=True)
+        try:
+            g = backward(r.handle, mesh)
+        finally:
+            pv.pevsl_finish(r.handle)
+    finally:
+        mv.free()
+    assert r.lam.size == 50 and np.max(np.abs(r.lam - ref) / ref) < 1e-10
+    lam_n, mu_n, rho_n = P.lam, P.mu, P.rho
+    gaps = np.minimum(np.diff(np.concatenate([[0.0], ref])), np.diff(np.concatenate([ref, [np.inf]])))
+    j = int(np.argmax(gaps / ref))
+    lamj = ref[j]
+    rng = np.random.default_rng(2018)
+    elems = rng.choice(mesh.n_elem, size=20, replace=False)
+    rel = 1e-6
+
+    def eig_near(rh, la, mu):
+        v_s = np.sqrt(mu / rh)
+        v_p = np.sqrt((la + 2 * mu) / rh)
+        Q = assemble(mesh, rho=rh, vp=v_p, vs=v_s)
+        ev = sla.eigsh(Q.A.to_scipy().tocsc(), k=4, M=Q.B.to_scipy().tocsc(), sigma=lamj * (1 - 1e-7),
+                       which="LM", tol=1e-15, return_eigenvectors=False)
+        return ev[np.argmin(np.abs(ev - lamj))]
+
+    for e in elems:
+        for k, base in ((0, rho_n), (1, lam_n), (2, mu_n)):
+            h = rel * float(np.abs(base[e]).mean())
+            vals = []
+            for sgn in (+1, -1):
+                arr = [rho_n.copy(), lam_n.copy(), mu_n.copy()]
+                arr[k][e] += sgn * h
+                vals.append(eig_near(*arr))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            gmax = np.abs(g[k][j]).max()
+            assert abs(fd - g[k][j, e]) <= 1e-5 * abs(g[k][j, e]) + 1e-6 * gmax, (int(e), k, fd, g[k][j, e])

@@ -500,12 +500,20 @@ struct Solver {
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
   DVec gw;                   // partial sums of the tall-skinny products
   bool borrowed = false;  // stream and rocBLAS handle belong to a parent solver
+  // k-column B-solves split into two halves on two streams (default priority)
+  int split = 0;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  DVec hb[6];
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
 
   ~Solver() {
     free_basis();
     dfree(X);
     if (borrowed) return;
+    if (s2) (void)hipStreamDestroy(s2);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (blas) rocblas_destroy_handle(blas);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -1061,6 +1069,28 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
 // y (interleaved n*K) ~ B~^{-1} b, all K columns at once
 static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
   NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
+  if (S.split && K >= 4 && !S.borrowed) {
+    // two K/2-column solves at once on two streams: independent columns, the
+    // same matrix read by both
+    if (!S.s2) {
+      NMG_HIP(hipStreamCreateWithFlags(&S.s2, hipStreamNonBlocking));
+      NMG_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
+      NMG_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
+    }
+    const int H = K / 2;
+    for (int i = 0; i < 6; ++i) S.hb[i].ensure(S.n * H);
+    k_split_halves(S.n, K, b, S.hb[0].p, S.hb[1].p, S.s);
+    NMG_HIP(hipEventRecord(S.ev[0], S.s));
+    NMG_HIP(hipStreamWaitEvent(S.s2, S.ev[0], 0));
+    block_cheb(*S.bsol, H, S.hb[0].p, S.hb[2].p, S.hb[4].p, S.s, S.fp32(), &S.probe[0]);
+    block_cheb(*S.bsol, H, S.hb[1].p, S.hb[3].p, S.hb[5].p, S.s2, S.fp32(), nullptr);
+    NMG_HIP(hipEventRecord(S.ev[1], S.s2));
+    NMG_HIP(hipStreamWaitEvent(S.s, S.ev[1], 0));
+    k_merge_halves(S.n, K, S.hb[2].p, S.hb[3].p, out, S.s);
+    S.bsol->ncalls += K;
+    S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
+    return;
+  }
   block_cheb(*S.bsol, K, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
   S.bsol->ncalls += K;
   S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
@@ -2177,6 +2207,13 @@ int nmg_set_residual_target(nmg_handle h, double target) {
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(target >= 0.0 && std::isfinite(target), NMG_EINVAL, "target must be >= 0");
     S->resid_target = target;
+  });
+}
+
+int nmg_set_split_streams(nmg_handle h, int on) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    S->split = on ? 1 : 0;
   });
 }
 

@@ -1071,6 +1071,36 @@ void k_block_colscale(int64_t n, int K, const double* mu, const double* x, doubl
   NMG_HIP(hipGetLastError());
 }
 
+// interleaved n x K <-> two interleaved n x K/2 halves (columns 0..K/2-1 and
+// K/2..K-1)
+__global__ void split_halves_k(int64_t n, int K, const double* __restrict__ x, double* __restrict__ a,
+                               double* __restrict__ b) {
+  const int H = K / 2;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = t / K;
+    const int c = (int)(t % K);
+    if (c < H) a[i * H + c] = x[t];
+    else b[i * H + c - H] = x[t];
+  }
+}
+__global__ void merge_halves_k(int64_t n, int K, const double* __restrict__ a, const double* __restrict__ b,
+                               double* __restrict__ y) {
+  const int H = K / 2;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = t / K;
+    const int c = (int)(t % K);
+    y[t] = c < H ? a[i * H + c] : b[i * H + c - H];
+  }
+}
+void k_split_halves(int64_t n, int K, const double* x, double* a, double* b, hipStream_t s) {
+  hipLaunchKernelGGL(split_halves_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, x, a, b);
+  NMG_HIP(hipGetLastError());
+}
+void k_merge_halves(int64_t n, int K, const double* a, const double* b, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(merge_halves_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, a, b, y);
+  NMG_HIP(hipGetLastError());
+}
+
 // column-major (n x K, ld) <-> interleaved (i*K + c)
 __global__ void to_interleaved_k(int64_t n, int K, const double* __restrict__ cm, int64_t ld,
                                  double* __restrict__ il) {

@@ -22,7 +22,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "pevsl_freepol", "pevsl_cheblannr", "pevsl_get_nev", "pevsl_copy_result",
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
-           "solve_slices", "slice_counts", "chebiter_set_mode", "chebiter_info",
+           "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
            "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "LAYOUTS"]
 
 FP64 = 0
@@ -321,6 +321,10 @@ def slice_counts(h: Handle, n_slices: int) -> np.ndarray:
     c = np.zeros(n_slices, dtype=np.int32)
     _check(_lib().nmg_slice_counts(_i64(h), _p(c, _i32p)))
     return c
+
+
+def set_split_streams(h: Handle, on: bool = True):
+    _check(_lib().nmg_set_split_streams(_i64(h), ctypes.c_int(int(on))))
 
 
 def set_residual_target(h: Handle, target: float):

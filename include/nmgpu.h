@@ -212,10 +212,10 @@ int nmg_set_residual_target(nmg_handle h, double target);
  * `block` vectors at once so that each sparse product streams the matrices
  * once per block (1, 2, 4, 8 or 16). */
 int nmg_set_block_size(nmg_handle h, int block);
-/* Split each k-column B-solve (k >= 4) into two k/2-column solves run at
- * once on two streams of the handle (no reference counterpart; off by
- * default). */
-int nmg_set_split_streams(nmg_handle h, int on);
+/* Split each k-column B-solve into `parts` (2 or 4; 0 or 1: off) solves of
+ * k/parts >= 2 columns, run at once on as many streams of the handle (no
+ * reference counterpart). */
+int nmg_set_split_streams(nmg_handle h, int parts);
 
 /* Kernel timing probe (no reference counterpart): with every > 0, every
  * `every`-th B-solve (which = 0: its deg back-to-back fused Chebyshev step

@@ -500,18 +500,20 @@ struct Solver {
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
   DVec gw;                   // partial sums of the tall-skinny products
   bool borrowed = false;  // stream and rocBLAS handle belong to a parent solver
-  // k-column B-solves split into two halves on two streams (default priority)
+  // k-column B-solves split into `split` parts on as many streams (default
+  // priority): the handle's stream and split-1 more
   int split = 0;
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  DVec hb[6];
+  hipStream_t sx[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DVec hb[12];
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
 
   ~Solver() {
     free_basis();
     dfree(X);
     if (borrowed) return;
-    if (s2) (void)hipStreamDestroy(s2);
+    for (auto x : sx)
+      if (x) (void)hipStreamDestroy(x);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
     if (blas) rocblas_destroy_handle(blas);
@@ -1069,24 +1071,33 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
 // y (interleaved n*K) ~ B~^{-1} b, all K columns at once
 static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
   NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
-  if (S.split && K >= 4 && !S.borrowed) {
-    // two K/2-column solves at once on two streams: independent columns, the
-    // same matrix read by both
-    if (!S.s2) {
-      NMG_HIP(hipStreamCreateWithFlags(&S.s2, hipStreamNonBlocking));
-      NMG_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
-      NMG_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
+  const int P = S.split;
+  if (P > 1 && K % P == 0 && K / P >= 2 && !S.borrowed) {
+    // P solves of K/P columns at once on P streams: independent columns, the
+    // same matrix read by all
+    if (!S.sx[0]) {
+      for (int q = 0; q < 3; ++q) NMG_HIP(hipStreamCreateWithFlags(&S.sx[q], hipStreamNonBlocking));
+      for (int q = 0; q < 4; ++q) NMG_HIP(hipEventCreateWithFlags(&S.ev[q], hipEventDisableTiming));
     }
-    const int H = K / 2;
-    for (int i = 0; i < 6; ++i) S.hb[i].ensure(S.n * H);
-    k_split_halves(S.n, K, b, S.hb[0].p, S.hb[1].p, S.s);
+    const int H = K / P;
+    for (int i = 0; i < 3 * P; ++i) S.hb[i].ensure(S.n * H);
+    double* in[4];
+    double* res[4];
+    for (int q = 0; q < P; ++q) {
+      in[q] = S.hb[3 * q].p;
+      res[q] = S.hb[3 * q + 1].p;
+    }
+    k_split_parts(S.n, K, P, b, in, S.s);
     NMG_HIP(hipEventRecord(S.ev[0], S.s));
-    NMG_HIP(hipStreamWaitEvent(S.s2, S.ev[0], 0));
-    block_cheb(*S.bsol, H, S.hb[0].p, S.hb[2].p, S.hb[4].p, S.s, S.fp32(), &S.probe[0]);
-    block_cheb(*S.bsol, H, S.hb[1].p, S.hb[3].p, S.hb[5].p, S.s2, S.fp32(), nullptr);
-    NMG_HIP(hipEventRecord(S.ev[1], S.s2));
-    NMG_HIP(hipStreamWaitEvent(S.s, S.ev[1], 0));
-    k_merge_halves(S.n, K, S.hb[2].p, S.hb[3].p, out, S.s);
+    for (int q = 1; q < P; ++q) NMG_HIP(hipStreamWaitEvent(S.sx[q - 1], S.ev[0], 0));
+    for (int q = 0; q < P; ++q)
+      block_cheb(*S.bsol, H, in[q], res[q], S.hb[3 * q + 2].p, q == 0 ? S.s : S.sx[q - 1], S.fp32(),
+                 q == 0 ? &S.probe[0] : nullptr);
+    for (int q = 1; q < P; ++q) {
+      NMG_HIP(hipEventRecord(S.ev[q], S.sx[q - 1]));
+      NMG_HIP(hipStreamWaitEvent(S.s, S.ev[q], 0));
+    }
+    k_merge_parts(S.n, K, P, res, out, S.s);
     S.bsol->ncalls += K;
     S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
     return;
@@ -2210,10 +2221,11 @@ int nmg_set_residual_target(nmg_handle h, double target) {
   });
 }
 
-int nmg_set_split_streams(nmg_handle h, int on) {
+int nmg_set_split_streams(nmg_handle h, int parts) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
-    S->split = on ? 1 : 0;
+    NMG_CHECK(parts >= 0 && parts <= 4 && parts != 3, NMG_EINVAL, "parts must be 0, 1, 2 or 4");
+    S->split = parts;
   });
 }
 

@@ -187,8 +187,9 @@ void k_block_filter_step(int K, const DevMat& A, const double* x, const double* 
 size_t block_dot_work_size(int K);
 void k_block_dots(int64_t n, int K, const double* x, const double* y, double* out, double* work, hipStream_t s);
 void k_block_colscale(int64_t n, int K, const double* mu, const double* x, double* y, hipStream_t s);
-void k_split_halves(int64_t n, int K, const double* x, double* a, double* b, hipStream_t s);
-void k_merge_halves(int64_t n, int K, const double* a, const double* b, double* y, hipStream_t s);
+// interleaved n x K <-> P (<= 4) interleaved parts of K/P columns
+void k_split_parts(int64_t n, int K, int P, const double* x, double* const* parts, hipStream_t s);
+void k_merge_parts(int64_t n, int K, int P, double* const* parts, double* y, hipStream_t s);
 void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s);
 void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s);
 // per-element eigenvalue sensitivities (see nmgpu_kernels.hip)

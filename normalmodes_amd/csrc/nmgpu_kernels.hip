@@ -1071,33 +1071,39 @@ void k_block_colscale(int64_t n, int K, const double* mu, const double* x, doubl
   NMG_HIP(hipGetLastError());
 }
 
-// interleaved n x K <-> two interleaved n x K/2 halves (columns 0..K/2-1 and
-// K/2..K-1)
-__global__ void split_halves_k(int64_t n, int K, const double* __restrict__ x, double* __restrict__ a,
-                               double* __restrict__ b) {
-  const int H = K / 2;
+// interleaved n x K <-> P interleaved n x K/P parts (part p holds columns
+// p*K/P .. (p+1)*K/P - 1)
+struct Parts {
+  double* p[4];
+};
+__global__ void split_parts_k(int64_t n, int K, int P, const double* __restrict__ x, Parts out) {
+  const int H = K / P;
   for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
     const int64_t i = t / K;
     const int c = (int)(t % K);
-    if (c < H) a[i * H + c] = x[t];
-    else b[i * H + c - H] = x[t];
+    out.p[c / H][i * H + c % H] = x[t];
   }
 }
-__global__ void merge_halves_k(int64_t n, int K, const double* __restrict__ a, const double* __restrict__ b,
-                               double* __restrict__ y) {
-  const int H = K / 2;
+__global__ void merge_parts_k(int64_t n, int K, int P, Parts in, double* __restrict__ y) {
+  const int H = K / P;
   for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n * K; t += (int64_t)gridDim.x * kBlock) {
     const int64_t i = t / K;
     const int c = (int)(t % K);
-    y[t] = c < H ? a[i * H + c] : b[i * H + c - H];
+    y[t] = in.p[c / H][i * H + c % H];
   }
 }
-void k_split_halves(int64_t n, int K, const double* x, double* a, double* b, hipStream_t s) {
-  hipLaunchKernelGGL(split_halves_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, x, a, b);
+void k_split_parts(int64_t n, int K, int P, const double* x, double* const* parts, hipStream_t s) {
+  NMG_CHECK(P >= 1 && P <= 4 && K % P == 0, NMG_EINVAL, "split into 1 to 4 equal parts");
+  Parts o{};
+  for (int q = 0; q < P; ++q) o.p[q] = parts[q];
+  hipLaunchKernelGGL(split_parts_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, P, x, o);
   NMG_HIP(hipGetLastError());
 }
-void k_merge_halves(int64_t n, int K, const double* a, const double* b, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(merge_halves_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, a, b, y);
+void k_merge_parts(int64_t n, int K, int P, double* const* parts, double* y, hipStream_t s) {
+  NMG_CHECK(P >= 1 && P <= 4 && K % P == 0, NMG_EINVAL, "merge of 1 to 4 equal parts");
+  Parts o{};
+  for (int q = 0; q < P; ++q) o.p[q] = parts[q];
+  hipLaunchKernelGGL(merge_parts_k, dim3(grid_for(n * K)), dim3(kBlock), 0, s, n, K, P, o, y);
   NMG_HIP(hipGetLastError());
 }
 

@@ -253,7 +253,7 @@ class EigResult:
 
 
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split=False):
+                 split=0):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -274,7 +274,7 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
         if block > 1:
             pv.set_block_size(h, block)
         if split:
-            pv.set_split_streams(h, True)
+            pv.set_split_streams(h, int(split))
         if resid_target is None and precision == pv.FP32_FILTER:
             # the FP32 B-solve bounds ||A~x - lam B~x||/|lam| near 1e-6 (its
             # eigenvalues still agree with FP64 to ~1e-14): stop at that floor
@@ -292,7 +292,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
-                keep: bool = False, split: bool = False) -> EigResult:
+                keep: bool = False, split: int = 0) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)

@@ -323,8 +323,9 @@ def slice_counts(h: Handle, n_slices: int) -> np.ndarray:
     return c
 
 
-def set_split_streams(h: Handle, on: bool = True):
-    _check(_lib().nmg_set_split_streams(_i64(h), ctypes.c_int(int(on))))
+def set_split_streams(h: Handle, parts: int = 2):
+    """k-column B-solves in `parts` (2 or 4) concurrent parts on as many streams; 0 turns it off."""
+    _check(_lib().nmg_set_split_streams(_i64(h), ctypes.c_int(int(parts))))
 
 
 def set_residual_target(h: Handle, target: float):

@@ -495,3 +495,29 @@ def test_bsr3_value_order():
     finally:
         pv.csr_free(ha)
         pv.csr_free(hb)
+
+
+def test_split_stream_bsolve_and_solve(small_p2, dense_eigs):
+    """k-column B-solves split over 2 and 4 streams: the same block products
+    (to rounding) and the same eigenpairs as one stream."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg="auto", tune=(1, 2, 4, 8))
+    try:
+        X = np.random.default_rng(41).standard_normal((8, small_p2.n))
+        out = {}
+        for parts in (0, 2, 4):
+            h = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, split=parts)
+            try:
+                out[parts] = pv.apply_block(h, 2, X)
+            finally:
+                pv.pevsl_finish(h)
+        assert _rel(out[2], out[0]) < 1e-14 and _rel(out[4], out[0]) < 1e-14
+        w = dense_eigs(small_p2)
+        ref = w[(w >= 1.0) & (w <= 12.0)]
+        for parts in (2, 4):
+            r = apply_pevsl(mv, interval=(1.0, 12.0), block=8, split=parts, vectors=False)
+            assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-12
+            assert np.all(r.res2 <= 1e-10)
+    finally:
+        mv.free()

@@ -106,15 +106,16 @@ def kernel_only(args):
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem
     P = build_problem(args.workload)
-    mv = upload_with_layouts(P, args.layouts, sorted({1, args.block}))
+    kpart = args.block // args.split if args.block > 1 and args.split > 1 else args.block
+    mv = upload_with_layouts(P, args.layouts, sorted({1, kpart}))
     driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
     n_copy = 1 << 24   # 16M doubles: 128 MiB read + 128 MiB written per launch
     t_copy = pv.bench_copy(n_copy, 5)
     info = pv.chebiter_info(mv.chebB)
-    if args.block > 1:   # the k-column Chebyshev step, repeated
+    if kpart > 1:   # the k-column Chebyshev step of one part, repeated
         reps = args.steps * mv.bdeg
-        t, b = pv.bench_block_chebstep(mv.chebB, args.block, reps)
-        kern = {"kernel": f"k-column Chebyshev step, K={args.block}", "launches": reps + 1}
+        t, b = pv.bench_block_chebstep(mv.chebB, kpart, reps)
+        kern = {"kernel": f"k-column Chebyshev step, K={kpart}", "launches": reps + 1}
     else:
         t, b = pv.bench_bsolve(mv.chebB, args.steps)
         kern = {"kernel": "B-solve (deg single-column steps)", "launches": args.steps + 1}
@@ -125,12 +126,12 @@ def kernel_only(args):
     mv.free()
 
 
-def solve_once(mv, interval, bdeg, precision, probe_every, block):
+def solve_once(mv, interval, bdeg, precision, probe_every, block, split=0):
     from normalmodes_amd import driver, pevsl as pv
     t0 = time.perf_counter()
     driver.setup_bsolve(mv, bdeg=bdeg)
     r = driver.apply_pevsl(mv, interval=interval, precision=precision, vectors=False,
-                           probe_every=probe_every, block=block)
+                           probe_every=probe_every, block=block, split=split if block > 1 else 0)
     return time.perf_counter() - t0, r
 
 
@@ -203,6 +204,9 @@ def main():
     ap.add_argument("--block", type=int, default=8,
                     help="Lanczos block width (1 = the reference's single-vector ChebLanNr; the block variant "
                          "reorthogonalises with FP64 GEMMs and streams the matrices once per K columns)")
+    ap.add_argument("--split", type=int, default=2,
+                    help="run each k-column B-solve as this many concurrent parts on as many streams (2 or 4; "
+                         "0 = one stream)")
     ap.add_argument("--single-compare", type=int, default=1,
                     help="with --block > 1, also time one single-vector (block 1) solve (0 skips)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
@@ -235,17 +239,18 @@ def main():
     interval = INTERVALS[args.workload]
     log(f"[bench] built {args.workload}: n={P.n} nnzA={P.A.nnz} nnzB={P.B.nnz} in {t_build:.1f}s")
     t0 = time.perf_counter()
-    mv = upload_with_layouts(P, args.layouts, sorted({1, args.block}))
+    kpart = args.block // args.split if args.block > 1 and args.split > 1 else args.block
+    mv = upload_with_layouts(P, args.layouts, sorted({1, args.block, kpart}))
     t_upload = time.perf_counter() - t0
     precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
 
     for i in range(args.warmup):
-        el, r = solve_once(mv, interval, bdeg, precision, 0, args.block)
+        el, r = solve_once(mv, interval, bdeg, precision, 0, args.block, args.split)
         log(f"[bench] warmup {i}: {el:.2f}s nev={r.lam.size} steps={r.lan_steps}")
     times = []
     r = None
     for i in range(args.steps):
-        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every, args.block)
+        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every, args.block, args.split)
         times.append(el)
         log(f"[bench] step {i}: {el:.2f}s nev={r.lam.size} deg={r.deg} steps={r.lan_steps} "
             f"max res2={np.max(r.res2) if r.res2.size else float('nan'):.2e}")
@@ -255,7 +260,7 @@ def main():
     achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
     persistent = pv.chebiter_info(mv.chebB)["persistent"]
     kname = ("cheb_persistent_k" if persistent else
-             ("mass3_chebstep" if args.block == 1 else f"block{args.block}_chebstep"))
+             ("mass3_chebstep" if args.block == 1 else f"block{kpart}_chebstep"))
     ktag = f"{kname}:{args.workload}"
     traffic, traffic_src = traffic_from_profiles(ktag)
     line = {
@@ -279,6 +284,7 @@ def main():
             "interval": list(interval), "nev": int(r.lam.size), "filter_degree": r.deg,
             "bsolve_degree": mv.bdeg, "lanczos_steps": r.lan_steps,
             "lanbounds_steps": r.lan_steps_bounds, "precision": args.precision, "block": args.block,
+            "bsolve_streams": args.block // kpart if args.block > 1 else 1,
             "kernel_layouts": mv.tuned,
             "parallelism": "1 GPU", "global_batch": 1, "seq_len": P.n,
         },
@@ -292,13 +298,18 @@ def main():
             "kernel": ("persistent Chebyshev B-solve (all steps in one cooperative launch; bytes = steps x "
                        "one fused step's algorithmic bytes)" if persistent else
                        "fused Chebyshev B-solve step (scalar mass x I3 product + update)" if args.block == 1 else
-                       f"fused {args.block}-column Chebyshev B-solve step (scalar mass x I3 product + update on "
-                       f"{args.block} interleaved columns)"),
+                       f"fused {kpart}-column Chebyshev B-solve step (scalar mass x I3 product + update on "
+                       f"{kpart} interleaved columns)" + (f"; {args.block // kpart} such solves run at once "
+                                                           f"on as many streams" if kpart != args.block else "")),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
             "bytes_per_launch": chk["bytes"], "mean_launch_us": round(chk["seconds"] * 1e6, 3),
             "samples": chk["samples"],
+            # with the B-solve split over streams, the other parts' launches run
+            # beside each timed one: their sum is the bandwidth the step draws
+            "concurrent_parts": args.block // kpart if args.block > 1 else 1,
+            "aggregate_achieved": round(achieved * (args.block // kpart if args.block > 1 else 1), 1),
             "filter_step": r.probe["filterstep"],
         },
     }
@@ -314,7 +325,7 @@ def main():
             line["single_vector"] = {"error": str(e)}
     if args.fp32_compare:
         try:
-            el32, r32 = solve_once(mv, interval, bdeg, pv.FP32_FILTER, 0, args.block)
+            el32, r32 = solve_once(mv, interval, bdeg, pv.FP32_FILTER, 0, args.block, args.split)
             m = min(r32.lam.size, r.lam.size)
             line["fp32_filter"] = {
                 "seconds": round(el32, 4), "nev": int(r32.lam.size),
@@ -328,7 +339,8 @@ def main():
     if args.adjoint:
         try:
             from normalmodes_amd.adjoint import backward
-            ra = driver.apply_pevsl(mv, interval=interval, vectors=False, block=args.block, keep=True)
+            ra = driver.apply_pevsl(mv, interval=interval, vectors=False, block=args.block, keep=True,
+                                    split=args.split if args.block > 1 else 0)
             try:
                 backward(ra.handle, P.mesh)   # first call: element data upload and kernel load
                 t0 = time.perf_counter()

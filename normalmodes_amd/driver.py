@@ -252,8 +252,16 @@ class EigResult:
         return None if self.X is None else self.X * d[None, :]
 
 
+def split_parts(block: int, split="auto") -> int:
+    """Streams the k-column B-solve is split over: 'auto' picks 2 for blocks
+    of 8 or more (two 4-column solves overlap their dispatch gaps), else 1."""
+    if split == "auto":
+        return 2 if block >= 8 and block % 2 == 0 else 0
+    return int(split or 0)
+
+
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split=0):
+                 split="auto"):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -273,8 +281,9 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
             pv.set_kernel_probe(h, probe_every)
         if block > 1:
             pv.set_block_size(h, block)
-        if split:
-            pv.set_split_streams(h, int(split))
+        parts = split_parts(block, split)
+        if parts > 1:
+            pv.set_split_streams(h, parts)
         if resid_target is None and precision == pv.FP32_FILTER:
             # the FP32 B-solve bounds ||A~x - lam B~x||/|lam| near 1e-6 (its
             # eigenvalues still agree with FP64 to ~1e-14): stop at that floor
@@ -292,7 +301,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
-                keep: bool = False, split: int = 0) -> EigResult:
+                keep: bool = False, split="auto") -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)

@@ -158,11 +158,22 @@ int nmg_residuals(nmg_handle h, double *res_ref, double *res2);
  * (physical, B-normalised).  elems: int32[n_elem*np] node ids (DoF 3*v+p),
  * detJ[n_elem], invJ[n_elem*9] (row-major, invJ = 2 B^-1 of the affine map),
  * refM[np*np], refD[3*np*np] reference-element mass and derivative matrices.
- * Outputs d_rho, d_lam, d_mu: f64[nev*n_elem], eigenpair-major.  job = 1
- * (no reference gravity). */
+ * Outputs d_rho, d_lam, d_mu: f64[nev*n_elem], eigenpair-major.  job must be
+ * 1 (no reference gravity); JOB=2 runs go through nmg_backward_grav. */
 int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t *elems, const double *detJ,
                  const double *invJ, const double *refM, const double *refD, int job, double *d_rho,
                  double *d_lam, double *d_mu);
+/* nmg_backward for JOB=2 (no reference counterpart; the gravity terms are
+ * OPrho of src/mod_cg_create_matrix.f90:1131-1180, linear in the nodal
+ * density, with the gravity field held fixed): d_rho also carries
+ * x_j^T (dA_e/drho_e) x_j.  gnode: f64[n_elem*np*3], the reference gravity
+ * at each element node (the g0 the stiffness was assembled with); gsym:
+ * f64[n_elem*9], the symmetric part (dg + dg^T)/2 of each element's
+ * least-squares gravity gradient, dg[r*3+c] = d g_c / d x_r
+ * (src/mod_cg_create_matrix.f90:1054-1080). */
+int nmg_backward_grav(nmg_handle h, int64_t n_elem, int np, const int32_t *elems, const double *detJ,
+                      const double *invJ, const double *refM, const double *refD, const double *gnode,
+                      const double *gsym, double *d_rho, double *d_lam, double *d_mu);
 
 /* Density of states by the kernel polynomial method, as pEVSL's kpmdos
  * (the reference fixes NSLICES = 1, src/mod_pevsl.f90:47, 104-105, and never

@@ -5,7 +5,9 @@ d lambda_j / d m_e = x_j^T (dA_e/dm_e - lambda_j dB_e/dm_e) x_j for m_e in
 x_j the physical, B-normalised eigenvector x = D x~.  The element matrices
 are those of the elastic operator (src/mod_cg_create_matrix.f90:1018-1219);
 the device computes the quadratic forms directly from the element's nodal
-fields (nmg_backward in include/nmgpu.h).
+fields (nmg_backward in include/nmgpu.h).  With reference gravity (JOB=2)
+the stiffness also depends on rho (OPrho, :1131-1180) with the gravity field
+held fixed, and d_rho carries that term (nmg_backward_grav).
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
-__all__ = ["element_geometry", "backward", "element_matrices"]
+__all__ = ["element_geometry", "element_gravity", "backward", "element_matrices"]
 
 
 def element_geometry(mesh):
@@ -41,12 +43,33 @@ def element_geometry(mesh):
     return detJ, invJ.reshape(ne, 3, 3)
 
 
-def backward(h, mesh, job: int = 1):
+def element_gravity(mesh, g0):
+    """Symmetric least-squares gravity gradients (n_elem, 3, 3) of the nodal
+    gravity g0 (n_elem, np, 3), as the stiffness uses them."""
+    L = load_asm()
+    ne, npl = mesh.n_elem, mesh.elems.shape[1]
+    g = np.ascontiguousarray(g0, dtype=np.float64)
+    if g.shape != (ne, npl, 3):
+        raise ValueError(f"g0 must be (n_elem, np, 3) = {(ne, npl, 3)}, got {g.shape}")
+    gsym = np.empty((ne, 9))
+    rc = L.nm_element_gravity(ctypes.c_int64(ne), ctypes.c_int(npl),
+                              np.ascontiguousarray(mesh.nodes).ctypes.data_as(_f64p),
+                              np.ascontiguousarray(mesh.elems, dtype=np.int32).ctypes.data_as(_i32p),
+                              g.ctypes.data_as(_f64p), gsym.ctypes.data_as(_f64p))
+    if rc != 0:
+        raise RuntimeError("element gravity gradients failed")
+    return gsym.reshape(ne, 3, 3)
+
+
+def backward(h, mesh, job: int = 1, g0=None):
     """Sensitivities of the eigenpairs held by solver handle h (after
     apply_pevsl(..., keep=True)).  Returns (d_rho, d_lam, d_mu), each
     (nev, n_elem), rows in ascending eigenvalue order as apply_pevsl returns
-    the pairs."""
+    the pairs.  job=2 takes the nodal reference gravity g0 (n_elem, np, 3)
+    the stiffness was assembled with."""
     L = load_gpu()
+    if job == 2 and g0 is None:
+        raise ValueError("job 2 needs g0")
     detJ, invJ = element_geometry(mesh)
     _, M, D = reference_element(mesh.porder)
     ne, npl = mesh.n_elem, mesh.elems.shape[1]
@@ -56,10 +79,16 @@ def backward(h, mesh, job: int = 1):
     M = np.ascontiguousarray(M)
     D = np.ascontiguousarray(D)
     invJ = np.ascontiguousarray(invJ.reshape(ne, 9))
-    pv._check(L.nmg_backward(ctypes.c_int64(h), ctypes.c_int64(ne), ctypes.c_int(npl), elems.ctypes.data_as(_i32p),
-                             detJ.ctypes.data_as(_f64p), invJ.ctypes.data_as(_f64p), M.ctypes.data_as(_f64p),
-                             D.ctypes.data_as(_f64p), ctypes.c_int(job), out[0].ctypes.data_as(_f64p),
-                             out[1].ctypes.data_as(_f64p), out[2].ctypes.data_as(_f64p)))
+    common = (ctypes.c_int64(h), ctypes.c_int64(ne), ctypes.c_int(npl), elems.ctypes.data_as(_i32p),
+              detJ.ctypes.data_as(_f64p), invJ.ctypes.data_as(_f64p), M.ctypes.data_as(_f64p),
+              D.ctypes.data_as(_f64p))
+    outs = tuple(o.ctypes.data_as(_f64p) for o in out)
+    if job == 2:
+        g = np.ascontiguousarray(g0, dtype=np.float64)
+        gsym = np.ascontiguousarray(element_gravity(mesh, g).reshape(ne, 9))
+        pv._check(L.nmg_backward_grav(*common, g.ctypes.data_as(_f64p), gsym.ctypes.data_as(_f64p), *outs))
+    else:
+        pv._check(L.nmg_backward(*common, ctypes.c_int(job), *outs))
     lam, _ = pv.pevsl_copy_result(h, 3 * mesh.n_nodes, False)
     order = np.argsort(lam, kind="stable")
     return tuple(o[order] for o in out)

@@ -75,6 +75,31 @@ inline bool inv3(const double B[9], double Bi[9], double* det) {
   return true;
 }
 
+// Least-squares gradient of the nodal gravity over one element
+// (src/mod_cg_create_matrix.f90:1054-1080): dg = (X^T X)^{-1} X^T (g - mean g)
+// with X the node coordinates minus their mean; dg[r*3+c] = d g_c / d x_r.
+void gravity_gradient(int n, const double* xn, const double* g, double dg[9]) {
+  double xm[3] = {0, 0, 0}, gm[3] = {0, 0, 0};
+  for (int a = 0; a < n; ++a)
+    for (int c = 0; c < 3; ++c) { xm[c] += xn[a * 3 + c]; gm[c] += g[a * 3 + c]; }
+  for (int c = 0; c < 3; ++c) { xm[c] /= n; gm[c] /= n; }
+  double N[9] = {0}, R[9] = {0};
+  for (int a = 0; a < n; ++a)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        N[r * 3 + c] += (xn[a * 3 + r] - xm[r]) * (xn[a * 3 + c] - xm[c]);
+        R[r * 3 + c] += (xn[a * 3 + r] - xm[r]) * (g[a * 3 + c] - gm[c]);
+      }
+  double Ni[9], dN;
+  inv3(N, Ni, &dN);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += Ni[r * 3 + k] * R[k * 3 + c];
+      dg[r * 3 + c] = s;
+    }
+}
+
 // Element stiffness (3np x 3np, block (i,j) at rows i*np.., cols j*np..) and
 // scalar mass (np x np).  Returns false for a flat element.
 bool element_matrices(const Elem& E, const double* X /*4x3 vertices*/, const double* lam,
@@ -123,26 +148,7 @@ bool element_matrices(const Elem& E, const double* X /*4x3 vertices*/, const dou
   if (grav) {
     for (int a = 0; a < n; ++a)
       for (int c = 0; c < 3; ++c) G[c][a] = g[a * 3 + c];
-    // least-squares gradient: dg = (dism^T dism)^{-1} dism^T (g - mean g)
-    double xm[3] = {0, 0, 0}, gm[3] = {0, 0, 0};
-    for (int a = 0; a < n; ++a)
-      for (int c = 0; c < 3; ++c) { xm[c] += xn[a * 3 + c]; gm[c] += g[a * 3 + c]; }
-    for (int c = 0; c < 3; ++c) { xm[c] /= n; gm[c] /= n; }
-    double N[9] = {0}, R[9] = {0};
-    for (int a = 0; a < n; ++a)
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          N[r * 3 + c] += (xn[a * 3 + r] - xm[r]) * (xn[a * 3 + c] - xm[c]);
-          R[r * 3 + c] += (xn[a * 3 + r] - xm[r]) * (g[a * 3 + c] - gm[c]);
-        }
-    double Ni[9], dN;
-    inv3(N, Ni, &dN);
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) {
-        double s = 0;
-        for (int k = 0; k < 3; ++k) s += Ni[r * 3 + k] * R[k * 3 + c];
-        dg[r * 3 + c] = s;
-      }
+    gravity_gradient(n, xn, g, dg);
   }
 
   const int L = 3 * n;
@@ -380,6 +386,25 @@ int nm_element_geometry(int64_t n_elem, int np, const double* nodes, const int32
 }  // extern "C"
 
 extern "C" {
+
+// Symmetric part of the least-squares gravity gradient of every element,
+// gsym[e*9 + i*3 + j] = (dg_ij + dg_ji)/2 with dg from the element's nodal
+// gravity g (n_elem x np x 3) and node coordinates: the coefficient of M in
+// the gravity block (i,j) of the stiffness (used by the adjoint with JOB=2).
+int nm_element_gravity(int64_t n_elem, int np, const double* nodes, const int32_t* elems, const double* g,
+                       double* gsym) {
+  if (np > kMaxNp) return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elem; ++e) {
+    double xn[kMaxNp * 3], dg[9];
+    for (int a = 0; a < np; ++a)
+      for (int c = 0; c < 3; ++c) xn[a * 3 + c] = nodes[(int64_t)elems[e * np + a] * 3 + c];
+    gravity_gradient(np, xn, g + e * np * 3, dg);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gsym[e * 9 + i * 3 + j] = 0.5 * (dg[i * 3 + j] + dg[j * 3 + i]);
+  }
+  return 0;
+}
 
 // Element matrices for a list of elements (used to check the adjoint):
 // K_e (3np x 3np) and M_e (np x np) for the given per-element-node lam, mu,

@@ -2111,23 +2111,23 @@ int nmg_residuals(nmg_handle h, double* res_ref, double* res2) {
   });
 }
 
-int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, const double* detJ,
-                 const double* invJ, const double* refM, const double* refD, int job, double* d_rho,
-                 double* d_lam, double* d_mu) {
+static int backward_impl(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, const double* detJ,
+                         const double* invJ, const double* refM, const double* refD, const double* gnode,
+                         const double* gsym, double* d_rho, double* d_lam, double* d_mu) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(S->nev > 0 && S->X, NMG_ESTATE, "no eigenpairs: run nmg_cheblannr first");
     NMG_CHECK(S->dscale.n >= S->n, NMG_ESTATE, "the operator was not set with nmg_set_amv_dad");
     NMG_CHECK(!S->fs, NMG_ENOTIMPL, "sensitivities of fluid-solid bodies are not available");
     NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "np must be 4 (P1) or 10 (P2)");
-    NMG_CHECK(job == 1, NMG_ENOTIMPL, "sensitivities with reference gravity (JOB=2) are not available");
     NMG_CHECK(n_elem > 0 && elems && detJ && invJ && refM && refD && d_rho && d_lam && d_mu, NMG_EINVAL,
               "null argument");
+    NMG_CHECK((gnode == nullptr) == (gsym == nullptr), NMG_EINVAL, "gravity and its gradients go together");
     const int64_t nn = S->n / 3;
     for (int64_t k = 0; k < n_elem * np; ++k)
       NMG_CHECK(elems[k] >= 0 && elems[k] < nn, NMG_EINVAL, "element node id out of range");
     const int nev = S->nev;
-    DVec dE, dJ, dI, dM, dD, dlam, gr, gl, gm;
+    DVec dE, dJ, dI, dM, dD, dlam, gr, gl, gm, dg, dgs;
     dE.ensure((n_elem * np + 1) / 2 + 1);
     dJ.ensure(n_elem);
     dI.ensure(9 * n_elem);
@@ -2143,13 +2143,37 @@ int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, con
     NMG_HIP(hipMemcpyAsync(dM.p, refM, sizeof(double) * np * np, hipMemcpyHostToDevice, S->s));
     NMG_HIP(hipMemcpyAsync(dD.p, refD, sizeof(double) * 3 * np * np, hipMemcpyHostToDevice, S->s));
     NMG_HIP(hipMemcpyAsync(dlam.p, S->lam.data(), sizeof(double) * nev, hipMemcpyHostToDevice, S->s));
+    if (gnode) {
+      dg.ensure(n_elem * np * 3);
+      dgs.ensure(n_elem * 9);
+      NMG_HIP(hipMemcpyAsync(dg.p, gnode, sizeof(double) * n_elem * np * 3, hipMemcpyHostToDevice, S->s));
+      NMG_HIP(hipMemcpyAsync(dgs.p, gsym, sizeof(double) * n_elem * 9, hipMemcpyHostToDevice, S->s));
+    }
     k_adjoint(np, n_elem, nev, (const int32_t*)dE.p, dJ.p, dI.p, dM.p, dD.p, S->X, S->n, S->dscale.p, dlam.p,
-              gr.p, gl.p, gm.p, S->s);
+              gnode ? dg.p : nullptr, gnode ? dgs.p : nullptr, gr.p, gl.p, gm.p, S->s);
     NMG_HIP(hipMemcpyAsync(d_rho, gr.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipMemcpyAsync(d_lam, gl.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipMemcpyAsync(d_mu, gm.p, sizeof(double) * n_elem * nev, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
   });
+}
+
+int nmg_backward(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, const double* detJ,
+                 const double* invJ, const double* refM, const double* refD, int job, double* d_rho,
+                 double* d_lam, double* d_mu) {
+  if (job != 1)
+    return guarded([&] {
+      NMG_CHECK(false, NMG_EINVAL, "job 2 (reference gravity) needs the gravity field: use nmg_backward_grav");
+    });
+  return backward_impl(h, n_elem, np, elems, detJ, invJ, refM, refD, nullptr, nullptr, d_rho, d_lam, d_mu);
+}
+
+int nmg_backward_grav(nmg_handle h, int64_t n_elem, int np, const int32_t* elems, const double* detJ,
+                      const double* invJ, const double* refM, const double* refD, const double* gnode,
+                      const double* gsym, double* d_rho, double* d_lam, double* d_mu) {
+  if (!gnode || !gsym)
+    return guarded([&] { NMG_CHECK(false, NMG_EINVAL, "null gravity argument"); });
+  return backward_impl(h, n_elem, np, elems, detJ, invJ, refM, refD, gnode, gsym, d_rho, d_lam, d_mu);
 }
 
 int nmg_kpmdos(nmg_handle h, int mdeg, int nvec, const double* intv, double* mu, double* ecnt) {

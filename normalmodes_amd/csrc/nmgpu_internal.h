@@ -192,10 +192,12 @@ void k_split_parts(int64_t n, int K, int P, const double* x, double* const* part
 void k_merge_parts(int64_t n, int K, int P, double* const* parts, double* y, hipStream_t s);
 void k_to_interleaved(int64_t n, int K, const double* cm, int64_t ld, double* il, hipStream_t s);
 void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t ld, hipStream_t s);
-// per-element eigenvalue sensitivities (see nmgpu_kernels.hip)
+// per-element eigenvalue sensitivities; gnode (ne x np x 3) and gsym (ne x 9)
+// null without reference gravity (see nmgpu_kernels.hip)
 void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* detJ, const double* invJ,
                const double* refM, const double* refD, const double* X, int64_t ldx, const double* d,
-               const double* lam, double* g_rho, double* g_lam, double* g_mu, hipStream_t s);
+               const double* lam, const double* gnode, const double* gsym, double* g_rho, double* g_lam,
+               double* g_mu, hipStream_t s);
 size_t dot_work_size();
 // tall-skinny C (m x K, ldc) = A^T B, A: n x m (lda), B: n x K (ldb), K in
 // {1, 2, 4, 8, 16}; work: gemm_tn_work_size doubles

@@ -1263,10 +1263,18 @@ bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStr
 //   d lambda / d rho_e = -lambda_j detJ sum_i u_i^T M u_i
 // u_i: component i of x at the element's nodes, D_i = sum_r invJ(i,r) Dr_r.
 // One thread per (element, eigenpair); the reference-element matrices sit in
-// LDS.  NP = 10 (P2) or 4 (P1).
+// LDS.  NP = 10 (P2) or 4 (P1).  With GRAV (JOB=2) the density derivative
+// also carries the gravity part of the stiffness, which is linear in the
+// nodal density: for a uniform change of rho over the element it adds
+//   detJ * sum_ij u_i^T OPrho_ij u_j
+//   = detJ * ( div^T M (g.u) - sum_ij gs_ij u_i^T M u_j
+//              - sum_a sum_ij g_i[a] (D_j u_i)[a] (M u_j)[a] )
+// (OPrho_ij of src/mod_cg_create_matrix.f90:1131-1180; div = sum_i D_i u_i,
+// (g.u)[a] = sum_i g_i[a] u_i[a], gs the symmetric least-squares gravity
+// gradient of the element; the two D^T G M terms of OPrho give equal forms).
 namespace nmg {
 
-template <int NP>
+template <int NP, bool GRAV>
 __global__ __launch_bounds__(kBlock) void adjoint_k(int64_t ne, int nev, const int32_t* __restrict__ elems,
                                                     const double* __restrict__ detJ,
                                                     const double* __restrict__ invJ,
@@ -1275,6 +1283,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_k(int64_t ne, int nev, const i
                                                     const double* __restrict__ X, int64_t ldx,
                                                     const double* __restrict__ d,
                                                     const double* __restrict__ lam,
+                                                    const double* __restrict__ gnode,
+                                                    const double* __restrict__ gsym,
                                                     double* __restrict__ g_rho, double* __restrict__ g_lam,
                                                     double* __restrict__ g_mu) {
   __shared__ double Ms[NP * NP];
@@ -1347,6 +1357,40 @@ __global__ __launch_bounds__(kBlock) void adjoint_k(int64_t ne, int nev, const i
 #pragma unroll
   for (int i = 0; i < 3; ++i) grho += quad(u[i], u[i]);
   grho *= -lam[j] * dJ;
+  if constexpr (GRAV) {
+    double Mu[3][NP], gu[NP];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int a = 0; a < NP; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) t += Ms[a * NP + b] * u[i][b];
+        Mu[i][a] = t;
+      }
+    double t4 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+      const double g0 = gnode[(e * NP + a) * 3 + 0], g1 = gnode[(e * NP + a) * 3 + 1],
+                   g2 = gnode[(e * NP + a) * 3 + 2];
+      gu[a] = g0 * u[0][a] + g1 * u[1][a] + g2 * u[2][a];
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+        t4 += (g0 * gr[0][jj][a] + g1 * gr[1][jj][a] + g2 * gr[2][jj][a]) * Mu[jj][a];
+    }
+    const double t1 = quad(dv, gu);
+    double t3 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) s += u[i][a] * Mu[jj][a];
+        t3 += gsym[e * 9 + i * 3 + jj] * s;
+      }
+    grho += dJ * (t1 - t3 - t4);
+  }
   const int64_t o = (int64_t)j * ne + e;
   g_rho[o] = grho;
   g_lam[o] = glam;
@@ -1355,18 +1399,27 @@ __global__ __launch_bounds__(kBlock) void adjoint_k(int64_t ne, int nev, const i
 
 void k_adjoint(int np, int64_t ne, int nev, const int32_t* elems, const double* detJ, const double* invJ,
                const double* refM, const double* refD, const double* X, int64_t ldx, const double* d,
-               const double* lam, double* g_rho, double* g_lam, double* g_mu, hipStream_t s) {
+               const double* lam, const double* gnode, const double* gsym, double* g_rho, double* g_lam,
+               double* g_mu, hipStream_t s) {
   const int64_t total = ne * (int64_t)nev;
   const int64_t grid = (total + kBlock - 1) / kBlock;
   if (grid <= 0) return;
-  if (np == 10)
-    hipLaunchKernelGGL(adjoint_k<10>, dim3((unsigned)grid), dim3(kBlock), 0, s, ne, nev, elems, detJ, invJ, refM,
-                       refD, X, ldx, d, lam, g_rho, g_lam, g_mu);
+  const bool grav = gnode != nullptr;
+  if (grav && !gsym) throw Error{NMG_EINVAL, "gravity without its element gradients"};
+#define NMG_ADJ(NP, G)                                                                                          \
+  hipLaunchKernelGGL((adjoint_k<NP, G>), dim3((unsigned)grid), dim3(kBlock), 0, s, ne, nev, elems, detJ, invJ, \
+                     refM, refD, X, ldx, d, lam, gnode, gsym, g_rho, g_lam, g_mu)
+  if (np == 10 && grav)
+    NMG_ADJ(10, true);
+  else if (np == 10)
+    NMG_ADJ(10, false);
+  else if (np == 4 && grav)
+    NMG_ADJ(4, true);
   else if (np == 4)
-    hipLaunchKernelGGL(adjoint_k<4>, dim3((unsigned)grid), dim3(kBlock), 0, s, ne, nev, elems, detJ, invJ, refM,
-                       refD, X, ldx, d, lam, g_rho, g_lam, g_mu);
+    NMG_ADJ(4, false);
   else
     throw Error{NMG_EINVAL, "adjoint supports P1 (4 nodes) and P2 (10 nodes) elements"};
+#undef NMG_ADJ
   NMG_HIP(hipGetLastError());
 }
 

@@ -212,6 +212,16 @@ module nmgpu_f90
       real(c_double), intent(in) :: detJ(*), invJ(*), refM(*), refD(*)
       real(c_double), intent(out) :: d_rho(*), d_lam(*), d_mu(*)
     end function
+    ! the same with reference gravity (JOB = 2): nodal g0 and element gradients
+    integer(c_int) function nmg_backward_grav(h, n_elem, np, elems, detJ, invJ, refM, refD, gnode, gsym, &
+        d_rho, d_lam, d_mu) bind(C, name="nmg_backward_grav")
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int64_t), value :: h, n_elem
+      integer(c_int), value :: np
+      integer(c_int32_t), intent(in) :: elems(*)
+      real(c_double), intent(in) :: detJ(*), invJ(*), refM(*), refD(*), gnode(*), gsym(*)
+      real(c_double), intent(out) :: d_rho(*), d_lam(*), d_mu(*)
+    end function
   end interface
 
 contains

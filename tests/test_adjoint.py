@@ -21,13 +21,16 @@ def solved(small_p2):
     return r, g, X
 
 
-def _reference_gradients(P, X, lam, sel):
+def _reference_gradients(P, X, lam, sel, g0=None):
+    """x^T (dK_e - lam dM_e) x from the producer's element matrices: with
+    lam = mu = 0 and rho = 1 the element stiffness is exactly the gravity
+    part of dK_e/drho_e (linear in rho), zero without gravity."""
     from normalmodes_amd.adjoint import element_matrices
     npl = P.mesh.elems.shape[1]
     one, zero = np.ones((sel.size, npl)), np.zeros((sel.size, npl))
     K_lam, _ = element_matrices(P.mesh, sel, one, zero, zero)
     K_mu, _ = element_matrices(P.mesh, sel, zero, one, zero)
-    _, M_rho = element_matrices(P.mesh, sel, zero, zero, one)
+    K_rho, M_rho = element_matrices(P.mesh, sel, zero, zero, one, None if g0 is None else g0[sel])
     out = np.zeros((3, lam.size, sel.size))
     for q, e in enumerate(sel):
         nodes = P.mesh.elems[e]
@@ -35,10 +38,24 @@ def _reference_gradients(P, X, lam, sel):
         for j in range(lam.size):
             xe = X[j, dofs]
             u = xe.reshape(3, npl)
-            out[0, j, q] = -lam[j] * sum(u[p] @ M_rho[q] @ u[p] for p in range(3))
+            out[0, j, q] = xe @ K_rho[q] @ xe - lam[j] * sum(u[p] @ M_rho[q] @ u[p] for p in range(3))
             out[1, j, q] = xe @ K_lam[q] @ xe
             out[2, j, q] = xe @ K_mu[q] @ xe
     return out
+
+
+GS = 0.5   # gravity of the self-gravitating unit ball: g0 = -GS x
+
+
+@pytest.fixture(scope="module")
+def grav_p2(small_p2):
+    """small_p2's mesh and material with reference gravity (JOB=2)."""
+    from normalmodes_amd.assemble import assemble
+    mesh = small_p2.mesh
+    g0 = -GS * mesh.nodes[mesh.elems]                      # (n_elem, np, 3)
+    rho, mu, lam = small_p2.rho, small_p2.mu, small_p2.lam
+    P = assemble(mesh, rho=rho, vp=np.sqrt((lam + 2 * mu) / rho), vs=np.sqrt(mu / rho), g0=g0, job=2)
+    return P, g0
 
 
 @pytest.mark.gpu
@@ -83,7 +100,29 @@ def test_reference_gradients_finite_differences(small_p2, dense_eigs):
     _check_fd(P, (R, g, None), dense_eigs)
 
 
-def _check_fd(small_p2, solved, dense_eigs):
+def test_reference_gradients_gravity_finite_differences(grav_p2):
+    """JOB=2: the gravity stiffness is linear in rho with g0 fixed.  The CPU
+    reference against central differences of the re-assembled pencil, and
+    the Euler identity sum_e (lam_e dl/dlam_e + mu_e dl/dmu_e + rho_e dl/drho_e)
+    = x^T A_el x + x^T A_g x - l x^T B x = 0."""
+    import scipy.linalg as sl
+    P, g0 = grav_p2
+    w, V = sl.eigh(P.A.to_scipy().toarray(), P.B.to_scipy().toarray())
+    sel = np.where((w > 1.0) & (w < 12.0))[0]
+    g = _reference_gradients(P, V[:, sel].T, w[sel], np.arange(P.mesh.n_elem), g0)
+    lam_e, mu_e, rho_e = P.lam.mean(axis=1), P.mu.mean(axis=1), P.rho.mean(axis=1)
+    assert np.max(np.abs(g[1] @ lam_e + g[2] @ mu_e + g[0] @ rho_e) / w[sel]) < 1e-11
+    # the gravity term is not negligible here: without it the density
+    # derivatives would sum to -lambda (toroidal modes, with no divergence
+    # and little radial motion, feel it least; spheroidal ones most)
+    assert np.max(np.abs(g[0] @ rho_e + w[sel]) / w[sel]) > 0.1
+
+    class R:
+        lam = w[sel]
+    _check_fd(P, (R, g, None), lambda Q: w, g0=g0)
+
+
+def _check_fd(small_p2, solved, dense_eigs, g0=None):
     import scipy.sparse.linalg as sla
     from normalmodes_amd.assemble import assemble
     r, g, X = solved
@@ -104,7 +143,7 @@ def _check_fd(small_p2, solved, dense_eigs):
     def eig_near(rho, lam, mu):
         vs = np.sqrt(mu / rho)
         vp = np.sqrt((lam + 2 * mu) / rho)
-        Q = assemble(P.mesh, rho=rho, vp=vp, vs=vs)
+        Q = assemble(P.mesh, rho=rho, vp=vp, vs=vs, g0=g0, job=1 if g0 is None else 2)
         ev = sla.eigsh(Q.A.to_scipy().tocsc(), k=4, M=Q.B.to_scipy().tocsc(), sigma=lamj * (1 - 1e-6),
                        which="LM", tol=1e-14, return_eigenvectors=False)
         return ev[np.argmin(np.abs(ev - lamj))]
@@ -119,6 +158,37 @@ def _check_fd(small_p2, solved, dense_eigs):
                 vals.append(eig_near(*arr))
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[k][j, e]) <= 1e-4 * abs(g[k][j, e]) + 1e-7 * lamj, (int(e), k, fd, g[k][j, e])
+
+
+@pytest.mark.gpu
+def test_adjoint_gravity(grav_p2):
+    """JOB=2 on the device (nmg_backward_grav) against the producer's element
+    matrices, the Euler identity and central differences."""
+    from normalmodes_amd.adjoint import backward
+    from normalmodes_amd.driver import apply_pevsl, setupmatvec
+    from normalmodes_amd import pevsl as pv
+    P, g0 = grav_p2
+    mv = setupmatvec(P, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(1.0, 12.0), keep=True)
+        try:
+            g = backward(r.handle, P.mesh, job=2, g0=g0)
+            with pytest.raises(pv.PevslError, match="nmg_backward_grav"):
+                backward(r.handle, P.mesh, job=3, g0=g0)   # nmg_backward takes job 1 only
+        finally:
+            pv.pevsl_finish(r.handle)
+        X = r.X * mv.d[None, :]
+    finally:
+        mv.free()
+    ref = _reference_gradients(P, X, r.lam, np.arange(P.mesh.n_elem), g0)
+    for k in range(3):
+        scale = np.abs(ref[k]).max()
+        assert np.max(np.abs(g[k] - ref[k])) < 1e-11 * scale, k
+    lam_e, mu_e, rho_e = P.lam.mean(axis=1), P.mu.mean(axis=1), P.rho.mean(axis=1)
+    assert np.max(np.abs(g[1] @ lam_e + g[2] @ mu_e + g[0] @ rho_e) / r.lam) < 1e-9
+    import scipy.linalg as sl
+    w = sl.eigh(P.A.to_scipy().toarray(), P.B.to_scipy().toarray(), eigvals_only=True)
+    _check_fd(P, (r, g, X), lambda Q: w, g0=g0)
 
 
 @pytest.mark.gpu

@@ -106,3 +106,48 @@ def test_const3k_demo_solve(tmp_path):
     for i in (0, ref.size - 1):
         x = read_mode(paths[i])
         assert abs(x @ (B @ x) - 1.0) < 1e-9
+
+
+@pytest.mark.gpu
+def test_const3k_job2_adjoint(const3k):
+    """The demo model with the reference's own gravity file (JOB=2, P1):
+    solve on the device, then d lambda / d(rho, lam, mu) per element through
+    nmg_backward_grav against the FE producer's element matrices and the
+    Euler identity sum_e (lam_e dl/dlam_e + mu_e dl/dmu_e + rho_e dl/drho_e) = 0
+    (elastic stiffness linear in (lam, mu), gravity stiffness and mass linear
+    in rho, gravity held fixed)."""
+    from normalmodes_amd.adjoint import backward, element_matrices
+    from normalmodes_amd.assemble import assemble
+    from normalmodes_amd.driver import apply_pevsl, freq_to_lambda, setupmatvec
+    from normalmodes_amd import pevsl as pv
+    M = const3k
+    g0 = M.g0 / 1e3
+    P = assemble(M.mesh, rho=M.rho, vp=M.vp, vs=M.vs, g0=g0, job=2)
+    mv = setupmatvec(P, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(freq_to_lambda(0.2), freq_to_lambda(2.0)), keep=True)
+        try:
+            g = backward(r.handle, P.mesh, job=2, g0=g0)
+        finally:
+            pv.pevsl_finish(r.handle)
+        X = r.X * mv.d[None, :]
+    finally:
+        mv.free()
+    assert r.lam.size > 0
+    ne, npl = P.mesh.n_elem, P.mesh.elems.shape[1]
+    sel = np.arange(ne)
+    one, zero = np.ones((ne, npl)), np.zeros((ne, npl))
+    K_lam, _ = element_matrices(P.mesh, sel, one, zero, zero)
+    K_mu, _ = element_matrices(P.mesh, sel, zero, one, zero)
+    K_rho, M_rho = element_matrices(P.mesh, sel, zero, zero, one, g0)
+    dofs = (3 * P.mesh.elems[:, None, :] + np.arange(3)[None, :, None]).reshape(ne, -1)
+    xe = X[:, dofs]                                              # (nev, ne, 3np)
+    u = xe.reshape(r.lam.size, ne, 3, npl)
+    ref = np.stack([
+        np.einsum("jea,eab,jeb->je", xe, K_rho, xe) - r.lam[:, None] * np.einsum("jepa,eab,jepb->je", u, M_rho, u),
+        np.einsum("jea,eab,jeb->je", xe, K_lam, xe),
+        np.einsum("jea,eab,jeb->je", xe, K_mu, xe)])
+    for k in range(3):
+        assert np.max(np.abs(g[k] - ref[k])) < 1e-10 * np.abs(ref[k]).max(), k
+    s = g[1] @ P.lam.mean(axis=1) + g[2] @ P.mu.mean(axis=1) + g[0] @ P.rho.mean(axis=1)
+    assert np.max(np.abs(s) / r.lam) < 1e-9

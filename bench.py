@@ -135,6 +135,20 @@ def solve_once(mv, interval, bdeg, precision, probe_every, block, split=0):
     return time.perf_counter() - t0, r
 
 
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
+
+
+def reorth_rate(n, m, k, reps=10):
+    """The block re-orthogonalisation product V^T W (V n x m, W n x k):
+    FP64 rate against the FP64 peak and its HBM rate against 8 TB/s."""
+    from normalmodes_amd import pevsl as pv
+    s, b = pv.bench_gemm_tn(n, m, k, reps)
+    flops = 2.0 * n * m * k
+    return dict(n=n, m=m, k=k, us=round(s * 1e6, 2), tflops=round(flops / s / 1e12, 2),
+                frac_fp64_peak=round(flops / s / 1e12 / FP64_PEAK_TFLOPS, 4),
+                gbs=round(b / s / 1e9, 1), frac_hbm=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
+
+
 def spmv_rates(name, reps=50):
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem
@@ -156,6 +170,7 @@ def spmv_rates(name, reps=50):
                                                              frac=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
     finally:
         mv.free()
+    out["reorth_gemm"] = reorth_rate(P.n, 400, 8)
     return out
 
 
@@ -313,6 +328,11 @@ def main():
             "filter_step": r.probe["filterstep"],
         },
     }
+    if args.block > 1:
+        try:
+            line["reorth_gemm"] = reorth_rate(P.n, r.lan_steps, args.block)
+        except Exception as e:
+            line["reorth_gemm"] = {"error": str(e)}
     if args.single_compare and args.block > 1:
         try:
             el1, r1 = solve_once(mv, interval, bdeg, precision, 0, 1)

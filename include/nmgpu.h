@@ -217,6 +217,12 @@ int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
  * accuracy). */
 int nmg_set_residual_target(nmg_handle h, double target);
 
+/* Replay every B-solve (all Chebyshev steps, and the split parts of a
+ * k-column solve with their fork and join) from a captured HIP graph, one
+ * executable graph per distinct set of buffers, so that a solve costs one
+ * host launch instead of one per step (no reference counterpart; on = 0 or
+ * 1, default 0). */
+int nmg_set_graphs(nmg_handle h, int on);
 /* Lanczos block width (no reference counterpart; pEVSL's ChebLanNr is the
  * block width 1, the default).  With block > 1, nmg_cheblannr runs the block
  * filtered Lanczos: the same filter, B-solve and extraction, applied to
@@ -254,10 +260,15 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double *X, double *Y);
  * library's own stream) and the algorithmic bytes of one product. */
 int nmg_bench_spmv(nmg_handle mat, int reps, int fp32, double *seconds, double *bytes);
 /* Same for one fused Chebyshev B-solve step (one product + vector update). */
+int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 /* The tall-skinny product of the block re-orthogonalisation on host
  * arrays (for checks): C (m x kc) = A^T B, A: n x m, B: n x kc, all
  * column-major. */
 int nmg_gemm_tn(int64_t n, int m, int kc, const double *A, const double *B, double *C);
+/* Timing of that product on device arrays (no reference counterpart): mean
+ * seconds per C = A^T B with A n x m and B n x kc filled on the device;
+ * bytes: its algorithmic HBM bytes 8 (n m + n kc + m kc). */
+int nmg_bench_gemm_tn(int64_t n, int m, int kc, int reps, double *seconds, double *bytes);
 /* A plain copy kernel of n doubles, `reps` times (calibration of HBM byte
  * counters: it moves 16 n bytes per launch); mean seconds per launch. */
 int nmg_bench_copy(int64_t n, int reps, double *seconds);
@@ -301,7 +312,6 @@ int nmg_tiling_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int m
  * cap: tile node capacity and threads: tile workgroup size (kind 2). */
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double *seconds, int *kind, int *rows,
                      int *cap, int *threads);
-int nmg_bench_chebstep(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -507,10 +507,30 @@ struct Solver {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DVec hb[12];
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
+  // B-solves replayed from captured HIP graphs (nmg_set_graphs): one
+  // executable graph per distinct set of buffers, built on its second use
+  bool graphs = false;
+  struct GraphKey {
+    const void* p[16];
+    int K, parts, fp32, deg;
+    double lmin, lmax;
+  };
+  struct GraphEntry {
+    GraphKey key;
+    int uses;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> gcache;
+  void drop_graphs() {
+    for (auto& e : gcache)
+      if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    gcache.clear();
+  }
 
   ~Solver() {
     free_basis();
     dfree(X);
+    drop_graphs();
     if (borrowed) return;
     for (auto x : sx)
       if (x) (void)hipStreamDestroy(x);
@@ -556,6 +576,76 @@ static void ensure_basis(Solver& S, int need, int keep, int maxcap) {
   S.V = V;
   S.Z = Z;
   S.cap = nc;
+}
+
+// Runs `body` (launches on S.s, possibly forked to S.sx and joined back) as
+// a captured HIP graph when graphs are on: the first use of a key runs
+// eagerly (so lazily created device data exists before any capture), the
+// second captures, instantiates and launches, later ones only launch.  The
+// body issues kernels and event waits only: no allocation, no memset, no
+// synchronisation.
+template <class F>
+static void run_graphed(Solver& S, const Solver::GraphKey& key, F&& body) {
+  if (!S.graphs) {
+    body();
+    return;
+  }
+  Solver::GraphEntry* hit = nullptr;
+  for (auto& e : S.gcache)
+    if (std::memcmp(&e.key, &key, sizeof(key)) == 0) {
+      hit = &e;
+      break;
+    }
+  if (hit && hit->exec) {
+    NMG_HIP(hipGraphLaunch(hit->exec, S.s));
+    return;
+  }
+  if (!hit) {
+    if (S.gcache.size() >= 16) {
+      if (S.gcache.front().exec) (void)hipGraphExecDestroy(S.gcache.front().exec);
+      S.gcache.erase(S.gcache.begin());
+    }
+    S.gcache.push_back(Solver::GraphEntry{key, 1, nullptr});
+    body();
+    return;
+  }
+  hipGraph_t g = nullptr;
+  NMG_HIP(hipStreamBeginCapture(S.s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    (void)hipStreamEndCapture(S.s, &junk);
+    if (junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  NMG_HIP(hipStreamEndCapture(S.s, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  NMG_HIP(rc);
+  hit->exec = ex;
+  hit->uses++;
+  NMG_HIP(hipGraphLaunch(ex, S.s));
+}
+
+static Solver::GraphKey graph_key(const Solver& S, const Cheb& C, int K, int parts, const void* b, const void* out,
+                                  const void* tmp) {
+  Solver::GraphKey k;
+  std::memset(&k, 0, sizeof(k));
+  k.p[0] = b;
+  k.p[1] = out;
+  k.p[2] = tmp;
+  k.p[3] = &C;
+  k.p[4] = C.mat;
+  for (int i = 0; i < 3 * parts && i < 11; ++i) k.p[5 + i] = S.hb[i].p;
+  k.K = K;
+  k.parts = parts;
+  k.fp32 = S.fp32() ? 1 : 0;
+  k.deg = C.deg;
+  k.lmin = C.lmin;
+  k.lmax = C.lmax;
+  return k;
 }
 
 // x ~ B~^{-1} b by the Chebyshev semi-iteration (x_0 = 0, deg products):
@@ -610,9 +700,21 @@ static void bsolve(Solver& S, const double* b, double* out, double* tmp) {
     return;
   }
   NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
-  cheb_solve(*S.bsol, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
-  S.bsol->ncalls++;
-  S.bsol->nmatvec += S.bsol->deg;
+  Cheb& C = *S.bsol;
+  if (S.graphs && !C.persistent(S.fp32())) {
+    // the whole solve is one graph; a sampled solve is bracketed around it
+    KernelProbe& pb = S.probe[0];
+    const bool pr = pb.armed();
+    if (pr) pb.before(S.s);
+    run_graphed(S, graph_key(S, C, 1, 0, b, out, tmp), [&] { cheb_solve(C, b, out, tmp, S.s, S.fp32(), nullptr); });
+    if (pr) pb.after(S.s);
+    pb.bytes = C.mat->matrix_bytes(S.fp32()) + 4.0 * 8.0 * (double)C.mat->nrow;
+    pb.launches = C.deg;
+  } else {
+    cheb_solve(C, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
+  }
+  C.ncalls++;
+  C.nmatvec += C.deg;
 }
 
 // y = p((A B^-1 - cc)/dd) z  (z-space; p((A - cc)/dd) z for a standard problem)
@@ -1071,16 +1173,25 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
 // y (interleaved n*K) ~ B~^{-1} b, all K columns at once
 static void block_bsolve(Solver& S, int K, const double* b, double* out, double* tmp) {
   NMG_CHECK(S.bsol != nullptr, NMG_ESTATE, "generalized problem without a B-solve");
-  const int P = S.split;
-  if (P > 1 && K % P == 0 && K / P >= 2 && !S.borrowed) {
+  Cheb& C = *S.bsol;
+  const int P = (S.split > 1 && K % S.split == 0 && K / S.split >= 2 && !S.borrowed) ? S.split : 1;
+  const int H = K / P;
+  if (P > 1) {
     // P solves of K/P columns at once on P streams: independent columns, the
     // same matrix read by all
     if (!S.sx[0]) {
       for (int q = 0; q < 3; ++q) NMG_HIP(hipStreamCreateWithFlags(&S.sx[q], hipStreamNonBlocking));
       for (int q = 0; q < 4; ++q) NMG_HIP(hipEventCreateWithFlags(&S.ev[q], hipEventDisableTiming));
     }
-    const int H = K / P;
     for (int i = 0; i < 3 * P; ++i) S.hb[i].ensure(S.n * H);
+  }
+  const bool graphed = S.graphs && !C.persistent(S.fp32());
+  KernelProbe* probe = &S.probe[0];
+  auto body = [&](KernelProbe* pb) {
+    if (P == 1) {
+      block_cheb(C, K, b, out, tmp, S.s, S.fp32(), pb);
+      return;
+    }
     double* in[4];
     double* res[4];
     for (int q = 0; q < P; ++q) {
@@ -1091,20 +1202,28 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
     NMG_HIP(hipEventRecord(S.ev[0], S.s));
     for (int q = 1; q < P; ++q) NMG_HIP(hipStreamWaitEvent(S.sx[q - 1], S.ev[0], 0));
     for (int q = 0; q < P; ++q)
-      block_cheb(*S.bsol, H, in[q], res[q], S.hb[3 * q + 2].p, q == 0 ? S.s : S.sx[q - 1], S.fp32(),
-                 q == 0 ? &S.probe[0] : nullptr);
+      block_cheb(C, H, in[q], res[q], S.hb[3 * q + 2].p, q == 0 ? S.s : S.sx[q - 1], S.fp32(),
+                 q == 0 ? pb : nullptr);
     for (int q = 1; q < P; ++q) {
       NMG_HIP(hipEventRecord(S.ev[q], S.sx[q - 1]));
       NMG_HIP(hipStreamWaitEvent(S.s, S.ev[q], 0));
     }
     k_merge_parts(S.n, K, P, res, out, S.s);
-    S.bsol->ncalls += K;
-    S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
-    return;
+  };
+  if (graphed) {
+    // the whole solve (all parts) is one graph; a sampled solve is
+    // bracketed around it on the solver's stream
+    const bool pr = probe->armed();
+    if (pr) probe->before(S.s);
+    run_graphed(S, graph_key(S, C, K, P, b, out, tmp), [&] { body(nullptr); });
+    if (pr) probe->after(S.s);
+    probe->bytes = C.mat->matrix_bytes(S.fp32()) + 4.0 * 8.0 * (double)C.mat->nrow * H;
+    probe->launches = C.deg;
+  } else {
+    body(probe);
   }
-  block_cheb(*S.bsol, K, b, out, tmp, S.s, S.fp32(), &S.probe[0]);
-  S.bsol->ncalls += K;
-  S.bsol->nmatvec += (int64_t)S.bsol->deg * K;
+  C.ncalls += K;
+  C.nmatvec += (int64_t)C.deg * K;
 }
 
 // A~ applied to K interleaved columns (K = 1: one vector): the sparse product,
@@ -2253,6 +2372,15 @@ int nmg_set_split_streams(nmg_handle h, int parts) {
   });
 }
 
+int nmg_set_graphs(nmg_handle h, int on) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(on == 0 || on == 1, NMG_EINVAL, "on must be 0 or 1");
+    S->drop_graphs();
+    S->graphs = on != 0;
+  });
+}
+
 int nmg_set_block_size(nmg_handle h, int block) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
@@ -2636,6 +2764,24 @@ int nmg_gemm_tn(int64_t n, int m, int kc, const double* A, const double* B, doub
     gemm_tn_any(S, n, m, kc, dA.p, n, dB.p, n, dC.p, m);
     NMG_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * m * kc, hipMemcpyDeviceToHost, S.s));
     NMG_HIP(hipStreamSynchronize(S.s));
+  });
+}
+
+int nmg_bench_gemm_tn(int64_t n, int m, int kc, int reps, double* seconds, double* bytes) {
+  return guarded([&] {
+    NMG_CHECK(n > 0 && m > 0 && kc > 0 && reps >= 1, NMG_EINVAL, "bad argument");
+    Solver S;
+    S.s = default_stream();
+    S.borrowed = true;
+    DVec dA, dB, dC;
+    dA.ensure(n * m);
+    dB.ensure(n * kc);
+    dC.ensure((int64_t)m * kc);
+    k_fill(n * m, 0.5, dA.p, S.s);
+    k_fill(n * kc, 0.25, dB.p, S.s);
+    const double sec = time_launches(S.s, reps, [&] { gemm_tn_any(S, n, m, kc, dA.p, n, dB.p, n, dC.p, m); });
+    if (seconds) *seconds = sec;
+    if (bytes) *bytes = 8.0 * ((double)n * m + (double)n * kc + (double)m * kc);
   });
 }
 

@@ -323,6 +323,11 @@ def slice_counts(h: Handle, n_slices: int) -> np.ndarray:
     return c
 
 
+def set_graphs(h: Handle, on: bool = True):
+    """Replay B-solves from captured HIP graphs (one host launch per solve)."""
+    _check(_lib().nmg_set_graphs(_i64(h), ctypes.c_int(int(bool(on)))))
+
+
 def set_split_streams(h: Handle, parts: int = 2):
     """k-column B-solves in `parts` (2 or 4) concurrent parts on as many streams; 0 turns it off."""
     _check(_lib().nmg_set_split_streams(_i64(h), ctypes.c_int(int(parts))))
@@ -411,6 +416,15 @@ def bench_copy(n: int, reps: int = 10) -> float:
     sec = _d()
     _check(_lib().nmg_bench_copy(_i64(n), ctypes.c_int(reps), ctypes.byref(sec)))
     return sec.value
+
+
+def bench_gemm_tn(n: int, m: int, kc: int, reps: int = 10):
+    """Mean seconds and algorithmic bytes of the re-orthogonalisation
+    product C = A^T B (A n x m, B n x kc) on device arrays."""
+    s, b = _d(), _d()
+    _check(_lib().nmg_bench_gemm_tn(_i64(n), ctypes.c_int(m), ctypes.c_int(kc), ctypes.c_int(reps),
+                                    ctypes.byref(s), ctypes.byref(b)))
+    return s.value, b.value
 
 
 def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):

@@ -521,3 +521,38 @@ def test_split_stream_bsolve_and_solve(small_p2, dense_eigs):
             assert np.all(r.res2 <= 1e-10)
     finally:
         mv.free()
+
+
+@pytest.mark.gpu
+def test_graph_replayed_bsolves(small_p2, dense_eigs):
+    """B-solves replayed from captured HIP graphs: four calls in a row on
+    different inputs (eager first use, capture on the second, replays after),
+    each against the same call without graphs, for one stream and for the
+    2-stream split; then whole solves with graphs, single-vector and block 8."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg="auto", tune=(1, 4, 8))
+    try:
+        rng = np.random.default_rng(43)
+        Xs = [rng.standard_normal((8, small_p2.n)) for _ in range(4)]
+        for parts in (0, 2):
+            eager = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, split=parts)
+            graph = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, split=parts, graphs=True)
+            try:
+                for X in Xs:
+                    ref = pv.apply_block(eager, 2, X)
+                    got = pv.apply_block(graph, 2, X)
+                    assert _rel(got, ref) < 1e-15, parts
+            finally:
+                pv.pevsl_finish(eager)
+                pv.pevsl_finish(graph)
+        w = dense_eigs(small_p2)
+        ref = w[(w >= 1.0) & (w <= 12.0)]
+        for block in (1, 8):
+            r0 = apply_pevsl(mv, interval=(1.0, 12.0), block=block, vectors=False)
+            r = apply_pevsl(mv, interval=(1.0, 12.0), block=block, vectors=False, graphs=True)
+            assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-12
+            assert np.max(np.abs(r.lam - r0.lam) / r0.lam) < 1e-14
+            assert np.all(r.res2 <= 1e-10)
+    finally:
+        mv.free()

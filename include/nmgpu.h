@@ -223,6 +223,10 @@ int nmg_set_residual_target(nmg_handle h, double target);
  * host launch instead of one per step (no reference counterpart; on = 0 or
  * 1, default 0). */
 int nmg_set_graphs(nmg_handle h, int on);
+/* Progress of long solves (no reference counterpart): a line on stderr
+ * every `every` Lanczos steps of nmg_lanbounds, nmg_cheblannr and
+ * nmg_solve_slices (0, the default: silent). */
+int nmg_set_verbose(nmg_handle h, int every);
 /* Lanczos block width (no reference counterpart; pEVSL's ChebLanNr is the
  * block width 1, the default).  With block > 1, nmg_cheblannr runs the block
  * filtered Lanczos: the same filter, B-solve and extraction, applied to

@@ -507,6 +507,7 @@ struct Solver {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DVec hb[12];
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
+  int verbose = 0;  // progress line on stderr every `verbose` Lanczos steps (0: silent)
   // B-solves replayed from captured HIP graphs (nmg_set_graphs): one
   // executable graph per distinct set of buffers, built on its second use
   bool graphs = false;
@@ -844,6 +845,13 @@ static double now() {
 }
 
 // ---------------------------------------------------------------------------
+// progress line of a long solve (nmg_set_verbose)
+static void progress(const Solver& S, const char* what, int step, double t0) {
+  if (S.verbose <= 0 || step % S.verbose != 0) return;
+  std::fprintf(stderr, "[nmg] %s: step %d, %.1f s\n", what, step, now() - t0);
+  std::fflush(stderr);
+}
+
 static void do_lanbounds(Solver& S, int mlan, int maxit, double tol, double* lmin, double* lmax) {
   NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
   const double t0 = now();
@@ -864,6 +872,7 @@ static void do_lanbounds(Solver& S, int mlan, int maxit, double tol, double* lmi
     hw.ensure((int64_t)gemv_work_size(S.n, k + 2) + k + 4);
     lanczos_step(S, k, 0, nullptr, hw.p);
     const int kd = k + 1;
+    progress(S, "lanbounds", kd, t0);
     if (kd % 10 == 0 || kd == m) {
       fetch_tridiag(S, kd, al, be);
       if (tridiag_eigvals(kd, al.data(), be.data(), w) != NMG_OK)
@@ -1112,6 +1121,7 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
     hw.ensure((int64_t)gemv_work_size(n, k + 2) + k + 4);
     lanczos_step(S, k, 1, &P, hw.p);
     kdim = k + 1;
+    progress(S, "cheblannr", kdim, t0);
     stop = lanczos_test(S, kdim, P, xintv, tol, L);
   }
   S.lan_steps = kdim;
@@ -1425,6 +1435,7 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
   int nblk = 0;
   bool converged = false, stopped_with_result = false;
   for (int j = 0; j < maxblk && !converged; ++j) {
+    progress(S, "block cheblannr", j * K, t0);
     ensure_basis(S, (j + 2) * K, (j + 1) * K, maxcols);
     double* Vj = S.V + (size_t)j * K * n;
     double* Zj = S.Z + (size_t)j * K * n;
@@ -1659,6 +1670,7 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
     int active = 0;
     for (int i = 0; i < ns; ++i) active += stop[i] == 0;
     if (active == 0) break;
+    progress(S, "slices", k, t0);
     for (int i = 0; i < ns; ++i)
       if (stop[i] == 0) ensure_basis(*sub[i], k + 2, k + 1, maxit + 1);
     // z_{k+1} = p_s(A~ B~^-1) z_k for every slice, one K-column filter
@@ -2369,6 +2381,14 @@ int nmg_set_split_streams(nmg_handle h, int parts) {
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(parts >= 0 && parts <= 4 && parts != 3, NMG_EINVAL, "parts must be 0, 1, 2 or 4");
     S->split = parts;
+  });
+}
+
+int nmg_set_verbose(nmg_handle h, int every) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(every >= 0, NMG_EINVAL, "every must be >= 0");
+    S->verbose = every;
   });
 }
 

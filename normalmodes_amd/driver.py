@@ -261,7 +261,7 @@ def split_parts(block: int, split="auto") -> int:
 
 
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split="auto", graphs=False):
+                 split="auto", graphs=False, verbose=0):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -286,6 +286,8 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
             pv.set_split_streams(h, parts)
         if graphs:
             pv.set_graphs(h, True)
+        if verbose:
+            pv.set_verbose(h, verbose)
         if resid_target is None and precision == pv.FP32_FILTER:
             # the FP32 B-solve bounds ||A~x - lam B~x||/|lam| near 1e-6 (its
             # eigenvalues still agree with FP64 to ~1e-14): stop at that floor
@@ -303,7 +305,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
-                keep: bool = False, split="auto", graphs: bool = False) -> EigResult:
+                keep: bool = False, split="auto", graphs: bool = False, verbose: int = 0) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
@@ -312,7 +314,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
     t = {}
     t0 = time.perf_counter()
     h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every, block=block,
-                     resid_target=resid_target, split=split, graphs=graphs)
+                     resid_target=resid_target, split=split, graphs=graphs, verbose=verbose)
     try:
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -377,7 +379,8 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
                  kpm_nvec: int = 16, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
                  lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7, rtol: float = 1e-9,
                  vectors: bool = True, residual_check: bool = True, probe_every: int = 0,
-                 resid_target: float | None = None, keep: bool = False) -> EigResult:
+                 resid_target: float | None = None, keep: bool = False, split="auto",
+                 verbose: int = 0) -> EigResult:
     """The interval split into n_slices spectrum slices, solved in lockstep.
 
     Slice boundaries: ``kpm`` places them at equal counts of the kernel
@@ -388,8 +391,9 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
     a, b = interval
     t = {}
     t0 = time.perf_counter()
+    width = 1 << max(0, (n_slices - 1).bit_length())   # the k-column width the slices run at
     h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every,
-                     resid_target=resid_target)
+                     resid_target=resid_target, split=split_parts(width, split), verbose=verbose)
     try:
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()

@@ -323,6 +323,11 @@ def slice_counts(h: Handle, n_slices: int) -> np.ndarray:
     return c
 
 
+def set_verbose(h: Handle, every: int):
+    """Progress line on stderr every `every` Lanczos steps (0: silent)."""
+    _check(_lib().nmg_set_verbose(_i64(h), ctypes.c_int(int(every))))
+
+
 def set_graphs(h: Handle, on: bool = True):
     """Replay B-solves from captured HIP graphs (one host launch per solve)."""
     _check(_lib().nmg_set_graphs(_i64(h), ctypes.c_int(int(bool(on)))))

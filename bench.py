@@ -77,8 +77,9 @@ def upload_with_layouts(P, layouts, tune):
 
     ``layouts``: 'auto', or comma-separated '<A|B><K>:<kind>:<params>' with
     kind 'lanes:<g>' (K = 1, g lanes per node row), 'rows:<r>' (r node rows
-    per wavefront) or 'tiled:<cap>:<r>[:<threads>]' (LDS tiles of at most cap
-    nodes, workgroups of 1024 or 256 threads)."""
+    per wavefront), 'tiled:<cap>:<r>[:<threads>]' (LDS tiles of at most cap
+    nodes, workgroups of 1024 or 256 threads) or 'groups:<g>' (K = 1, row
+    groups, g per wavefront)."""
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
@@ -95,6 +96,8 @@ def upload_with_layouts(P, layouts, tune):
                 pv.set_tiling(mat, 1, 0, 1)   # route single products through the row-block kernels
         elif f[1] == "tiled":
             pv.set_tiling(mat, K, int(f[2]), int(f[3]), int(f[4]) if len(f) > 4 else 1024)
+        elif f[1] == "groups":
+            pv.set_row_groups(mat, int(f[2]))
         else:
             raise ValueError(part)
         mv.tuned[f[0]] = {"fixed": ":".join(f[1:])}

@@ -298,6 +298,14 @@ int nmg_set_value_order(nmg_handle mat, int tiled);
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */
 int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave);
+/* Single-column products of a blocked matrix (BSR3 / MASS3) through row
+ * groups (no reference counterpart): consecutive node rows holding at most
+ * 64 entries (or one longer row) per group, one entry per lane, per_wave
+ * groups per wavefront with the next group's loads issued while the
+ * current one is reduced (per_wave = 1, 2, 4 or 8; 0 drops the groups).
+ * Takes precedence over the other single-column layouts; the autotuner
+ * (kind 3) sets it when it is the fastest. */
+int nmg_set_row_groups(nmg_handle mat, int per_wave);
 /* LDS-tiled K-column products: rows cut into tiles whose referenced nodes
  * (at most max_nodes, <= 64 KB of x in LDS) are staged once per tile by a
  * workgroup of `threads` (256 or 1024; 0 = 1024); max_nodes = 0 removes the
@@ -311,8 +319,9 @@ int nmg_tiling_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int m
 /* Autotuner (no reference counterpart): times the product layouts of mat for
  * K columns on the device (lanes per row; rows per wavefront; LDS tiles with
  * capacities on a geometric ladder up to 64 KB, workgroups of 1024 or 256
- * threads) and keeps the fastest.  kind: 0 lanes-per-row, 1 rows per
- * wavefront, 2 tiled; rows: lanes per row (kind 0) or rows per wavefront;
+ * threads; for K = 1 also row groups) and keeps the fastest.  kind: 0
+ * lanes-per-row, 1 rows per wavefront, 2 tiled, 3 row groups; rows: lanes
+ * per row (kind 0), rows per wavefront (1, 2) or groups per wavefront (3);
  * cap: tile node capacity and threads: tile workgroup size (kind 2). */
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double *seconds, int *kind, int *rows,
                      int *cap, int *threads);

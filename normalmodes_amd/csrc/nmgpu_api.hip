@@ -116,11 +116,18 @@ Tiling::~Tiling() {
   dfree(lcol);
 }
 
+RowGroups::~RowGroups() {
+  dfree(grow);
+  dfree(gk);
+}
+
 void DevMat::release() {
   for (auto& t : tiling) {
     delete t;
     t = nullptr;
   }
+  delete rgroups;
+  rgroups = nullptr;
   dfree(rowptr);
   dfree(col);
   dfree(val);
@@ -294,6 +301,38 @@ static void compute_tiling(int64_t nb, const int64_t* rp, const int32_t* ci, int
   if (row > start) close_tile();
 }
 
+void build_row_groups(DevMat& M, int per_wave) {
+  delete M.rgroups;
+  M.rgroups = nullptr;
+  if (per_wave <= 0 || M.layout == L_CSR) return;
+  NMG_CHECK(per_wave == 1 || per_wave == 2 || per_wave == 4 || per_wave == 8, NMG_EINVAL,
+            "groups per wavefront must be 1, 2, 4 or 8");
+  NMG_CHECK((int64_t)M.h_browptr.size() == M.nb + 1, NMG_ESTATE, "no host pattern for row groups");
+  const int64_t* rp = M.h_browptr.data();
+  std::vector<int64_t> gr, gk;
+  int64_t r = 0;
+  while (r < M.nb) {
+    gr.push_back(r);
+    gk.push_back(rp[r]);
+    if (rp[r + 1] - rp[r] > 64) {  // a long row is a group of its own
+      ++r;
+      continue;
+    }
+    const int64_t start = r;
+    while (r < M.nb && r - start < 64 && rp[r + 1] - rp[start] <= 64) ++r;
+  }
+  gr.push_back(M.nb);
+  gk.push_back(rp[M.nb]);
+  std::unique_ptr<RowGroups> G(new RowGroups());
+  G->ngroups = (int64_t)gr.size() - 1;
+  G->per_wave = per_wave;
+  G->grow = dmalloc<int64_t>(gr.size());
+  G->gk = dmalloc<int64_t>(gk.size());
+  NMG_HIP(hipMemcpy(G->grow, gr.data(), sizeof(int64_t) * gr.size(), hipMemcpyHostToDevice));
+  NMG_HIP(hipMemcpy(G->gk, gk.data(), sizeof(int64_t) * gk.size(), hipMemcpyHostToDevice));
+  M.rgroups = G.release();
+}
+
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block) {
   int idx = -1;
   for (int i = 0; i < 5; ++i)
@@ -357,6 +396,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->tiled_vals = S.tiled_vals;
   for (int i = 0; i < 5; ++i)
     if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave, S.tiling[i]->block);
+  if (S.rgroups) build_row_groups(*M, S.rgroups->per_wave);
   return M.release();
 }
 
@@ -2546,6 +2586,7 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
     NMG_CHECK(group == 0 || group == 8 || group == 16 || group == 32 || group == 64, NMG_EINVAL,
               "group must be 0, 8, 16, 32 or 64");
     M->group = group;
+    if (group > 0 || K == 1) build_row_groups(*M, 0);  // the single-column route is set explicitly
     if (K > 0) {
       int idx = -1;
       for (int i = 0; i < 5; ++i)
@@ -2588,7 +2629,7 @@ static double time_launches(hipStream_t s, int reps, F&& f) {
 
 struct TuneResult {
   double seconds = 1e30;
-  int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block
+  int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block, 3 row groups (K = 1)
   int group = 0, rows = 1, cap = 0;
   int64_t ntiles = 0;
   int block = 1024;  // tile workgroup threads
@@ -2610,8 +2651,13 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   // untiled layouts (the single-column route is K = 1's choice alone: tuning
   // another width leaves it as it is)
   build_tiling(M, K, 0, 1);
-  if (K == 1) M.single_blocked = false;
   if (K == 1) {
+    M.single_blocked = false;
+    for (int pw : {1, 2, 4, 8}) {
+      build_row_groups(M, pw);
+      consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 3, 0, pw, 0, 0);
+    }
+    build_row_groups(M, 0);
     for (int g : {8, 16, 32, 64}) {
       M.group = g;
       consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 0, g, 1, 0, 0);
@@ -2648,6 +2694,8 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   M.block_map[idx] = 0;
   if (best.kind == 0) {
     M.group = best.group;
+  } else if (best.kind == 3) {
+    build_row_groups(M, best.rows);
   } else if (best.kind == 1) {
     M.block_map[idx] = best.rows;
     if (K == 1) M.single_blocked = true;
@@ -2738,7 +2786,20 @@ int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int 
               "bad rows per wavefront");
     NMG_CHECK(max_nodes * 24 * K <= 64 * 1024, NMG_EINVAL, "tile exceeds 64 KB of LDS");
     build_tiling(*M, K, max_nodes, rows_per_wave, threads > 0 ? threads : 1024);
-    if (K == 1) M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
+    if (K == 1) {
+      M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
+      build_row_groups(*M, 0);
+    }
+  });
+}
+
+int nmg_set_row_groups(nmg_handle mat, int per_wave) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(per_wave == 0 || per_wave == 1 || per_wave == 2 || per_wave == 4 || per_wave == 8, NMG_EINVAL,
+              "groups per wavefront must be 0, 1, 2, 4 or 8");
+    NMG_CHECK(per_wave == 0 || M->layout != L_CSR, NMG_EINVAL, "row groups apply to the blocked layouts");
+    build_row_groups(*M, per_wave);
   });
 }
 

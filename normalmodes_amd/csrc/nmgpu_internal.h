@@ -53,6 +53,17 @@ struct Tiling {
   ~Tiling();
 };
 
+// Row groups of the single-column products (BSR3 / MASS3): group g owns node
+// rows [grow[g], grow[g+1]) and entries [gk[g], gk[g+1]); a group holds at
+// most 64 entries or is one longer row.  per_wave groups per wavefront.
+struct RowGroups {
+  int64_t ngroups = 0;
+  int per_wave = 8;
+  int64_t* grow = nullptr;
+  int64_t* gk = nullptr;
+  ~RowGroups();
+};
+
 struct DevMat {
   int64_t nrow = 0, ncol = 0, nnz = 0;  // scalar (DoF-level) sizes
   int layout = L_CSR;
@@ -75,6 +86,7 @@ struct DevMat {
   int block_map[5] = {0, 0, 0, 0, 0};
   Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
+  RowGroups* rgroups = nullptr; // single-column products through the row-group kernels (first choice)
   int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)
   int nt = 0;                   // non-temporal matrix loads in the row kernels
   // BSR3 value order: 0 planes (value q of entry k at q*nbnz + k); 1 tiles of
@@ -124,6 +136,8 @@ inline int kflags(const DevMat& M) { return (M.xcd ? 1 : 0) | (M.nt ? 2 : 0) | (
 // or back to planes (0)
 void set_value_order(DevMat& M, int tiled, hipStream_t s);
 
+// builds (or, with per_wave <= 0, drops) the row groups of M (blocked layouts)
+void build_row_groups(DevMat& M, int per_wave);
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024);
 // creates a DevMat from host CSR (0-based); picks the layout

@@ -189,6 +189,169 @@ __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* 
   if (wr) epi.store(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2), pre);
 }
 
+// --------------------------------------------------------------------------
+// Row-group kernels (single column, BSR3 and MASS3).  The node rows are cut
+// on the host into groups of consecutive rows holding at most 64 entries
+// (or one longer row); group g owns rows [grow[g], grow[g+1]) and entries
+// [gk[g], gk[g+1]).  A wavefront takes NG consecutive groups: lane l holds
+// entry gk + l of the current group, so every lane carries an entry whatever
+// the row lengths, and the entries' columns and values need no row offset
+// first (only the group descriptor, loaded for all NG groups at once).  The
+// loads of group i+1 (columns, values, row starts, epilogue operands) are
+// issued right after the x gathers of group i, so they are in flight while
+// group i is multiplied, reduced and stored.  Lanes find their row by a
+// binary search over the group's row starts held one per lane, sum their
+// products per row by a segmented butterfly, and lane t < 3 * rows writes
+// DoF 3 * grow + t.  Longer rows (one per group) loop over the rest.
+__device__ __forceinline__ int64_t rl64(int64_t v, int i) {
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), i);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), i);
+  return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+template <bool BSR, class VT, class Epi>
+struct GroupStage {
+  int c;                   // column node of this lane's entry (0 when none)
+  double v[BSR ? 9 : 1];   // its value(s) (0 when none)
+  int b;                   // lane j < rows: start of row j within the group
+  typename Epi::Pre pre;   // lane t < 3 * rows: epilogue operands of DoF 3 * r0 + t
+};
+
+template <bool BSR, int NG, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void group_rows(int64_t ngroups, const int64_t* __restrict__ grow,
+                                                     const int64_t* __restrict__ gk,
+                                                     const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ci,
+                                                     const VT* __restrict__ val, int64_t nbnz,
+                                                     const double* __restrict__ x, Epi epi, int flags) {
+  constexpr int NV = BSR ? 9 : 1;
+  const int lane = threadIdx.x & 63;
+  const int64_t g0 = (xcd_block(flags) * (kBlock / 64) + (threadIdx.x >> 6)) * NG;
+  if (g0 >= ngroups) return;
+  const int ng = (int)((ngroups - g0) < NG ? (ngroups - g0) : NG);
+  // descriptors of groups g0 .. g0 + ng (the last one closes group ng - 1)
+  int64_t dr = 0, dk = 0;
+  if (lane <= ng) {
+    dr = grow[g0 + lane];
+    dk = gk[g0 + lane];
+  }
+  using Stage = GroupStage<BSR, VT, Epi>;
+  auto load = [&](int i, Stage& st) {
+    const int64_t r0 = rl64(dr, i), r1 = rl64(dr, i + 1);
+    const int64_t k0 = rl64(dk, i), k1 = rl64(dk, i + 1);
+    const int nrows = (int)(r1 - r0);
+    const int64_t k = k0 + lane;
+    const bool ok = k < k1;
+    st.c = ok ? (int)ldm(ci + k, flags) : 0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      st.v[q] = ok ? (double)ldm(val + (BSR ? bidx(k, q, nbnz, flags) : k), flags) : 0.0;
+    st.b = lane < nrows ? (int)(rp[r0 + lane] - k0) : 0x7fffffff;
+    if (lane < 3 * nrows) st.pre = epi.load(3 * r0 + lane);
+  };
+  Stage cur, nxt;
+  load(0, cur);
+  for (int i = 0; i < ng; ++i) {
+    const int64_t r0 = rl64(dr, i), r1 = rl64(dr, i + 1);
+    const int64_t k0 = rl64(dk, i), k1 = rl64(dk, i + 1);
+    const int nrows = (int)(r1 - r0);
+    const int len = (int)(k1 - k0);
+    // gathers of this group, then the next group's loads
+    const int64_t c3 = 3 * (int64_t)cur.c;
+    const double xa = x[c3], xb = x[c3 + 1], xc = x[c3 + 2];
+    if (i + 1 < ng) load(i + 1, nxt);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (lane < len) {
+      if (BSR) {
+        y0 = cur.v[0] * xa + cur.v[1 % NV] * xb + cur.v[2 % NV] * xc;
+        y1 = cur.v[3 % NV] * xa + cur.v[4 % NV] * xb + cur.v[5 % NV] * xc;
+        y2 = cur.v[6 % NV] * xa + cur.v[7 % NV] * xb + cur.v[8 % NV] * xc;
+      } else {
+        y0 = cur.v[0] * xa;
+        y1 = cur.v[0] * xb;
+        y2 = cur.v[0] * xc;
+      }
+    }
+    if (len > 64) {
+      // one long row: the remaining entries, 64 at a time
+      for (int64_t k = k0 + 64 + lane; k < k1; k += 64) {
+        const int64_t d3 = 3 * (int64_t)ldm(ci + k, flags);
+        double v[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = (double)ldm(val + (BSR ? bidx(k, q, nbnz, flags) : k), flags);
+        const double ua = x[d3], ub = x[d3 + 1], uc = x[d3 + 2];
+        if (BSR) {
+          y0 += v[0] * ua + v[1 % NV] * ub + v[2 % NV] * uc;
+          y1 += v[3 % NV] * ua + v[4 % NV] * ub + v[5 % NV] * uc;
+          y2 += v[6 % NV] * ua + v[7 % NV] * ub + v[8 % NV] * uc;
+        } else {
+          y0 += v[0] * ua;
+          y1 += v[0] * ub;
+          y2 += v[0] * uc;
+        }
+      }
+    }
+    // row of this lane's entry: the largest j < nrows with b_j <= lane
+    int rid = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int cand = rid + step;
+      const int bc = __shfl(cur.b, cand & 63, 64);
+      if (cand < nrows && bc <= lane) rid = cand;
+    }
+    // segmented sums: afterwards the first lane of each row holds the row's sum
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int orid = __shfl_down(rid, d, 64);
+      const double o0 = __shfl_down(y0, d, 64), o1 = __shfl_down(y1, d, 64), o2 = __shfl_down(y2, d, 64);
+      if (lane + d < 64 && orid == rid) {
+        y0 += o0;
+        y1 += o1;
+        y2 += o2;
+      }
+    }
+    // DoF 3 * r0 + t of row j = t / 3, component t % 3
+    for (int t0 = 0; t0 < 3 * nrows; t0 += 64) {
+      const int t = t0 + lane;
+      const int j = t / 3 < 64 ? t / 3 : 63;
+      const int bj = __shfl(cur.b, j, 64);
+      const int bn = __shfl(cur.b, (j + 1) & 63, 64);
+      const int end = (j + 1 < nrows) ? bn : len;
+      const int src = bj & 63;
+      const double s0 = __shfl(y0, src, 64), s1 = __shfl(y1, src, 64), s2 = __shfl(y2, src, 64);
+      if (t < 3 * nrows) {
+        const int p = t % 3;
+        const double sum = (end > bj) ? (p == 0 ? s0 : (p == 1 ? s1 : s2)) : 0.0;
+        if (t0 == 0) epi.store(3 * r0 + t, sum, cur.pre);
+        else epi.store(3 * r0 + t, sum, epi.load(3 * r0 + t));
+      }
+    }
+    cur = nxt;
+  }
+}
+
+template <bool BSR, int NG, class VT, class Epi>
+static void launch_groups_NG(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  const RowGroups& G = *M.rgroups;
+  const int64_t waves = (G.ngroups + NG - 1) / NG;
+  const int64_t grid = (waves + kBlock / 64 - 1) / (kBlock / 64);
+  if (grid <= 0) return;
+  hipLaunchKernelGGL((group_rows<BSR, NG, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, G.ngroups, G.grow,
+                     G.gk, M.browptr, M.bcol, v, M.nbnz, x, epi, kflags(M));
+}
+
+template <class VT, class Epi>
+static void launch_groups(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  const bool bsr = M.layout == L_BSR3;
+  switch (M.rgroups->per_wave) {
+    case 1: bsr ? launch_groups_NG<true, 1>(M, v, x, epi, s) : launch_groups_NG<false, 1>(M, v, x, epi, s); break;
+    case 2: bsr ? launch_groups_NG<true, 2>(M, v, x, epi, s) : launch_groups_NG<false, 2>(M, v, x, epi, s); break;
+    case 4: bsr ? launch_groups_NG<true, 4>(M, v, x, epi, s) : launch_groups_NG<false, 4>(M, v, x, epi, s); break;
+    default: bsr ? launch_groups_NG<true, 8>(M, v, x, epi, s) : launch_groups_NG<false, 8>(M, v, x, epi, s); break;
+  }
+  NMG_HIP(hipGetLastError());
+}
+
 static int pick_group(double avg) {
   if (avg > 40.0) return 64;
   if (avg > 20.0) return 32;
@@ -233,6 +396,11 @@ static void launch_rows(const DevMat& M, const VT* v, const double* x, Epi epi, 
 
 template <class Epi>
 static void dispatch(const DevMat& M, const double* x, Epi epi, hipStream_t s, bool fp32) {
+  if (M.rgroups && M.layout != L_CSR) {
+    if (fp32 && M.bval32) launch_groups(M, M.bval32, x, epi, s);
+    else launch_groups(M, M.bval, x, epi, s);
+    return;
+  }
   if (M.layout == L_CSR) {
     if (fp32 && M.val32) launch_rows(M, M.val32, x, epi, s);
     else launch_rows(M, M.val, x, epi, s);
@@ -243,13 +411,13 @@ static void dispatch(const DevMat& M, const double* x, Epi epi, hipStream_t s, b
 }
 
 void k_spmv(const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32) {
-  if (M.single_blocked) return k_block_spmv(1, M, x, y, s, fp32);
+  if (M.single_blocked && !M.rgroups) return k_block_spmv(1, M, x, y, s, fp32);
   dispatch(M, x, EpiStore{y}, s, fp32);
 }
 
 void k_cheb_step(const DevMat& M, const double* b, const double* xk, double* xm, double w,
                  double ith, hipStream_t s, bool fp32) {
-  if (M.single_blocked) return k_block_cheb_step(1, M, b, xk, xm, w, ith, s, fp32);
+  if (M.single_blocked && !M.rgroups) return k_block_cheb_step(1, M, b, xk, xm, w, ith, s, fp32);
   dispatch(M, xk, EpiCheb{b, xk, xm, w, ith}, s, fp32);
 }
 

@@ -556,3 +556,80 @@ def test_graph_replayed_bsolves(small_p2, dense_eigs):
             assert np.all(r.res2 <= 1e-10)
     finally:
         mv.free()
+
+
+def _ragged_node_pattern(nb, seed):
+    """Node pattern with empty rows, short rows and rows longer than 64 and
+    128 entries (sorted unique columns), as a scipy CSR of ones."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, 40, nb)
+    lens[rng.choice(nb, 8, replace=False)] = 0
+    lens[rng.choice(nb, 4, replace=False)] = [70, 130, 200, 65]
+    rows, cols = [], []
+    for r, L in enumerate(lens):
+        c = np.sort(rng.choice(nb, size=min(int(L), nb), replace=False))
+        rows.extend([r] * c.size)
+        cols.extend(c.tolist())
+    return sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(nb, nb))
+
+
+def test_row_group_products(small_p2, dense_eigs):
+    """Single-column products through row groups (1, 2 and 8 groups per
+    wavefront) for the stiffness (3x3 blocks) and mass (scalar x I3) of a P2
+    mesh and for ragged synthetic patterns with empty rows and rows of 65 to
+    200 entries, against the CPU product; Chebyshev B-solve and a whole solve
+    with both matrices on row groups."""
+    import scipy.sparse as sp
+    pv = _pv()
+    rng = np.random.default_rng(17)
+    S = _ragged_node_pattern(300, 5)
+    Sb = S.tocoo()
+    blocks = rng.standard_normal((Sb.nnz, 3, 3))
+    A_r = sp.bsr_matrix((blocks, Sb.col, S.indptr), shape=(900, 900)).tocsr()
+    A_r.sort_indices()
+    M_r = sp.kron(S.multiply(rng.uniform(0.5, 1.5, S.shape)).tocsr(), sp.eye(3), format="csr")
+    M_r.sort_indices()
+    mats = [(small_p2.A.to_scipy(), "bsr3"), (small_p2.B.to_scipy(), "mass3"), (A_r, "bsr3"), (M_r, "mass3")]
+    for M, layout in mats:
+        h = pv.csr_create(M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data)
+        try:
+            assert pv.csr_info(h)["layout"] == layout
+            x = rng.standard_normal(M.shape[0])
+            ref = M @ x
+            scale = np.abs(M) @ np.abs(x) + 1e-300
+            for pw in (1, 2, 8):
+                pv.set_row_groups(h, pw)
+                y = pv.parcsr_matvec(x, h)
+                assert np.max(np.abs(y - ref) / scale) < 1e-14, (layout, pw)
+            pv.set_row_groups(h, 0)
+            assert np.max(np.abs(pv.parcsr_matvec(x, h) - ref) / scale) < 1e-14
+        finally:
+            pv.csr_free(h)
+    # Chebyshev B-solve and a whole solve on row groups
+    B = small_p2.B
+    d_ref, val_ref = orc.diagscale(B, 1.0)
+    h = pv.csr_create(B.rowptr, B.col, B.val)
+    try:
+        pv.csr_diagscale(h, 1.0)
+        pv.set_row_groups(h, 4)
+        b = rng.standard_normal(B.n_rows)
+        c = pv.pevsl_setup_chebiter(0.28, 4.4, 45, h)
+        x = pv.pevsl_chebiter(2, b, c)
+        ref = orc.chebiter(orc.CSRView(B.rowptr, B.col, val_ref), 0.28, 4.4, 45, b)
+        assert _rel(x, ref) < 1e-13
+        pv.pevsl_chebiter_free(c)
+    finally:
+        pv.csr_free(h)
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    mv = setupmatvec(small_p2, bdeg="auto", tune=())
+    try:
+        for mat in (mv.matA, mv.matB):
+            pv.set_row_groups(mat, 2)
+        w = dense_eigs(small_p2)
+        ref = w[(w >= 1.0) & (w <= 12.0)]
+        r = apply_pevsl(mv, interval=(1.0, 12.0), vectors=False)
+        assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-12
+        assert np.all(r.res2 <= 1e-10)
+    finally:
+        mv.free()

@@ -26,7 +26,8 @@ Also reported on the same JSON line:
 
 ``--kernel-only`` runs no solve: a copy kernel of known size, then the
 dominant kernel alone (one B-solve launch per repetition), for hardware
-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE).
+counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE).  ``--spmv-only``
+does the same for the sparse products at the ``--spmv-workload`` size.
 """
 from __future__ import annotations
 
@@ -231,6 +232,8 @@ def main():
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="copy kernel + the --spmv-workload product rates only, --steps launches each (counter runs)")
     ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint of the solved pairs (0 skips)")
     ap.add_argument("--layouts", default="auto",
                     help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
@@ -246,6 +249,14 @@ def main():
     if args.kernel_only:
         args.bdeg = bdeg
         kernel_only(args)
+        return
+    if args.spmv_only:
+        # copy calibration, then the sparse-product rates at --spmv-workload
+        from normalmodes_amd import pevsl as pv
+        t_copy = pv.bench_copy(1 << 24, 5)
+        print(json.dumps({"copy": {"doubles": 1 << 24, "bytes_per_launch": 16 << 24, "launches": 6,
+                                   "us": round(t_copy * 1e6, 2)},
+                          "spmv_hbm": spmv_rates(args.spmv_workload, reps=args.steps)}), flush=True)
         return
 
     from normalmodes_amd import driver, pevsl as pv

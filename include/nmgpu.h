@@ -37,7 +37,7 @@ enum {
 };
 
 /* precision of the Chebyshev filter (nmg_set_precision) */
-enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1 };
+enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1, NMG_FP32_BSOLVE = 2 };
 
 const char *nmg_last_error(void);
 int nmg_version(int *major, int *minor);
@@ -99,8 +99,11 @@ int nmg_set_geneig(nmg_handle h);
 /* Filter precision (no reference counterpart: pEVSL is FP64 only).
  * NMG_FP32_FILTER: the Chebyshev B-solve inside the filter reads FP32 values
  * of B~ (FP64 vectors, FP64 row accumulation); A~, the Lanczos recurrence,
- * re-orthogonalisation, Rayleigh-Ritz and residuals stay FP64.  Call after
- * nmg_set_bsol_chebiter. */
+ * re-orthogonalisation, Rayleigh-Ritz and residuals stay FP64.
+ * NMG_FP32_BSOLVE: as NMG_FP32_FILTER, and the k-column B-solves of the
+ * block and slice paths also hold b and their iterates in FP32 (each
+ * product and update formed in FP64, rounded once on store): half the
+ * vector bytes of every Chebyshev step.  Call after nmg_set_bsol_chebiter. */
 int nmg_set_precision(nmg_handle h, int mode);
 /* Lanczos start vector seed (pEVSL draws a random vinit). */
 int nmg_set_seed(nmg_handle h, uint64_t seed);
@@ -279,6 +282,9 @@ int nmg_bench_copy(int64_t n, int reps, double *seconds);
 /* Timing of one whole Chebyshev solve (deg steps) on device vectors, in the
  * cheb's current mode; bytes: deg x the algorithmic bytes of one step. */
 int nmg_bench_bsolve(nmg_handle cheb, int reps, int fp32, double *seconds, double *bytes);
+/* One K-column fused Chebyshev step, `reps` times on device vectors: fp32 =
+ * 0 FP64, 1 FP32 values, 2 FP32 values and vectors; bytes: its algorithmic
+ * bytes (matrix once, b, x_k, x_(k-1) read and x_(k+1) written). */
 int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double *seconds, double *bytes);
 /* Workgroup order of the sparse-product kernels of mat (no reference
  * counterpart): 1 (default) gives each of the 8 XCDs one contiguous range

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -485,6 +486,17 @@ struct DVec {
   }
   ~DVec() { dfree(p); }
 };
+struct FVec {
+  float* p = nullptr;
+  int64_t n = 0;
+  void ensure(int64_t m) {
+    if (m <= n) return;
+    dfree(p);
+    p = dmalloc<float>(m);
+    n = m;
+  }
+  ~FVec() { dfree(p); }
+};
 
 // Fluid-solid operator of sparsefsAV (src/mod_matvec.f90:498-520):
 //   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))],  C_p ~ Ap~^-1 by Chebyshev
@@ -546,13 +558,14 @@ struct Solver {
   hipStream_t sx[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DVec hb[12];
+  FVec hf[4];  // FP32 work of the k-column B-solve parts (3 blocks each)
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
   int verbose = 0;  // progress line on stderr every `verbose` Lanczos steps (0: silent)
   // B-solves replayed from captured HIP graphs (nmg_set_graphs): one
   // executable graph per distinct set of buffers, built on its second use
   bool graphs = false;
   struct GraphKey {
-    const void* p[16];
+    const void* p[24];
     int K, parts, fp32, deg;
     double lmin, lmax;
   };
@@ -595,7 +608,8 @@ struct Solver {
   void check_A() const {
     if (!fs) (void)A();
   }
-  bool fp32() const { return precision == NMG_FP32_FILTER; }
+  bool fp32() const { return precision != NMG_FP64; }                // FP32 values in the B-solve
+  bool fp32v() const { return precision == NMG_FP32_BSOLVE; }        // and FP32 vectors (k-column solves)
 };
 
 static void apply_A_block(Solver& S, int K, const double* x, double* y);
@@ -679,7 +693,8 @@ static Solver::GraphKey graph_key(const Solver& S, const Cheb& C, int K, int par
   k.p[2] = tmp;
   k.p[3] = &C;
   k.p[4] = C.mat;
-  for (int i = 0; i < 3 * parts && i < 11; ++i) k.p[5 + i] = S.hb[i].p;
+  for (int i = 0; i < 3 * parts && i < 12; ++i) k.p[5 + i] = S.hb[i].p;
+  for (int i = 0; i < 4; ++i) k.p[17 + i] = S.hf[i].p;
   k.K = K;
   k.parts = parts;
   k.fp32 = S.fp32() ? 1 : 0;
@@ -1184,9 +1199,11 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
 
 // K interleaved columns of x ~ M^{-1} b by the Chebyshev semi-iteration of
 // cheb_solve, one K-column fused step per degree
+// With w32 (3 n K floats), the iterates and b are held in FP32 (the product
+// and the update still formed in FP64 per entry): half the vector bytes.
 static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
-                       KernelProbe* probe) {
-  if (K == 1) return cheb_solve(C, b, out, tmp, s, fp32, probe);
+                       KernelProbe* probe, float* w32 = nullptr) {
+  if (K == 1 && !w32) return cheb_solve(C, b, out, tmp, s, fp32, probe);
   const int64_t nk = C.mat->nrow * K;
   const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
   if (const PersistPlan* P = C.persistent(fp32)) {
@@ -1201,6 +1218,27 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
     return;
   }
   const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
+  if (w32) {
+    float* b32 = w32;
+    float* fk = w32 + nk;
+    float* fm = w32 + 2 * nk;
+    k_cheb32_start(nk, 1.0 / th, b, b32, fk, fm, s);
+    const bool pr = probe && probe->armed();
+    if (pr) probe->before(s);
+    double w = 2.0;
+    for (int k = 1; k <= C.deg; ++k) {
+      w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+      k_block_cheb_step32(K, *C.mat, b32, fk, fm, w, 1.0 / th, s, fp32);
+      std::swap(fk, fm);
+    }
+    if (pr) probe->after(s);
+    k_f32_to_f64(nk, fk, out, s);
+    if (probe) {
+      probe->bytes = C.mat->matrix_bytes(fp32) + 4.0 * 4.0 * (double)nk;
+      probe->launches = C.deg;
+    }
+    return;
+  }
   double* xk = (C.deg % 2 == 0) ? out : tmp;
   double* xm = (C.deg % 2 == 0) ? tmp : out;
   k_scale_copy(nk, 1.0 / th, b, xk, s);
@@ -1235,11 +1273,14 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
     }
     for (int i = 0; i < 3 * P; ++i) S.hb[i].ensure(S.n * H);
   }
+  const bool v32 = S.fp32v() && !C.persistent(S.fp32());
+  if (v32)
+    for (int q = 0; q < P; ++q) S.hf[q].ensure(3 * S.n * H);
   const bool graphed = S.graphs && !C.persistent(S.fp32());
   KernelProbe* probe = &S.probe[0];
   auto body = [&](KernelProbe* pb) {
     if (P == 1) {
-      block_cheb(C, K, b, out, tmp, S.s, S.fp32(), pb);
+      block_cheb(C, K, b, out, tmp, S.s, S.fp32(), pb, v32 ? S.hf[0].p : nullptr);
       return;
     }
     double* in[4];
@@ -1253,7 +1294,7 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
     for (int q = 1; q < P; ++q) NMG_HIP(hipStreamWaitEvent(S.sx[q - 1], S.ev[0], 0));
     for (int q = 0; q < P; ++q)
       block_cheb(C, H, in[q], res[q], S.hb[3 * q + 2].p, q == 0 ? S.s : S.sx[q - 1], S.fp32(),
-                 q == 0 ? pb : nullptr);
+                 q == 0 ? pb : nullptr, v32 ? S.hf[q].p : nullptr);
     for (int q = 1; q < P; ++q) {
       NMG_HIP(hipEventRecord(S.ev[q], S.sx[q - 1]));
       NMG_HIP(hipStreamWaitEvent(S.s, S.ev[q], 0));
@@ -1267,7 +1308,7 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
     if (pr) probe->before(S.s);
     run_graphed(S, graph_key(S, C, K, P, b, out, tmp), [&] { body(nullptr); });
     if (pr) probe->after(S.s);
-    probe->bytes = C.mat->matrix_bytes(S.fp32()) + 4.0 * 8.0 * (double)C.mat->nrow * H;
+    probe->bytes = C.mat->matrix_bytes(S.fp32()) + 4.0 * (v32 ? 4.0 : 8.0) * (double)C.mat->nrow * H;
     probe->launches = C.deg;
   } else {
     body(probe);
@@ -2118,9 +2159,11 @@ int nmg_set_geneig(nmg_handle h) {
 int nmg_set_precision(nmg_handle h, int mode) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
-    NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER, NMG_EINVAL, "unknown precision");
+    NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER || mode == NMG_FP32_BSOLVE, NMG_EINVAL,
+              "unknown precision");
     S->precision = mode;
-    if (mode == NMG_FP32_FILTER) {
+    S->drop_graphs();
+    if (mode != NMG_FP64) {
       // FP32 values in the Chebyshev B-solve only: B~ is well conditioned
       // (kappa ~ 15 at P2), while FP32 rounding of the stiffness would be
       // amplified by lambda_max/lambda on the low modes
@@ -2673,7 +2716,7 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   // row (a tile must hold one row) up to 64 KB of LDS per workgroup
   int64_t longest = 0;
   for (int64_t r = 0; r < M.nb; ++r) longest = std::max(longest, M.h_browptr[r + 1] - M.h_browptr[r]);
-  const int limit = std::min(65535, 64 * 1024 / (24 * K));
+  const int limit = std::min(65535, tile_cap_limit(K));
   std::vector<int> caps;
   for (double f = 1.0; f * (double)longest <= (double)limit * 1.0001; f *= 1.28)
     caps.push_back((int)std::ceil(f * (double)longest));
@@ -2732,27 +2775,48 @@ int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double*
     Cheb* C = get<Cheb>(cheb, K_CHEB);
     NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
     NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(fp32 >= 0 && fp32 <= 2, NMG_EINVAL, "fp32 must be 0 (FP64), 1 (FP32 values) or 2 (and FP32 vectors)");
     DevMat* M = C->mat;
     hipStream_t s = default_stream();
     if (fp32 && !(M->val32 || M->bval32)) make_fp32(*M, s);
     const int64_t nk = M->nrow * K;
-    DVec b, xk, xm;
-    b.ensure(nk);
-    xk.ensure(nk);
-    xm.ensure(nk);
-    k_fill(nk, 1.0, b.p, s);
-    k_fill(nk, 0.5, xk.p, s);
-    k_fill(nk, 0.0, xm.p, s);
     const double th = 0.5 * (C->lmax + C->lmin);
-    k_block_cheb_step(K, *M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
+    DVec b, xk, xm;
+    FVec f;
+    std::function<void()> step;
+    if (fp32 == 2) {
+      f.ensure(3 * nk);
+      float* fb = f.p;
+      float* fk = f.p + nk;
+      float* fm = f.p + 2 * nk;
+      b.ensure(nk);
+      k_fill(nk, 1.0, b.p, s);
+      k_cheb32_start(nk, 1.0 / th, b.p, fb, fk, fm, s);
+      step = [=, &s]() mutable {
+        k_block_cheb_step32(K, *M, fb, fk, fm, 1.5, 1.0 / th, s, true);
+        std::swap(fk, fm);
+      };
+    } else {
+      b.ensure(nk);
+      xk.ensure(nk);
+      xm.ensure(nk);
+      k_fill(nk, 1.0, b.p, s);
+      k_fill(nk, 0.5, xk.p, s);
+      k_fill(nk, 0.0, xm.p, s);
+      double* pk = xk.p;
+      double* pm = xm.p;
+      const double* pb = b.p;
+      step = [=, &s]() mutable {
+        k_block_cheb_step(K, *M, pb, pk, pm, 1.5, 1.0 / th, s, fp32 != 0);
+        std::swap(pk, pm);
+      };
+    }
+    step();
     hipEvent_t e0, e1;
     NMG_HIP(hipEventCreate(&e0));
     NMG_HIP(hipEventCreate(&e1));
     NMG_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) {
-      k_block_cheb_step(K, *M, b.p, xk.p, xm.p, 1.5, 1.0 / th, s, fp32 != 0);
-      std::swap(xk.p, xm.p);
-    }
+    for (int r = 0; r < reps; ++r) step();
     NMG_HIP(hipEventRecord(e1, s));
     NMG_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -2760,7 +2824,7 @@ int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double*
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (seconds) *seconds = ms * 1e-3 / reps;
-    if (bytes) *bytes = M->matrix_bytes(fp32 != 0) + 4.0 * 8.0 * (double)nk;
+    if (bytes) *bytes = M->matrix_bytes(fp32 != 0) + 4.0 * (fp32 == 2 ? 4.0 : 8.0) * (double)nk;
   });
 }
 
@@ -2784,7 +2848,7 @@ int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int 
     DevMat* M = get<DevMat>(mat, K_MAT);
     NMG_CHECK(rows_per_wave >= 1 && rows_per_wave <= 8 && K * rows_per_wave <= 64, NMG_EINVAL,
               "bad rows per wavefront");
-    NMG_CHECK(max_nodes * 24 * K <= 64 * 1024, NMG_EINVAL, "tile exceeds 64 KB of LDS");
+    NMG_CHECK(max_nodes <= tile_cap_limit(K), NMG_EINVAL, "tile exceeds 64 KB of LDS");
     build_tiling(*M, K, max_nodes, rows_per_wave, threads > 0 ? threads : 1024);
     if (K == 1) {
       M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;

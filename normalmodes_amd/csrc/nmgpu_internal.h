@@ -138,6 +138,12 @@ void set_value_order(DevMat& M, int tiled, hipStream_t s);
 
 // builds (or, with per_wave <= 0, drops) the row groups of M (blocked layouts)
 void build_row_groups(DevMat& M, int per_wave);
+// LDS stride (doubles) of one node's x block in the tiled k-column kernels
+// (a stride padded by one double, spreading the blocks over LDS banks, was
+// measured no faster on MI355X)
+constexpr int tile_stride(int K) { return 3 * K; }
+// largest tile node capacity within 64 KB of LDS
+inline int tile_cap_limit(int K) { return 64 * 1024 / (8 * tile_stride(K)); }
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024);
 // creates a DevMat from host CSR (0-based); picks the layout
@@ -193,6 +199,11 @@ void k_resid(int64_t n, const double* ax, const double* bx, const double* x, dou
 // k-column products on column-interleaved block vectors (entry (i, c) at i*K + c),
 // K in {1, 2, 4, 8, 16}; mu: K per-column filter coefficients of this degree
 void k_block_spmv(int K, const DevMat& M, const double* x, double* y, hipStream_t s, bool fp32 = false);
+// the same step with FP32 vectors (FP64 product and update, one rounding on store)
+void k_block_cheb_step32(int K, const DevMat& M, const float* b, const float* xk, float* xm, double w,
+                         double ith, hipStream_t s, bool fp32);
+void k_cheb32_start(int64_t n, double ith, const double* b, float* b32, float* x1, float* x0, hipStream_t s);
+void k_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t s);
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32 = false);
 void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,

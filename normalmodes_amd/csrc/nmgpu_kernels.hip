@@ -960,6 +960,25 @@ struct BEpiCheb {
     xm[i * K + c] = p.xm + w * (p.xk - p.xm + (p.b - v) * ith);
   }
 };
+// the Chebyshev step with FP32 vectors (b, x_k, x_{k-1} stored in FP32; the
+// product and the update are formed in FP64 and rounded once on store)
+template <int K>
+struct BEpiCheb32 {
+  const float* b;
+  const float* xk;
+  float* xm;
+  double w, ith;
+  struct Pre {
+    double b, xk, xm;
+  };
+  __device__ __forceinline__ Pre load(int64_t i, int c) const {
+    const int64_t o = i * K + c;
+    return Pre{(double)b[o], (double)xk[o], (double)xm[o]};
+  }
+  __device__ __forceinline__ void store(int64_t i, int c, double v, const Pre& p) const {
+    xm[i * K + c] = (float)(p.xm + w * (p.xk - p.xm + (p.b - v) * ith));
+  }
+};
 template <int K>
 struct BEpiFilter {
   const double* vk;
@@ -1011,11 +1030,11 @@ __device__ __forceinline__ double slots_sum(double v) {
   return v;
 }
 
-template <int K, int R, class VT, class Epi>
+template <int K, int R, class VT, class Epi, class XT>
 __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                            const int32_t* __restrict__ ci,
                                                            const VT* __restrict__ mv,
-                                                           const double* __restrict__ x, Epi epi, int flags) {
+                                                           const XT* __restrict__ x, Epi epi, int flags) {
   constexpr int E = BlockMap<K, R>::E, U = BlockMap<K, R>::U;
   const int lane = threadIdx.x & 63;
   const int64_t row = ((xcd_block(flags) * kBlock + threadIdx.x) >> 6) * R + lane / (K * E);
@@ -1041,9 +1060,9 @@ __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int
       double xa[U], xb[U], xc[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        xa[u] = x[base[u]];
-        xb[u] = x[base[u] + K];
-        xc[u] = x[base[u] + 2 * K];
+        xa[u] = (double)x[base[u]];
+        xb[u] = (double)x[base[u] + K];
+        xc[u] = (double)x[base[u] + 2 * K];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1063,12 +1082,12 @@ __global__ __launch_bounds__(kBlock) void mass3_block_rows(int64_t nb, const int
   }
 }
 
-template <int K, int R, class VT, class Epi>
+template <int K, int R, class VT, class Epi, class XT>
 __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nbnz,
                                                           const int64_t* __restrict__ rp,
                                                           const int32_t* __restrict__ ci,
                                                           const VT* __restrict__ bv,
-                                                          const double* __restrict__ x, Epi epi, int flags) {
+                                                          const XT* __restrict__ x, Epi epi, int flags) {
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U;  // 9 values per entry
   const int lane = threadIdx.x & 63;
@@ -1097,9 +1116,9 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
       for (int u = 0; u < U; ++u) {
 #pragma unroll
         for (int p = 0; p < 9; ++p) a[u][p] = ok[u] ? (double)ldm(bv + bidx(kk[u], p, nbnz, flags), flags) : 0.0;
-        xa[u] = x[base[u]];
-        xb[u] = x[base[u] + K];
-        xc[u] = x[base[u] + 2 * K];
+        xa[u] = (double)x[base[u]];
+        xb[u] = (double)x[base[u] + K];
+        xc[u] = (double)x[base[u] + 2 * K];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1120,11 +1139,11 @@ __global__ __launch_bounds__(kBlock) void bsr3_block_rows(int64_t nb, int64_t nb
 }
 
 // generic CSR with K columns: one wavefront per scalar row, lanes (e, c)
-template <int K, class VT, class Epi>
+template <int K, class VT, class Epi, class XT>
 __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int64_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci,
                                                          const VT* __restrict__ va,
-                                                         const double* __restrict__ x, Epi epi, int flags) {
+                                                         const XT* __restrict__ x, Epi epi, int flags) {
   constexpr int E = 64 / K;
   const int64_t row = (xcd_block(flags) * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -1136,7 +1155,7 @@ __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int
   if (row < nrow) {
     const int64_t end = rp[row + 1];
     for (int64_t k = rp[row] + e; k < end; k += E)
-      s += (double)ldm(va + k, flags) * x[(int64_t)ldm(ci + k, flags) * K + c];
+      s += (double)ldm(va + k, flags) * (double)x[(int64_t)ldm(ci + k, flags) * K + c];
   }
   s = slot_sum<K>(s);
   if (wr) epi.store(row, c, s, pre);
@@ -1150,25 +1169,25 @@ static int block_rows(const DevMat& M, int K) {
   return 1;
 }
 
-template <int K, int R, class VT, class Epi>
-static void launch_block_R(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+template <int K, int R, class VT, class Epi, class XT>
+static void launch_block_R(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
   const int64_t waves = (M.nb + R - 1) / R;
   const int64_t grid = (waves * 64 + kBlock - 1) / kBlock;
   if (grid <= 0) return;
   if (M.layout == L_BSR3)
-    hipLaunchKernelGGL((bsr3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+    hipLaunchKernelGGL((bsr3_block_rows<K, R, VT, Epi, XT>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
                        M.nbnz, M.browptr, M.bcol, v, x, epi, kflags(M));
   else
-    hipLaunchKernelGGL((mass3_block_rows<K, R, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+    hipLaunchKernelGGL((mass3_block_rows<K, R, VT, Epi, XT>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
                        M.browptr, M.bcol, v, x, epi, kflags(M));
 }
 
-template <int K, class VT, class Epi>
-static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+template <int K, class VT, class Epi, class XT>
+static void launch_block(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
   if (M.layout == L_CSR) {
     const int64_t grid = (M.nrow * 64 + kBlock - 1) / kBlock;
     if (grid > 0)
-      hipLaunchKernelGGL((csr_block_rows<K, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
+      hipLaunchKernelGGL((csr_block_rows<K, VT, Epi, XT>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
                          M.rowptr, M.col, v, x, epi, kflags(M));
   } else {
     const int R = block_rows(M, K);
@@ -1181,11 +1200,11 @@ static void launch_block(const DevMat& M, const VT* v, const double* x, Epi epi,
   NMG_HIP(hipGetLastError());
 }
 
-template <int K, class VT, class Epi>
-bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s);
+template <int K, class VT, class Epi, class XT>
+bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s);
 
-template <int K, class Epi>
-static void dispatch_block(const DevMat& M, const double* x, Epi epi, hipStream_t s, bool fp32) {
+template <int K, class Epi, class XT>
+static void dispatch_block(const DevMat& M, const XT* x, Epi epi, hipStream_t s, bool fp32) {
   if (M.layout != L_CSR) {
     const bool f = fp32 && M.bval32;
     if (f ? launch_tiled<K>(M, M.bval32, x, epi, s) : launch_tiled<K>(M, M.bval, x, epi, s)) return;
@@ -1221,6 +1240,34 @@ void k_block_spmv(int K, const DevMat& M, const double* x, double* y, hipStream_
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32) {
   NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(M, xk, BEpiCheb<KK>{b, xk, xm, w, ith}, s, fp32)));
+}
+
+void k_block_cheb_step32(int K, const DevMat& M, const float* b, const float* xk, float* xm, double w,
+                         double ith, hipStream_t s, bool fp32) {
+  NMG_BLOCK_SWITCH(K, (dispatch_block<KK>(M, xk, BEpiCheb32<KK>{b, xk, xm, w, ith}, s, fp32)));
+}
+
+// start of the FP32-vector Chebyshev solve: b32 = fl(b), x1 = fl(b/th), x0 = 0
+__global__ void cheb32_start_k(int64_t n, double ith, const double* __restrict__ b, float* __restrict__ b32,
+                               float* __restrict__ x1, float* __restrict__ x0) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double v = b[i];
+    b32[i] = (float)v;
+    x1[i] = (float)(v * ith);
+    x0[i] = 0.0f;
+  }
+}
+void k_cheb32_start(int64_t n, double ith, const double* b, float* b32, float* x1, float* x0, hipStream_t s) {
+  hipLaunchKernelGGL(cheb32_start_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ith, b, b32, x1, x0);
+  NMG_HIP(hipGetLastError());
+}
+__global__ void f32_to_f64_k(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = (double)x[i];
+}
+void k_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(f32_to_f64_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y);
+  NMG_HIP(hipGetLastError());
 }
 
 void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
@@ -1314,16 +1361,17 @@ void k_from_interleaved(int64_t n, int K, const double* il, double* cm, int64_t 
 namespace nmg {
 
 // TB threads (TB/64 wavefronts) share one tile's LDS: 1024 or 256
-template <int K, int R, class VT, class Epi, bool BSR, int TB>
+template <int K, int R, class VT, class Epi, bool BSR, int TB, class XT>
 __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict__ trp,
                                                            const int64_t* __restrict__ tcp,
                                                            const int32_t* __restrict__ tcols,
                                                            const int64_t* __restrict__ rp,
                                                            const uint16_t* __restrict__ lcol,
                                                            const VT* __restrict__ val, int64_t nbnz,
-                                                           const double* __restrict__ x, Epi epi, int flags) {
+                                                           const XT* __restrict__ x, Epi epi, int flags) {
   extern __shared__ double xs[];
   constexpr int W3 = 3 * K;
+  constexpr int WS = tile_stride(K);  // LDS stride of one node's block
   constexpr int E = BlockMap<K, R>::E;
   constexpr int U = BSR ? (BlockMap<K, R>::U > 2 ? 2 : BlockMap<K, R>::U) : BlockMap<K, R>::U;
   const int64_t tile = xcd_block(flags);
@@ -1333,7 +1381,7 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
   // stage the tile's x blocks
   for (int t = threadIdx.x; t < nu * W3; t += TB) {
     const int node = t / W3, j = t - node * W3;
-    xs[t] = x[(int64_t)tcols[c0 + node] * W3 + j];
+    xs[node * WS + j] = (double)x[(int64_t)tcols[c0 + node] * W3 + j];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -1355,7 +1403,7 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
         for (int u = 0; u < U; ++u) {
           const int64_t k = k0 + (int64_t)u * E;
           if (k < end) {
-            const double* xn = xs + (int)ldm(lcol + k, flags) * W3 + c;
+            const double* xn = xs + (int)ldm(lcol + k, flags) * WS + c;
             const double xa = xn[0], xb = xn[K], xc = xn[2 * K];
             if (BSR) {
               double v[9];
@@ -1385,27 +1433,27 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
   }
 }
 
-template <int K, int R, class VT, class Epi, int TB>
-static void launch_tiled_RB(const DevMat& M, const Tiling& T, const VT* v, const double* x, Epi epi,
+template <int K, int R, class VT, class Epi, int TB, class XT>
+static void launch_tiled_RB(const DevMat& M, const Tiling& T, const VT* v, const XT* x, Epi epi,
                             hipStream_t s) {
-  const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
+  const size_t shmem = (size_t)T.max_nodes * tile_stride(K) * sizeof(double);
   if (M.layout == L_BSR3)
-    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
-                       T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, true, TB, XT>), dim3((unsigned)T.ntiles), dim3(TB), shmem,
+                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
   else
-    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
-                       T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
+    hipLaunchKernelGGL((tiled_block_rows<K, R, VT, Epi, false, TB, XT>), dim3((unsigned)T.ntiles), dim3(TB), shmem,
+                       s, T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, M.nbnz, x, epi, kflags(M));
 }
 
-template <int K, int R, class VT, class Epi>
-static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const double* x, Epi epi,
+template <int K, int R, class VT, class Epi, class XT>
+static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const XT* x, Epi epi,
                            hipStream_t s) {
   if (T.block == 256) launch_tiled_RB<K, R, VT, Epi, 256>(M, T, v, x, epi, s);
   else launch_tiled_RB<K, R, VT, Epi, 1024>(M, T, v, x, epi, s);
 }
 
-template <int K, class VT, class Epi>
-bool launch_tiled(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+template <int K, class VT, class Epi, class XT>
+bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
   int idx = 0;
   while ((1 << idx) != K) ++idx;
   const Tiling* T = M.tiling[idx];

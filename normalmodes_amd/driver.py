@@ -288,7 +288,7 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
             pv.set_graphs(h, True)
         if verbose:
             pv.set_verbose(h, verbose)
-        if resid_target is None and precision == pv.FP32_FILTER:
+        if resid_target is None and precision != pv.FP64:
             # the FP32 B-solve bounds ||A~x - lam B~x||/|lam| near 1e-6 (its
             # eigenvalues still agree with FP64 to ~1e-14): stop at that floor
             resid_target = 1e-5

@@ -23,10 +23,11 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "LAYOUTS"]
+           "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
+FP32_BSOLVE = 2   # FP32 values and FP32 vectors in the k-column B-solves
 LAYOUTS = {0: "csr", 1: "bsr3", 2: "mass3"}
 
 _i64 = ctypes.c_int64
@@ -439,7 +440,8 @@ def bench_spmv(mat: Handle, reps: int = 20, fp32: bool = False):
     return s.value, b.value
 
 
-def bench_block_chebstep(cheb: Handle, K: int, reps: int = 20, fp32: bool = False):
+def bench_block_chebstep(cheb: Handle, K: int, reps: int = 20, fp32: int = 0):
+    """fp32: 0 FP64, 1 FP32 values, 2 FP32 values and vectors."""
     s, b = _d(), _d()
     _check(_lib().nmg_bench_block_chebstep(_i64(cheb), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
                                            ctypes.byref(s), ctypes.byref(b)))

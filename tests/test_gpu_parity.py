@@ -633,3 +633,37 @@ def test_row_group_products(small_p2, dense_eigs):
         assert np.all(r.res2 <= 1e-10)
     finally:
         mv.free()
+
+
+def test_fp32_vector_bsolve(small_p2, dense_eigs):
+    """FP32_BSOLVE: the k-column B-solve with FP32 values and FP32 iterates
+    (FP64 product and update per entry) against the FP64 solve, one stream
+    and split over two; then block-8 solves whose eigenvalues stay within
+    1e-10 of the dense pencil's."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg="auto", tune=(1, 4, 8))
+    try:
+        X = np.random.default_rng(47).standard_normal((8, small_p2.n))
+        h64 = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, split=0)
+        try:
+            ref = pv.apply_block(h64, 2, X)
+        finally:
+            pv.pevsl_finish(h64)
+        for parts in (0, 2):
+            h = _open_solver(mv, precision=pv.FP32_BSOLVE, seed=7, rtol=1e-9, split=parts)
+            try:
+                for _ in range(2):
+                    y = pv.apply_block(h, 2, X)
+                    assert 1e-12 < _rel(y, ref) < 1e-5, (parts, _rel(y, ref))
+            finally:
+                pv.pevsl_finish(h)
+        w = dense_eigs(small_p2)
+        lam = w[(w >= 1.0) & (w <= 12.0)]
+        for split in (0, 2):
+            r = apply_pevsl(mv, interval=(1.0, 12.0), block=8, split=split, precision=pv.FP32_BSOLVE,
+                            vectors=False)
+            assert r.lam.size == lam.size and np.max(np.abs(r.lam - lam) / lam) < 1e-10
+            assert np.all(r.res2 <= 1e-4)
+    finally:
+        mv.free()

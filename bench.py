@@ -79,8 +79,9 @@ def upload_with_layouts(P, layouts, tune):
     ``layouts``: 'auto', or comma-separated '<A|B><K>:<kind>:<params>' with
     kind 'lanes:<g>' (K = 1, g lanes per node row), 'rows:<r>' (r node rows
     per wavefront), 'tiled:<cap>:<r>[:<threads>]' (LDS tiles of at most cap
-    nodes, workgroups of 1024 or 256 threads) or 'groups:<g>' (K = 1, row
-    groups, g per wavefront)."""
+    nodes, workgroups of 1024 or 256 threads), 'groups:<g>' (K = 1, row
+    groups, g per wavefront) or 'wide:<g>' (K >= 2, scalar mass, g lanes per
+    node row)."""
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
@@ -99,6 +100,8 @@ def upload_with_layouts(P, layouts, tune):
             pv.set_tiling(mat, K, int(f[2]), int(f[3]), int(f[4]) if len(f) > 4 else 1024)
         elif f[1] == "groups":
             pv.set_row_groups(mat, int(f[2]))
+        elif f[1] == "wide":
+            pv.set_wide(mat, K, int(f[2]))
         else:
             raise ValueError(part)
         mv.tuned[f[0]] = {"fixed": ":".join(f[1:])}

@@ -376,6 +376,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->h_bcol = S.h_bcol;
   M->group = S.group;
   for (int i = 0; i < 5; ++i) M->block_map[i] = S.block_map[i];
+  for (int i = 0; i < 5; ++i) M->wide[i] = S.wide[i];
   auto cp = [&](auto* src, auto*& dst, size_t count) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
     if (!src) return;
@@ -2640,6 +2641,7 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
                 NMG_EINVAL, "rows per wavefront must be 0, 1, 2, 4 or 8");
       NMG_CHECK(K * std::max(rows_per_wave, 1) <= 64, NMG_EINVAL, "K x rows per wavefront must be <= 64");
       M->block_map[idx] = rows_per_wave;
+      M->wide[idx] = 0;
     }
   });
 }
@@ -2672,7 +2674,8 @@ static double time_launches(hipStream_t s, int reps, F&& f) {
 
 struct TuneResult {
   double seconds = 1e30;
-  int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block, 3 row groups (K = 1)
+  int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block, 3 row groups (K = 1),
+                 // 4 wide (scalar mass, K >= 2)
   int group = 0, rows = 1, cap = 0;
   int64_t ntiles = 0;
   int block = 1024;  // tile workgroup threads
@@ -2694,6 +2697,14 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   // untiled layouts (the single-column route is K = 1's choice alone: tuning
   // another width leaves it as it is)
   build_tiling(M, K, 0, 1);
+  M.wide[idx] = 0;
+  if (M.layout == L_MASS3 && K >= 2) {
+    for (int G : {4, 8, 16, 32}) {
+      M.wide[idx] = G;
+      consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 4, 0, G, 0, 0);
+    }
+    M.wide[idx] = 0;
+  }
   if (K == 1) {
     M.single_blocked = false;
     for (int pw : {1, 2, 4, 8}) {
@@ -2739,6 +2750,8 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
     M.group = best.group;
   } else if (best.kind == 3) {
     build_row_groups(M, best.rows);
+  } else if (best.kind == 4) {
+    M.wide[idx] = best.rows;
   } else if (best.kind == 1) {
     M.block_map[idx] = best.rows;
     if (K == 1) M.single_blocked = true;
@@ -2850,10 +2863,26 @@ int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int 
               "bad rows per wavefront");
     NMG_CHECK(max_nodes <= tile_cap_limit(K), NMG_EINVAL, "tile exceeds 64 KB of LDS");
     build_tiling(*M, K, max_nodes, rows_per_wave, threads > 0 ? threads : 1024);
+    for (int i = 0; i < 5; ++i)
+      if ((1 << i) == K) M->wide[i] = 0;
     if (K == 1) {
       M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
       build_row_groups(*M, 0);
     }
+  });
+}
+
+int nmg_set_wide(nmg_handle mat, int K, int lanes) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    int idx = -1;
+    for (int i = 1; i < 5; ++i)
+      if ((1 << i) == K) idx = i;
+    NMG_CHECK(idx >= 1, NMG_EINVAL, "K must be 2, 4, 8 or 16");
+    NMG_CHECK(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, NMG_EINVAL,
+              "lanes per row must be 0, 4, 8, 16 or 32");
+    NMG_CHECK(lanes == 0 || M->layout == L_MASS3, NMG_EINVAL, "the wide layout applies to scalar-mass matrices");
+    M->wide[idx] = lanes;
   });
 }
 

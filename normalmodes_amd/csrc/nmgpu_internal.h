@@ -85,6 +85,9 @@ struct DevMat {
   int group = 0;
   int block_map[5] = {0, 0, 0, 0, 0};
   Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
+  // lanes per node row of the wide k-column scalar-mass kernel (one entry
+  // per lane, K columns in registers), per K = 1,2,4,8,16; 0 = not used
+  int wide[5] = {0, 0, 0, 0, 0};
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
   RowGroups* rgroups = nullptr; // single-column products through the row-group kernels (first choice)
   int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)

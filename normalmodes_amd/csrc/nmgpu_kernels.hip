@@ -1161,6 +1161,112 @@ __global__ __launch_bounds__(kBlock) void csr_block_rows(int64_t nrow, const int
   if (wr) epi.store(row, c, s, pre);
 }
 
+// "Wide" k-column scalar-mass product: one lane per matrix entry, the K
+// columns in registers.  G lanes share a node row (64/G rows per
+// wavefront); a lane reads the entry's column and value once and the
+// neighbour's 3K contiguous x values with 16-byte loads, so the loads per
+// entry do not grow with K.  The 3K row sums are reduced over the G lanes by
+// a butterfly; lane g then stores outputs g, g + G, ... of its row.
+template <class XT>
+__device__ __forceinline__ void load16(const XT* p, double* out);
+template <>
+__device__ __forceinline__ void load16<double>(const double* p, double* out) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void load16<float>(const float* p, double* out) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+  out[3] = v.w;
+}
+
+template <int K, int G, class VT, class Epi, class XT>
+__global__ __launch_bounds__(kBlock) void mass3_wide_rows(int64_t nb, const int64_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ ci,
+                                                          const VT* __restrict__ mv,
+                                                          const XT* __restrict__ x, Epi epi, int flags) {
+  constexpr int W3 = 3 * K;
+  constexpr int PER = 16 / sizeof(XT);  // values per 16-byte load
+  static_assert(W3 % PER == 0, "3K must fill 16-byte loads");
+  constexpr int Q = (W3 + G - 1) / G;   // outputs per lane
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((xcd_block(flags) * kBlock + threadIdx.x) >> 6) * (64 / G) + lane / G;
+  const int g = lane % G;
+  const bool live = row < nb;
+  typename Epi::Pre pre[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = g + q * G;
+    if (live && j < W3) pre[q] = epi.load(3 * row + j / K, j % K);
+  }
+  double acc[W3];
+#pragma unroll
+  for (int j = 0; j < W3; ++j) acc[j] = 0.0;
+  if (live) {
+    const int64_t end = rp[row + 1];
+    for (int64_t k = rp[row] + g; k < end; k += G) {
+      const XT* xb = x + 3 * (int64_t)ldm(ci + k, flags) * K;
+      const double m = (double)ldm(mv + k, flags);
+      double xv[W3];
+#pragma unroll
+      for (int j = 0; j < W3; j += PER) load16<XT>(xb + j, xv + j);
+#pragma unroll
+      for (int j = 0; j < W3; ++j) acc[j] += m * xv[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < W3; ++j) acc[j] = group_sum<G>(acc[j]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = g + q * G;
+    double v = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < W3; ++jj)
+      if (jj == j) v = acc[jj];
+    if (live && j < W3) epi.store(3 * row + j / K, j % K, v, pre[q]);
+  }
+}
+
+template <int K, int G, class VT, class Epi, class XT>
+static void launch_wide_G(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
+  const int64_t waves = (M.nb + (64 / G) - 1) / (64 / G);
+  const int64_t grid = (waves * 64 + kBlock - 1) / kBlock;
+  if (grid <= 0) return;
+  hipLaunchKernelGGL((mass3_wide_rows<K, G, VT, Epi, XT>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb,
+                     M.browptr, M.bcol, v, x, epi, kflags(M));
+}
+
+// the wide layout applies to scalar-mass matrices with 3K a multiple of the
+// 16-byte load (K >= 2 for FP64 vectors, K >= 4 for FP32)
+template <int K, class XT>
+constexpr bool wide_ok() {
+  return K >= 2 && (3 * K) % (16 / (int)sizeof(XT)) == 0;
+}
+
+template <int K, class VT, class Epi, class XT>
+static bool launch_wide(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
+  if constexpr (!wide_ok<K, XT>()) {
+    return false;
+  } else {
+    int idx = 0;
+    while ((1 << idx) != K) ++idx;
+    const int G = M.wide[idx];
+    if (G <= 0 || M.layout != L_MASS3) return false;
+    switch (G) {
+      case 4: launch_wide_G<K, 4>(M, v, x, epi, s); break;
+      case 8: launch_wide_G<K, 8>(M, v, x, epi, s); break;
+      case 16: launch_wide_G<K, 16>(M, v, x, epi, s); break;
+      default: launch_wide_G<K, 32>(M, v, x, epi, s); break;
+    }
+    NMG_HIP(hipGetLastError());
+    return true;
+  }
+}
+
 // rows per wavefront of the k-column kernels: chosen per (matrix, K) by the
 // autotuner (tune_block_map); 0 means "not tuned" and uses R = 1
 static int block_rows(const DevMat& M, int K) {
@@ -1205,6 +1311,10 @@ bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_
 
 template <int K, class Epi, class XT>
 static void dispatch_block(const DevMat& M, const XT* x, Epi epi, hipStream_t s, bool fp32) {
+  if (M.layout == L_MASS3) {
+    const bool f = fp32 && M.bval32;
+    if (f ? launch_wide<K>(M, M.bval32, x, epi, s) : launch_wide<K>(M, M.bval, x, epi, s)) return;
+  }
   if (M.layout != L_CSR) {
     const bool f = fp32 && M.bval32;
     if (f ? launch_tiled<K>(M, M.bval32, x, epi, s) : launch_tiled<K>(M, M.bval, x, epi, s)) return;

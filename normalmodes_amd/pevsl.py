@@ -469,6 +469,12 @@ def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1, thre
                                  ctypes.c_int(threads)))
 
 
+def set_wide(mat: Handle, K: int, lanes: int):
+    """K-column scalar-mass products with one entry per lane (lanes per node
+    row 4, 8, 16 or 32; 0 drops the layout)."""
+    _check(_lib().nmg_set_wide(_i64(mat), ctypes.c_int(K), ctypes.c_int(lanes)))
+
+
 def set_row_groups(mat: Handle, per_wave: int):
     """Single-column products through row groups (per_wave groups per
     wavefront: 1, 2, 4 or 8; 0 drops them)."""
@@ -481,7 +487,7 @@ def csr_autotune(mat: Handle, K: int, reps: int = 10, fp32: bool = False):
     _check(_lib().nmg_csr_autotune(_i64(mat), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
                                    ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap),
                                    ctypes.byref(thr)))
-    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups"][kind.value],
+    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups", "wide"][kind.value],
                 rows=rows.value, cap=cap.value, threads=thr.value)
 
 

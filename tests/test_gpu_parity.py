@@ -667,3 +667,39 @@ def test_fp32_vector_bsolve(small_p2, dense_eigs):
             assert np.all(r.res2 <= 1e-4)
     finally:
         mv.free()
+
+
+def test_wide_mass_products(small_p2):
+    """The wide k-column scalar-mass layout (one entry per lane, 4 to 32
+    lanes per row) gives the products, B-solves and FP32-vector B-solves of
+    the untiled layout, for K = 2, 4, 8, 16."""
+    from normalmodes_amd.driver import setupmatvec, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg="auto", tune=())
+    rng = np.random.default_rng(53)
+
+    def products(X, prec):
+        h = _open_solver(mv, precision=prec, seed=7, rtol=1e-9, split=0)
+        try:
+            return [pv.apply_block(h, which, X) for which in (1, 2)]
+        finally:
+            pv.pevsl_finish(h)
+
+    try:
+        for K in (2, 4, 8, 16):
+            X = rng.standard_normal((K, small_p2.n))
+            pv.set_tiling(mv.matB, K, 0, 1)
+            ref = products(X, pv.FP64)
+            ref32 = products(X, pv.FP32_BSOLVE)
+            for G in (4, 8, 16, 32):
+                pv.set_wide(mv.matB, K, G)
+                for which, (Y, Yr) in enumerate(zip(products(X, pv.FP64), ref)):
+                    assert _rel(Y, Yr) < 1e-14, (K, G, which)
+                Y32 = products(X, pv.FP32_BSOLVE)
+                assert _rel(Y32[0], ref32[0]) < 1e-14, (K, G)
+                assert _rel(Y32[1], ref32[1]) < 1e-6, (K, G)
+            pv.set_wide(mv.matB, K, 0)
+        with pytest.raises(pv.PevslError):
+            pv.set_wide(mv.matA, 4, 8)   # 3x3 blocks: not a scalar mass
+    finally:
+        mv.free()

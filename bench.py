@@ -164,7 +164,8 @@ def spmv_rates(name, reps=50):
     t_build = time.perf_counter() - t0
     mv = driver.upload(P, tune=(1,))
     driver.setup_bsolve(mv, bdeg="auto")
-    out = dict(workload=name, n_dof=P.n, nnz_A=P.A.nnz, nnz_B=P.B.nnz, build_s=round(t_build, 2))
+    out = dict(workload=name, n_dof=P.n, nnz_A=P.A.nnz, nnz_B=P.B.nnz, build_s=round(t_build, 2),
+               kernel_layouts=mv.tuned)
     try:
         for key, mat in (("A_bsr3", mv.matA), ("B_mass3", mv.matB)):
             for fp32 in (False, True):

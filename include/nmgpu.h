@@ -220,6 +220,12 @@ int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
  * accuracy). */
 int nmg_set_residual_target(nmg_handle h, double target);
 
+/* With nmg_set_split_streams(h, p > 1) (no reference counterpart): on = 1
+ * (default) runs each block filter application as p independent chains of
+ * k/p columns, one per stream (B-solve, stiffness product and recurrence
+ * per degree), joined once per application; on = 0 splits only the
+ * B-solves and joins after each. */
+int nmg_set_split_filter(nmg_handle h, int on);
 /* Replay every B-solve (all Chebyshev steps, and the split parts of a
  * k-column solve with their fork and join) from a captured HIP graph, one
  * executable graph per distinct set of buffers, so that a solve costs one

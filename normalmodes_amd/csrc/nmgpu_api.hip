@@ -556,10 +556,12 @@ struct Solver {
   // k-column B-solves split into `split` parts on as many streams (default
   // priority): the handle's stream and split-1 more
   int split = 0;
+  bool split_filter = true;  // with split: whole filter chains per part (nmg_set_split_streams mode)
   hipStream_t sx[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DVec hb[12];
   FVec hf[4];  // FP32 work of the k-column B-solve parts (3 blocks each)
+  DVec pf[4][7];  // per-part filter work of the split block filter
   std::vector<int> slice_nev;  // pairs per slice after nmg_solve_slices
   int verbose = 0;  // progress line on stderr every `verbose` Lanczos steps (0: silent)
   // B-solves replayed from captured HIP graphs (nmg_set_graphs): one
@@ -1363,9 +1365,86 @@ static void block_filter_step(Solver& S, int K, const double* in, const double* 
   if (pr) S.probe[1].after(S.s);
 }
 
+// the parts of a split filter: the columns of K split into P groups of
+// K/P, each filtered as its own chain (B-solve, product, recurrence) on its
+// own stream; 1 when the filter is not split
+static int filter_parts(const Solver& S, int K) {
+  const int P = S.split;
+  return (P > 1 && K % P == 0 && K / P >= 2 && !S.borrowed && !S.fs && S.geneig) ? P : 1;
+}
+
+// block_filter with the columns split into P independent chains: the chains
+// share only the matrices, so each part's stiffness product overlaps the
+// other parts' B-solves and no per-degree split, merge or join is needed
+static void block_filter_split(Solver& S, int K, int P, const PolyFilter& Pf, const double* mu_dev,
+                               const double* z, double* y) {
+  const int H = K / P;
+  const int64_t nh = S.n * H;
+  Cheb& C = *S.bsol;
+  if (!S.sx[0]) {
+    for (int q = 0; q < 3; ++q) NMG_HIP(hipStreamCreateWithFlags(&S.sx[q], hipStreamNonBlocking));
+    for (int q = 0; q < 4; ++q) NMG_HIP(hipEventCreateWithFlags(&S.ev[q], hipEventDisableTiming));
+  }
+  const bool v32 = S.fp32v() && !C.persistent(S.fp32());
+  double* zq[4];
+  double* yq[4];
+  double* mq[4];
+  for (int q = 0; q < P; ++q) {
+    for (int i = 0; i < 6; ++i) S.pf[q][i].ensure(nh);
+    S.pf[q][6].ensure((int64_t)(Pf.deg + 1) * H);
+    zq[q] = S.pf[q][0].p;
+    yq[q] = S.pf[q][1].p;
+    mq[q] = S.pf[q][6].p;
+    if (v32) S.hf[q].ensure(3 * nh);
+  }
+  k_split_parts(S.n, K, P, z, zq, S.s);
+  k_split_parts(Pf.deg + 1, K, P, mu_dev, mq, S.s);  // per-part filter coefficients
+  NMG_HIP(hipEventRecord(S.ev[0], S.s));
+  for (int q = 1; q < P; ++q) NMG_HIP(hipStreamWaitEvent(S.sx[q - 1], S.ev[0], 0));
+  // the host issues the chains degree by degree, alternating between the
+  // streams, so that every stream always has queued work
+  hipStream_t sq[4];
+  double* vk[4];
+  double* vkm[4];
+  for (int q = 0; q < P; ++q) {
+    sq[q] = q == 0 ? S.s : S.sx[q - 1];
+    vk[q] = S.pf[q][2].p;
+    vkm[q] = S.pf[q][3].p;
+    k_copy(nh, zq[q], vk[q], sq[q]);
+    k_block_colscale(S.n, H, mq[q], vk[q], yq[q], sq[q]);
+  }
+  for (int k = 1; k <= Pf.deg; ++k) {
+    const double t = (k == 1 ? 1.0 : 2.0) / Pf.dd;
+    for (int q = 0; q < P; ++q) {
+      double* b1 = S.pf[q][4].p;
+      double* b2 = S.pf[q][5].p;
+      block_cheb(C, H, vk[q], b1, b2, sq[q], S.fp32(), q == 0 ? &S.probe[0] : nullptr,
+                 v32 ? S.hf[q].p : nullptr);
+      const bool pr = q == 0 && S.probe[1].armed();
+      if (pr) S.probe[1].before(sq[q]);
+      k_block_filter_step(H, S.A(), b1, vk[q], vkm[q], yq[q], mq[q] + (size_t)k * H, t, Pf.cc, k == 1, sq[q],
+                          false);
+      if (pr) S.probe[1].after(sq[q]);
+      std::swap(vk[q], vkm[q]);
+    }
+  }
+  for (int q = 1; q < P; ++q) {
+    NMG_HIP(hipEventRecord(S.ev[q], S.sx[q - 1]));
+    NMG_HIP(hipStreamWaitEvent(S.s, S.ev[q], 0));
+  }
+  k_merge_parts(S.n, K, P, yq, y, S.s);
+  C.ncalls += (int64_t)K * Pf.deg;
+  C.nmatvec += (int64_t)C.deg * K * Pf.deg;
+  S.filter_applies += K;
+}
+
 // y = p((A B^-1 - cc)/dd) z for K interleaved columns; mu_dev: (deg+1)*K
 static void block_filter(Solver& S, int K, const PolyFilter& P, const double* mu_dev, const double* z,
                          double* y) {
+  if (S.split_filter) {
+    const int parts = filter_parts(S, K);
+    if (parts > 1) return block_filter_split(S, K, parts, P, mu_dev, z, y);
+  }
   const int64_t nk = S.n * K;
   for (int i = 0; i < 4; ++i) S.bw[i].ensure(nk);
   double* vk = S.bw[0].p;
@@ -2465,6 +2544,14 @@ int nmg_set_split_streams(nmg_handle h, int parts) {
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(parts >= 0 && parts <= 4 && parts != 3, NMG_EINVAL, "parts must be 0, 1, 2 or 4");
     S->split = parts;
+  });
+}
+
+int nmg_set_split_filter(nmg_handle h, int on) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(on == 0 || on == 1, NMG_EINVAL, "on must be 0 or 1");
+    S->split_filter = on != 0;
   });
 }
 

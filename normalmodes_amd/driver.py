@@ -261,7 +261,7 @@ def split_parts(block: int, split="auto") -> int:
 
 
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split="auto", graphs=False, verbose=0):
+                 split="auto", graphs=False, verbose=0, split_filter=True):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -284,6 +284,8 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
         parts = split_parts(block, split)
         if parts > 1:
             pv.set_split_streams(h, parts)
+            if not split_filter:
+                pv.set_split_filter(h, False)
         if graphs:
             pv.set_graphs(h, True)
         if verbose:
@@ -305,7 +307,8 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7,
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
-                keep: bool = False, split="auto", graphs: bool = False, verbose: int = 0) -> EigResult:
+                keep: bool = False, split="auto", graphs: bool = False, verbose: int = 0,
+                split_filter: bool = True) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
@@ -314,7 +317,8 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
     t = {}
     t0 = time.perf_counter()
     h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every, block=block,
-                     resid_target=resid_target, split=split, graphs=graphs, verbose=verbose)
+                     resid_target=resid_target, split=split, graphs=graphs, verbose=verbose,
+                     split_filter=split_filter)
     try:
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()

@@ -329,6 +329,12 @@ def set_verbose(h: Handle, every: int):
     _check(_lib().nmg_set_verbose(_i64(h), ctypes.c_int(int(every))))
 
 
+def set_split_filter(h: Handle, on: bool = True):
+    """With split streams, run whole filter chains per part (default) or
+    split only the B-solves."""
+    _check(_lib().nmg_set_split_filter(_i64(h), ctypes.c_int(int(bool(on)))))
+
+
 def set_graphs(h: Handle, on: bool = True):
     """Replay B-solves from captured HIP graphs (one host launch per solve)."""
     _check(_lib().nmg_set_graphs(_i64(h), ctypes.c_int(int(bool(on)))))

@@ -703,3 +703,42 @@ def test_wide_mass_products(small_p2):
             pv.set_wide(mv.matA, 4, 8)   # 3x3 blocks: not a scalar mass
     finally:
         mv.free()
+
+
+def test_split_filter_chains(small_p2, dense_eigs):
+    """Block filters run as independent per-stream chains of k/2 columns
+    (the default with two streams) against B-solve-only splitting and one
+    stream: the same eigenvalues, FP64 and FP32 values; and spectrum slices
+    through the same path."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl, apply_slices, _open_solver
+    pv = _pv()
+    mv = setupmatvec(small_p2, bdeg="auto", tune=(1, 4, 8))
+    try:
+        w = dense_eigs(small_p2)
+        ref = w[(w >= 1.0) & (w <= 12.0)]
+        r0 = apply_pevsl(mv, interval=(1.0, 12.0), block=8, split=0, vectors=False)
+        r1 = apply_pevsl(mv, interval=(1.0, 12.0), block=8, split=2, vectors=False)
+        assert r1.lam.size == ref.size and np.max(np.abs(r1.lam - ref) / ref) < 1e-12
+        assert np.max(np.abs(r1.lam - r0.lam) / r0.lam) < 1e-13 and np.all(r1.res2 <= 1e-10)
+        r2 = apply_pevsl(mv, interval=(1.0, 12.0), block=8, split=2, vectors=False, precision=pv.FP32_FILTER)
+        assert r2.lam.size == ref.size and np.max(np.abs(r2.lam - ref) / ref) < 1e-12
+        # B-solve-only splitting through the switch gives the same pairs
+        h = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, block=8, split=2)
+        try:
+            pv.set_split_filter(h, False)
+            lmin, lmax = pv.pevsl_lanbounds(h, 3000, 5000, 1e-5)
+            pol = pv.pevsl_findpol([1.0, 12.0, lmin, lmax], 0.8, 0.7)
+            pv.pevsl_cheblannr(h, [1.0, 12.0, lmin, lmax], 9624, 1e-5, pol)
+            lam, _ = pv.pevsl_copy_result(h, small_p2.n, False)
+            pv.pevsl_freepol(pol)
+        finally:
+            pv.pevsl_finish(h)
+        lam = np.sort(lam)
+        assert lam.size == ref.size and np.max(np.abs(lam - ref) / ref) < 1e-12
+        # five slices run at width 8, so the slice filter takes the chains too
+        # (wide band: narrow slices low in the spectrum need very high degrees)
+        ref60 = w[(w >= 1.0) & (w <= 60.0)]
+        s = apply_slices(mv, interval=(1.0, 60.0), n_slices=5, slicer="kpm", kpm_nvec=16, vectors=False)
+        assert s.lam.size == ref60.size and np.max(np.abs(s.lam - ref60) / ref60) < 1e-12
+    finally:
+        mv.free()

@@ -80,8 +80,8 @@ def upload_with_layouts(P, layouts, tune):
     kind 'lanes:<g>' (K = 1, g lanes per node row), 'rows:<r>' (r node rows
     per wavefront), 'tiled:<cap>:<r>[:<threads>]' (LDS tiles of at most cap
     nodes, workgroups of 1024 or 256 threads), 'groups:<g>' (K = 1, row
-    groups, g per wavefront) or 'wide:<g>' (K >= 2, scalar mass, g lanes per
-    node row)."""
+    groups, g per wavefront), 'wide:<g>' (K >= 2, scalar mass, g lanes per
+    node row) or 'twide:<cap>:<g>' (the same on LDS tiles)."""
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
@@ -102,6 +102,8 @@ def upload_with_layouts(P, layouts, tune):
             pv.set_row_groups(mat, int(f[2]))
         elif f[1] == "wide":
             pv.set_wide(mat, K, int(f[2]))
+        elif f[1] == "twide":
+            pv.set_tiled_wide(mat, K, int(f[2]), int(f[3]))
         else:
             raise ValueError(part)
         mv.tuned[f[0]] = {"fixed": ":".join(f[1:])}

@@ -310,6 +310,13 @@ int nmg_set_value_order(nmg_handle mat, int tiled);
  * products (0 = by row length, else 8/16/32/64) and, for K > 0, rows per
  * wavefront of its K-column products (0 = default, 1, 2, 4 or 8). */
 int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave);
+/* K-column scalar-mass products through LDS tiles of at most max_nodes
+ * nodes (as nmg_set_tiling, 1024-thread workgroups) formed in the wide
+ * manner (no reference counterpart): one entry per lane, `lanes` lanes per
+ * node row (4, 8 or 16), the K columns in registers, x read from the tile's
+ * LDS copy.  max_nodes <= 0 drops the tiling.  The autotuner (kind 5) sets
+ * it when fastest. */
+int nmg_set_tiled_wide(nmg_handle mat, int K, int max_nodes, int lanes);
 /* K-column products of a scalar-mass matrix (MASS3) through the wide layout
  * (no reference counterpart): one matrix entry per lane with the K columns
  * in registers and each neighbour's 3K values read by 16-byte loads,
@@ -339,9 +346,9 @@ int nmg_tiling_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int m
  * K columns on the device (lanes per row; rows per wavefront; LDS tiles with
  * capacities on a geometric ladder up to 64 KB, workgroups of 1024 or 256
  * threads; for K = 1 also row groups) and keeps the fastest.  kind: 0
- * lanes-per-row, 1 rows per wavefront, 2 tiled, 3 row groups, 4 wide
- * (scalar mass, K >= 2); rows: lanes per row (kinds 0 and 4), rows per
- * wavefront (1, 2) or groups per wavefront (3);
+ * lanes-per-row, 1 rows per wavefront, 2 tiled, 3 row groups, 4 wide,
+ * 5 tiled wide (4 and 5: scalar mass, K >= 2); rows: lanes per row (kinds
+ * 0, 4, 5), rows per wavefront (1, 2) or groups per wavefront (3);
  * cap: tile node capacity and threads: tile workgroup size (kind 2). */
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double *seconds, int *kind, int *rows,
                      int *cap, int *threads);

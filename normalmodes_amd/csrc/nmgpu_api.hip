@@ -334,7 +334,7 @@ void build_row_groups(DevMat& M, int per_wave) {
   M.rgroups = G.release();
 }
 
-void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block) {
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block, int wide) {
   int idx = -1;
   for (int i = 0; i < 5; ++i)
     if ((1 << i) == K) idx = i;
@@ -352,6 +352,9 @@ void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block)
   T->rows_per_wave = rows_per_wave > 0 ? rows_per_wave : 1;
   NMG_CHECK(block == 256 || block == 1024, NMG_EINVAL, "tile workgroups have 256 or 1024 threads");
   T->block = block;
+  NMG_CHECK(wide == 0 || ((wide == 4 || wide == 8 || wide == 16) && K >= 2 && M.layout == L_MASS3 && block == 1024),
+            NMG_EINVAL, "tiled wide rows: 4, 8 or 16 lanes per row, K >= 2, scalar mass, 1024 threads");
+  T->wide = wide;
   T->staged_nodes = (double)H.tcols.size();
   T->trp = dmalloc<int64_t>(H.trp.size());
   T->tcp = dmalloc<int64_t>(H.tcp.size());
@@ -397,7 +400,8 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->nt = S.nt;
   M->tiled_vals = S.tiled_vals;
   for (int i = 0; i < 5; ++i)
-    if (S.tiling[i]) build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave, S.tiling[i]->block);
+    if (S.tiling[i])
+      build_tiling(*M, 1 << i, S.tiling[i]->cap, S.tiling[i]->rows_per_wave, S.tiling[i]->block, S.tiling[i]->wide);
   if (S.rgroups) build_row_groups(*M, S.rgroups->per_wave);
   return M.release();
 }
@@ -2762,7 +2766,7 @@ static double time_launches(hipStream_t s, int reps, F&& f) {
 struct TuneResult {
   double seconds = 1e30;
   int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block, 3 row groups (K = 1),
-                 // 4 wide (scalar mass, K >= 2)
+                 // 4 wide (scalar mass, K >= 2), 5 tiled wide (scalar mass, K >= 2)
   int group = 0, rows = 1, cap = 0;
   int64_t ntiles = 0;
   int block = 1024;  // tile workgroup threads
@@ -2829,6 +2833,12 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
                  M.tiling[idx] ? M.tiling[idx]->ntiles : 0, tb);
       }
     }
+    if (M.layout == L_MASS3 && K >= 2)
+      for (int G : {4, 8, 16}) {
+        build_tiling(M, K, cap, 1, 1024, G);
+        consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 5, 0, G, cap,
+                 M.tiling[idx] ? M.tiling[idx]->ntiles : 0, 1024);
+      }
   }
   // apply the winner
   build_tiling(M, K, 0, 1);
@@ -2839,6 +2849,8 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
     build_row_groups(M, best.rows);
   } else if (best.kind == 4) {
     M.wide[idx] = best.rows;
+  } else if (best.kind == 5) {
+    build_tiling(M, K, best.cap, 1, 1024, best.rows);
   } else if (best.kind == 1) {
     M.block_map[idx] = best.rows;
     if (K == 1) M.single_blocked = true;
@@ -2956,6 +2968,17 @@ int nmg_set_tiling(nmg_handle mat, int K, int max_nodes, int rows_per_wave, int 
       M->single_blocked = max_nodes > 0 || M->block_map[0] > 0;
       build_row_groups(*M, 0);
     }
+  });
+}
+
+int nmg_set_tiled_wide(nmg_handle mat, int K, int max_nodes, int lanes) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 2, 4, 8 or 16");
+    NMG_CHECK(max_nodes * 24 * K <= 64 * 1024, NMG_EINVAL, "tile exceeds 64 KB of LDS");
+    build_tiling(*M, K, max_nodes, 1, 1024, max_nodes > 0 ? lanes : 0);
+    for (int i = 0; i < 5; ++i)
+      if ((1 << i) == K) M->wide[i] = 0;
   });
 }
 

@@ -45,6 +45,7 @@ struct Tiling {
   int cap = 0;            // node capacity the tiles were cut for
   int rows_per_wave = 1;  // R of the lane layout
   int block = 1024;       // threads of the workgroup sharing one tile (1024 or 256)
+  int wide = 0;           // scalar mass, K >= 2: G lanes per row, one entry per lane (0: rows layout)
   int64_t* trp = nullptr;
   int64_t* tcp = nullptr;
   int32_t* tcols = nullptr;
@@ -148,7 +149,7 @@ constexpr int tile_stride(int K) { return 3 * K; }
 // largest tile node capacity within 64 KB of LDS
 inline int tile_cap_limit(int K) { return 64 * 1024 / (8 * tile_stride(K)); }
 // builds (or, with max_nodes <= 0, drops) the row tiling of M for K columns
-void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024);
+void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block = 1024, int wide = 0);
 // creates a DevMat from host CSR (0-based); picks the layout
 DevMat* make_devmat(int64_t nrow, int64_t ncol, const int64_t* rowptr, const int32_t* col,
                     const double* val, bool allow_blocked = true);

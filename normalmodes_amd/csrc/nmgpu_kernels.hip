@@ -1543,6 +1543,103 @@ __global__ __launch_bounds__(TB) void tiled_block_rows(const int64_t* __restrict
   }
 }
 
+// Tiled wide form (scalar mass, K >= 2): the tile's x blocks are staged in
+// LDS as in tiled_block_rows (each node once per tile, contiguous 3K-value
+// reads), and the rows are formed as in mass3_wide_rows: one entry per lane,
+// G lanes per row, the K columns in registers, 16-byte LDS reads.  The x
+// gathers then come from LDS instead of L2.  TB = 1024 threads for K <= 4;
+// 512 for K >= 8, whose 3K accumulators per lane need more registers.
+template <int K, int G, class VT, class Epi, class XT, int TB>
+__global__ __launch_bounds__(TB) void tiled_wide_rows(const int64_t* __restrict__ trp,
+                                                        const int64_t* __restrict__ tcp,
+                                                        const int32_t* __restrict__ tcols,
+                                                        const int64_t* __restrict__ rp,
+                                                        const uint16_t* __restrict__ lcol,
+                                                        const VT* __restrict__ val, const XT* __restrict__ x,
+                                                        Epi epi, int flags) {
+  extern __shared__ double2 xs2[];  // 16-byte aligned node blocks
+  double* xs = reinterpret_cast<double*>(xs2);
+  constexpr int W3 = 3 * K;
+  static_assert(W3 % 2 == 0, "node blocks of whole double2");
+  constexpr int Q = (W3 + G - 1) / G;
+  const int64_t tile = xcd_block(flags);
+  const int64_t r0 = trp[tile], r1 = trp[tile + 1];
+  const int64_t c0 = tcp[tile];
+  const int nu = (int)(tcp[tile + 1] - c0);
+  for (int t = threadIdx.x; t < nu * W3; t += TB) {
+    const int node = t / W3, j = t - node * W3;
+    xs[t] = (double)x[(int64_t)tcols[c0 + node] * W3 + j];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane % G;
+  for (int64_t rb = r0 + (int64_t)wave * (64 / G); rb < r1; rb += (int64_t)(TB / 64) * (64 / G)) {
+    const int64_t row = rb + lane / G;
+    const bool live = row < r1;
+    typename Epi::Pre pre[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = g + q * G;
+      if (live && j < W3) pre[q] = epi.load(3 * row + j / K, j % K);
+    }
+    double acc[W3];
+#pragma unroll
+    for (int j = 0; j < W3; ++j) acc[j] = 0.0;
+    if (live) {
+      const int64_t end = rp[row + 1];
+      for (int64_t k = rp[row] + g; k < end; k += G) {
+        const double2* xb = xs2 + (int)ldm(lcol + k, flags) * (W3 / 2);
+        const double m = (double)ldm(val + k, flags);
+#pragma unroll
+        for (int j = 0; j < W3 / 2; ++j) {
+          const double2 v = xb[j];
+          acc[2 * j] += m * v.x;
+          acc[2 * j + 1] += m * v.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < W3; ++j) acc[j] = group_sum<G>(acc[j]);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = g + q * G;
+      double v = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < W3; ++jj)
+        if (jj == j) v = acc[jj];
+      if (live && j < W3) epi.store(3 * row + j / K, j % K, v, pre[q]);
+    }
+  }
+}
+
+template <int K, class VT, class Epi, class XT>
+static bool launch_tiled_wide(const DevMat& M, const Tiling& T, const VT* v, const XT* x, Epi epi,
+                              hipStream_t s) {
+  if constexpr (K < 2) {
+    return false;
+  } else {
+    if (M.layout != L_MASS3 || T.wide <= 0) return false;
+    constexpr int TB = K >= 8 ? 512 : 1024;
+    const size_t shmem = (size_t)T.max_nodes * 3 * K * sizeof(double);
+    switch (T.wide) {
+      case 4:
+        hipLaunchKernelGGL((tiled_wide_rows<K, 4, VT, Epi, XT, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
+                           T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, x, epi, kflags(M));
+        break;
+      case 8:
+        hipLaunchKernelGGL((tiled_wide_rows<K, 8, VT, Epi, XT, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
+                           T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, x, epi, kflags(M));
+        break;
+      default:
+        hipLaunchKernelGGL((tiled_wide_rows<K, 16, VT, Epi, XT, TB>), dim3((unsigned)T.ntiles), dim3(TB), shmem, s,
+                           T.trp, T.tcp, T.tcols, M.browptr, T.lcol, v, x, epi, kflags(M));
+        break;
+    }
+    return true;
+  }
+}
+
 template <int K, int R, class VT, class Epi, int TB, class XT>
 static void launch_tiled_RB(const DevMat& M, const Tiling& T, const VT* v, const XT* x, Epi epi,
                             hipStream_t s) {
@@ -1568,6 +1665,10 @@ bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_
   while ((1 << idx) != K) ++idx;
   const Tiling* T = M.tiling[idx];
   if (!T || T->ntiles <= 0 || M.layout == L_CSR) return false;
+  if (launch_tiled_wide<K>(M, *T, v, x, epi, s)) {
+    NMG_HIP(hipGetLastError());
+    return true;
+  }
   const int R = T->rows_per_wave;
   if (R == 2 && K * 2 <= 64) launch_tiled_R<K, (K * 2 <= 64 ? 2 : 1)>(M, *T, v, x, epi, s);
   else if (R == 4 && K * 4 <= 64) launch_tiled_R<K, (K * 4 <= 64 ? 4 : 1)>(M, *T, v, x, epi, s);

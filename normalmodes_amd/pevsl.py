@@ -475,6 +475,12 @@ def set_tiling(mat: Handle, K: int, max_nodes: int, rows_per_wave: int = 1, thre
                                  ctypes.c_int(threads)))
 
 
+def set_tiled_wide(mat: Handle, K: int, max_nodes: int, lanes: int):
+    """K-column scalar-mass products on LDS tiles of at most max_nodes nodes,
+    one entry per lane (lanes per row 4, 8 or 16); max_nodes <= 0 drops it."""
+    _check(_lib().nmg_set_tiled_wide(_i64(mat), ctypes.c_int(K), ctypes.c_int(max_nodes), ctypes.c_int(lanes)))
+
+
 def set_wide(mat: Handle, K: int, lanes: int):
     """K-column scalar-mass products with one entry per lane (lanes per node
     row 4, 8, 16 or 32; 0 drops the layout)."""
@@ -493,7 +499,7 @@ def csr_autotune(mat: Handle, K: int, reps: int = 10, fp32: bool = False):
     _check(_lib().nmg_csr_autotune(_i64(mat), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
                                    ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap),
                                    ctypes.byref(thr)))
-    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups", "wide"][kind.value],
+    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups", "wide", "tiled_wide"][kind.value],
                 rows=rows.value, cap=cap.value, threads=thr.value)
 
 

@@ -671,8 +671,8 @@ def test_fp32_vector_bsolve(small_p2, dense_eigs):
 
 def test_wide_mass_products(small_p2):
     """The wide k-column scalar-mass layout (one entry per lane, 4 to 32
-    lanes per row) gives the products, B-solves and FP32-vector B-solves of
-    the untiled layout, for K = 2, 4, 8, 16."""
+    lanes per row), plain and on LDS tiles, gives the products, B-solves and
+    FP32-vector B-solves of the untiled layout, for K = 2, 4, 8, 16."""
     from normalmodes_amd.driver import setupmatvec, _open_solver
     pv = _pv()
     mv = setupmatvec(small_p2, bdeg="auto", tune=())
@@ -699,8 +699,20 @@ def test_wide_mass_products(small_p2):
                 assert _rel(Y32[0], ref32[0]) < 1e-14, (K, G)
                 assert _rel(Y32[1], ref32[1]) < 1e-6, (K, G)
             pv.set_wide(mv.matB, K, 0)
+            # the same on LDS tiles (two tile sizes)
+            longest = int(np.max(np.diff(small_p2.B.to_scipy().indptr)))
+            for cap in (longest, min(3 * longest, 64 * 1024 // (24 * K))):
+                for G in (4, 8, 16):
+                    pv.set_tiled_wide(mv.matB, K, cap, G)
+                    for which, (Y, Yr) in enumerate(zip(products(X, pv.FP64), ref)):
+                        assert _rel(Y, Yr) < 1e-14, ("tiled", K, cap, G, which)
+                    Y32 = products(X, pv.FP32_BSOLVE)
+                    assert _rel(Y32[1], ref32[1]) < 1e-6, ("tiled", K, cap, G)
+            pv.set_tiled_wide(mv.matB, K, 0, 0)
         with pytest.raises(pv.PevslError):
             pv.set_wide(mv.matA, 4, 8)   # 3x3 blocks: not a scalar mass
+        with pytest.raises(pv.PevslError):
+            pv.set_tiled_wide(mv.matA, 4, 64, 8)
     finally:
         mv.free()
 

@@ -230,7 +230,8 @@ int nmg_set_split_filter(nmg_handle h, int on);
  * k-column solve with their fork and join) from a captured HIP graph, one
  * executable graph per distinct set of buffers, so that a solve costs one
  * host launch instead of one per step (no reference counterpart; on = 0 or
- * 1, default 0). */
+ * 1, default 0).  A change of the B matrix's kernel layout (autotuning,
+ * tiling, value order, …) retires the graphs captured before it. */
 int nmg_set_graphs(nmg_handle h, int on);
 /* Progress of long solves (no reference counterpart): a line on stderr
  * every `every` Lanczos steps of nmg_lanbounds, nmg_cheblannr and

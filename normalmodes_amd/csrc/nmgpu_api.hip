@@ -303,6 +303,7 @@ static void compute_tiling(int64_t nb, const int64_t* rp, const int32_t* ci, int
 }
 
 void build_row_groups(DevMat& M, int per_wave) {
+  M.touch();
   delete M.rgroups;
   M.rgroups = nullptr;
   if (per_wave <= 0 || M.layout == L_CSR) return;
@@ -335,6 +336,7 @@ void build_row_groups(DevMat& M, int per_wave) {
 }
 
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block, int wide) {
+  M.touch();
   int idx = -1;
   for (int i = 0; i < 5; ++i)
     if ((1 << i) == K) idx = i;
@@ -573,7 +575,8 @@ struct Solver {
   bool graphs = false;
   struct GraphKey {
     const void* p[24];
-    int K, parts, fp32, deg;
+    uint64_t gen;  // layout generation of the matrix
+    int K, parts, fp32, deg, mode;
     double lmin, lmax;
   };
   struct GraphEntry {
@@ -583,6 +586,8 @@ struct Solver {
   };
   std::vector<GraphEntry> gcache;
   void drop_graphs() {
+    // a launched graph may still run on the stream: let it finish first
+    if (s && !gcache.empty()) (void)hipStreamSynchronize(s);
     for (auto& e : gcache)
       if (e.exec) (void)hipGraphExecDestroy(e.exec);
     gcache.clear();
@@ -664,7 +669,10 @@ static void run_graphed(Solver& S, const Solver::GraphKey& key, F&& body) {
   }
   if (!hit) {
     if (S.gcache.size() >= 16) {
-      if (S.gcache.front().exec) (void)hipGraphExecDestroy(S.gcache.front().exec);
+      if (S.gcache.front().exec) {
+        NMG_HIP(hipStreamSynchronize(S.s));  // its last launch may still run
+        (void)hipGraphExecDestroy(S.gcache.front().exec);
+      }
       S.gcache.erase(S.gcache.begin());
     }
     S.gcache.push_back(Solver::GraphEntry{key, 1, nullptr});
@@ -700,6 +708,8 @@ static Solver::GraphKey graph_key(const Solver& S, const Cheb& C, int K, int par
   k.p[2] = tmp;
   k.p[3] = &C;
   k.p[4] = C.mat;
+  k.gen = C.mat->gen;
+  k.mode = C.mode;
   for (int i = 0; i < 3 * parts && i < 12; ++i) k.p[5 + i] = S.hb[i].p;
   for (int i = 0; i < 4; ++i) k.p[17 + i] = S.hf[i].p;
   k.K = K;
@@ -2141,6 +2151,7 @@ int nmg_csr_diagscale(nmg_handle mat, double sign, double* d) {
                   "mass layout with component-dependent diagonal");
     NMG_HIP(hipMemcpyAsync(dd.p, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     scale_devmat(*M, dd.p, dd.p, sign, s);
+    M->touch();
     NMG_HIP(hipStreamSynchronize(s));
     if (d) std::memcpy(d, hd.data(), sizeof(double) * n);
   });
@@ -2698,6 +2709,7 @@ int nmg_set_xcd_order(nmg_handle mat, int on) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     M->xcd = on ? 1 : 0;
+    M->touch();
   });
 }
 
@@ -2712,6 +2724,7 @@ int nmg_set_stream_loads(nmg_handle mat, int on) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
     M->nt = on ? 1 : 0;
+    M->touch();
   });
 }
 
@@ -2721,6 +2734,7 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
     NMG_CHECK(group == 0 || group == 8 || group == 16 || group == 32 || group == 64, NMG_EINVAL,
               "group must be 0, 8, 16, 32 or 64");
     M->group = group;
+    M->touch();
     if (group > 0 || K == 1) build_row_groups(*M, 0);  // the single-column route is set explicitly
     if (K > 0) {
       int idx = -1;
@@ -2874,6 +2888,7 @@ int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds,
     hipStream_t s = default_stream();
     if (fp32 && !M->bval32) make_fp32(*M, s);
     TuneResult r = tune_matrix(*M, K, reps, fp32 != 0, s);
+    M->touch();
     if (seconds) *seconds = r.seconds;
     if (kind) *kind = r.kind;
     if (rows) *rows = r.kind == 0 ? r.group : r.rows;
@@ -2993,6 +3008,7 @@ int nmg_set_wide(nmg_handle mat, int K, int lanes) {
               "lanes per row must be 0, 4, 8, 16 or 32");
     NMG_CHECK(lanes == 0 || M->layout == L_MASS3, NMG_EINVAL, "the wide layout applies to scalar-mass matrices");
     M->wide[idx] = lanes;
+    M->touch();
   });
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -65,8 +66,21 @@ struct RowGroups {
   ~RowGroups();
 };
 
+// Process-wide source of layout generations (unique across matrices, so a
+// matrix freed and re-created at the same address never repeats one).
+inline uint64_t next_layout_gen() {
+  static std::atomic<uint64_t> g{0};
+  return ++g;
+}
+
 struct DevMat {
   int64_t nrow = 0, ncol = 0, nnz = 0;  // scalar (DoF-level) sizes
+  // changes with every change of the device buffers or of the kernel
+  // mapping below; captured HIP graphs of products on this matrix are keyed
+  // on it, so a retuned or re-tiled matrix is never replayed with stale
+  // kernels or freed buffers
+  uint64_t gen = next_layout_gen();
+  void touch() { gen = next_layout_gen(); }
   int layout = L_CSR;
   // CSR (any matrix): rowptr[nrow+1], col[nnz], val[nnz]
   int64_t* rowptr = nullptr;

@@ -913,6 +913,7 @@ __global__ void to_f32_k(int64_t n, const double* __restrict__ x, float* __restr
 }
 
 void make_fp32(DevMat& M, hipStream_t s) {
+  M.touch();
   if (M.layout == L_CSR) {
     if (!M.val32) NMG_HIP(hipMalloc(&M.val32, sizeof(float) * (M.nnz > 0 ? M.nnz : 1)));
     hipLaunchKernelGGL(to_f32_k, dim3(grid_for(M.nnz)), dim3(kBlock), 0, s, M.nnz, M.val, M.val32);
@@ -2042,6 +2043,7 @@ void set_value_order(DevMat& M, int tiled, hipStream_t s) {
   NMG_CHECK(M.layout == L_BSR3, NMG_EINVAL, "the value order applies to 3x3-block matrices");
   tiled = tiled ? 1 : 0;
   if (tiled == M.tiled_vals) return;
+  M.touch();
   const int64_t ntiles = 9 * ((M.nbnz + 63) / 64 * 64);
   const int64_t nplanes = 9 * M.nbnz;
   double* out = nullptr;

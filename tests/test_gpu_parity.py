@@ -528,7 +528,8 @@ def test_graph_replayed_bsolves(small_p2, dense_eigs):
     """B-solves replayed from captured HIP graphs: four calls in a row on
     different inputs (eager first use, capture on the second, replays after),
     each against the same call without graphs, for one stream and for the
-    2-stream split; then whole solves with graphs, single-vector and block 8."""
+    2-stream split, and again after the B products are given new tiles; then
+    whole solves with graphs, single-vector and block 8."""
     from normalmodes_amd.driver import setupmatvec, apply_pevsl, _open_solver
     pv = _pv()
     mv = setupmatvec(small_p2, bdeg="auto", tune=(1, 4, 8))
@@ -540,6 +541,16 @@ def test_graph_replayed_bsolves(small_p2, dense_eigs):
             graph = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9, split=parts, graphs=True)
             try:
                 for X in Xs:
+                    ref = pv.apply_block(eager, 2, X)
+                    got = pv.apply_block(graph, 2, X)
+                    assert _rel(got, ref) < 1e-15, parts
+                # new LDS tiles for the B products of the widths in use (8
+                # columns, 4 per part when split): the graphs captured above
+                # hold the old kernels and tile buffers and must not replay
+                longest = int(np.diff(small_p2.B.rowptr).max())
+                for K in (4, 8):
+                    pv.set_tiling(mv.matB, K, 2 * longest, 1)
+                for X in Xs[:3]:
                     ref = pv.apply_block(eager, 2, X)
                     got = pv.apply_block(graph, 2, X)
                     assert _rel(got, ref) < 1e-15, parts

@@ -1,6 +1,6 @@
 """Fluid-solid Schur operator (sparsefsAV) on the device.
 
-The fluid assembly is not part of this build, so the pencil is synthetic
+The finite-element producer assembles solids only, so the pencil is synthetic
 but has the reference's block structure (src/mod_matvec.f90:98-222): Ad
 (solid stiffness), Ap = -(pressure mass) negative definite, ET coupling
 pressure to displacements on the fluid nodes' stencils, E = ET^T.  The

@@ -83,29 +83,44 @@ def upload_with_layouts(P, layouts, tune):
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
+    specs = parse_layouts(layouts)
     mv = driver.upload(P, tune=())
-    for part in layouts.split(","):
-        f = part.strip().split(":")
-        mat = mv.matA if f[0][0] == "A" else mv.matB
-        K = int(f[0][1:] or 1)
-        if f[1] == "lanes":
-            pv.set_kernel_map(mat, group=int(f[2]))
-        elif f[1] == "rows":
-            pv.set_kernel_map(mat, 0, K, int(f[2]))
+    for name, K, kind, p in specs:
+        mat = mv.matA if name == "A" else mv.matB
+        if kind == "lanes":
+            pv.set_kernel_map(mat, group=p[0])
+        elif kind == "rows":
+            pv.set_kernel_map(mat, 0, K, p[0])
             if K == 1:
                 pv.set_tiling(mat, 1, 0, 1)   # route single products through the row-block kernels
-        elif f[1] == "tiled":
-            pv.set_tiling(mat, K, int(f[2]), int(f[3]), int(f[4]) if len(f) > 4 else 1024)
-        elif f[1] == "groups":
-            pv.set_row_groups(mat, int(f[2]))
-        elif f[1] == "wide":
-            pv.set_wide(mat, K, int(f[2]))
-        elif f[1] == "twide":
-            pv.set_tiled_wide(mat, K, int(f[2]), int(f[3]))
+        elif kind == "tiled":
+            pv.set_tiling(mat, K, p[0], p[1], p[2] if len(p) > 2 else 1024)
+        elif kind == "groups":
+            pv.set_row_groups(mat, p[0])
+        elif kind == "wide":
+            pv.set_wide(mat, K, p[0])
         else:
-            raise ValueError(part)
-        mv.tuned[f[0]] = {"fixed": ":".join(f[1:])}
+            pv.set_tiled_wide(mat, K, p[0], p[1])
+        mv.tuned[f"{name}{K}"] = {"fixed": ":".join([kind] + [str(v) for v in p])}
     return mv
+
+
+_LAYOUT_ARGS = {"lanes": (1,), "rows": (1,), "tiled": (2, 3), "groups": (1,), "wide": (1,), "twide": (2,)}
+
+
+def parse_layouts(spec):
+    """'<A|B><K>:<kind>:<params>,...' -> [(matrix, K, kind, [ints])]; raises
+    ValueError on anything else, before any device work."""
+    out = []
+    for part in spec.split(","):
+        f = part.strip().split(":")
+        if len(f) < 2 or not f[0] or f[0][0] not in "AB" or f[1] not in _LAYOUT_ARGS:
+            raise ValueError(f"bad layout {part!r}")
+        K = int(f[0][1:] or 1)
+        if K not in (1, 2, 4, 8, 16) or len(f) - 2 not in _LAYOUT_ARGS[f[1]]:
+            raise ValueError(f"bad layout {part!r}")
+        out.append((f[0][0], K, f[1], [int(v) for v in f[2:]]))
+    return out
 
 
 def kernel_only(args):

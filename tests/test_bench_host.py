@@ -21,10 +21,35 @@ def test_traffic_of_each_dominant_kernel_is_committed(tag):
     """Block 8 on two streams (4 columns a part), block 8 on one stream and
     the single vector: each has a FETCH_SIZE/WRITE_SIZE summary whose tag the
     bench finds."""
-    if str(ROOT) not in sys.path:
-        sys.path.insert(0, str(ROOT))
-    import bench
+    bench = _bench()
     traffic, src = bench.traffic_from_profiles(tag)
     assert traffic is not None and traffic > 1e6, tag
     assert (ROOT / "profiles" / src).is_file()
     assert bench.traffic_from_profiles(tag + "x") == (None, None)
+
+
+def _bench():
+    if str(ROOT) not in sys.path:
+        sys.path.insert(0, str(ROOT))
+    import bench
+    return bench
+
+
+def test_profile_commands_use_valid_layout_specs():
+    """Every fixed-layout command recorded under profiles/ parses with the
+    bench's current layout syntax, so the recorded commands can be rerun."""
+    import re
+    bench = _bench()
+    specs = []
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        specs += re.findall(r"--layouts\s+(\S+)", f.read_text())
+    assert specs
+    for spec in specs:
+        parsed = bench.parse_layouts(spec)
+        assert parsed and all(name in "AB" and K in (1, 2, 4, 8, 16) for name, K, _, _ in parsed), spec
+
+
+@pytest.mark.parametrize("spec", ["A:32", "C1:lanes:16", "A1:tiled:231", "A3:rows:1", "B4:wide", "B8:twide:200"])
+def test_bad_layout_specs_are_refused(spec):
+    with pytest.raises(ValueError):
+        _bench().parse_layouts(spec)

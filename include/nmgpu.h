@@ -39,6 +39,9 @@ enum {
 /* precision of the Chebyshev filter (nmg_set_precision) */
 enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1, NMG_FP32_BSOLVE = 2 };
 
+/* Error message of the last failing call on this thread, library version,
+ * and the device's name, CU count and HBM size (no reference counterpart:
+ * pEVSL reports no errors). */
 const char *nmg_last_error(void);
 int nmg_version(int *major, int *minor);
 int nmg_device_info(char *name, int name_len, int *cu_count, double *hbm_gbytes);
@@ -60,11 +63,15 @@ int nmg_setprobsizes(nmg_handle h, int64_t n_glob, int64_t n_loc, int64_t nfirst
  * x I3, or CSR) is chosen from its structure. */
 int nmg_parcsr_create(int nrow_g, int ncol_g, const int *row_starts, const int *col_starts,
                       const int *ia, const int *ja, const double *a, int comm, nmg_handle *mat);
-/* the same with int64 row offsets (nnz beyond 2^31) */
+/* the same with int64 row offsets, for nnz beyond 2^31 (no reference
+ * counterpart: its rowdist is int32, src/mod_cg_datatype.f90:40) */
 int nmg_csr_create(int64_t nrow, int64_t ncol, const int64_t *rowptr, const int32_t *col,
                    const double *val, nmg_handle *mat);
+/* releases the device copy (no reference counterpart: the reference never
+ * frees its ParCSR handles before PEVSL_FINISH_F90) */
 int nmg_csr_free(nmg_handle mat);
-/* layout chosen at creation: 0 CSR, 1 3x3-block CSR, 2 scalar mass (x) I3 */
+/* layout chosen at creation: 0 CSR, 1 3x3-block CSR, 2 scalar mass (x) I3
+ * (no reference counterpart) */
 int nmg_csr_info(nmg_handle mat, int64_t *nrow, int64_t *ncol, int64_t *nnz, int *layout);
 /* pevsl_parcsrmatvec_f90(x, y, mat)  src/mod_matvec.f90:453,471,495,507,509,515 */
 int nmg_parcsr_matvec(const double *x, double *y, nmg_handle mat);
@@ -116,6 +123,8 @@ int nmg_setup_chebiter(double lmin, double lmax, int deg, nmg_handle mat, nmg_ha
 int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
 /* pEVSL_chebiterstatsprint_f90(cheb)  src/mod_pevsl.f90:132-135 */
 int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
+/* releases a B-solve (no reference counterpart: chebB lives until the end
+ * of the run) */
 int nmg_chebiter_free(nmg_handle cheb);
 /* How the solve runs (no reference counterpart): mode 0 (default) launches
  * one fused kernel per step; mode 1 runs all deg steps in one persistent
@@ -125,7 +134,8 @@ int nmg_chebiter_free(nmg_handle cheb);
  * measured, because of the barrier cost). */
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
 /* persistent: 1 if the persistent solve is planned and selected, 2 if it
- * could be selected (mode 0 now), 0 if it does not apply; grid, LDS bytes */
+ * could be selected (mode 0 now), 0 if it does not apply; grid, LDS bytes
+ * (no reference counterpart) */
 int nmg_chebiter_info(nmg_handle cheb, int fp32, int *persistent, int *grid, int *lds_bytes);
 
 /* --- eigensolver ------------------------------------------------------- */
@@ -134,8 +144,9 @@ int nmg_chebiter_info(nmg_handle cheb, int fp32, int *persistent, int *grid, int
 int nmg_lanbounds(nmg_handle h, int mlan, int maxit, double tol, double *lmin, double *lmax);
 /* pEVSL_FINDPOL_F90(xintv, thr_int, thr_ext, pol)  src/mod_pevsl.f90:115 */
 int nmg_findpol(const double *xintv, double thr_int, double thr_ext, nmg_handle *pol);
-/* polynomial parameters: degree, damping type, centre/half-width of the
- * spectrum map, delta centre gam, acceptance bar; mu: f64[deg+1] or NULL */
+/* polynomial parameters of a FindPol result: degree, centre/half-width of
+ * the spectrum map, delta centre gam, acceptance bar; mu: f64[deg+1] or
+ * NULL (no reference counterpart: POL is opaque to the reference) */
 int nmg_pol_info(nmg_handle pol, int *deg, double *cc, double *dd, double *gam, double *bar,
                  double *mu);
 /* pEVSL_FREEPOL_F90(pol)  src/mod_pevsl.f90:215 */
@@ -186,7 +197,8 @@ int nmg_backward_grav(nmg_handle h, int64_t n_elem, int np, const int32_t *elems
  * cc = (lmax+lmin)/2, dd = (lmax-lmin)/2.  ecnt: Jackson-damped estimate of
  * the number of eigenvalues in [a, b]. */
 int nmg_kpmdos(nmg_handle h, int mdeg, int nvec, const double *intv, double *mu, double *ecnt);
-/* Jackson-damped eigenvalue count in [intv[0], intv[1]] from the moments. */
+/* Jackson-damped eigenvalue count in [intv[0], intv[1]] from the moments
+ * (pEVSL's count behind spslicer; not called by the reference). */
 int nmg_dos_count(const double *mu, int mdeg, const double *intv, double *count);
 /* pEVSL spslicer: boundaries sli[0] = a < ... < sli[n_slices] = b with equal
  * estimated counts (bisection of the closed-form count). */
@@ -201,15 +213,18 @@ int nmg_spslicer(double *sli, const double *mu, int mdeg, const double *intv, in
  * nmg_backward return, slice by slice. */
 int nmg_solve_slices(nmg_handle h, int n_slices, const double *sli, double lmin, double lmax, double thr_int,
                      double thr_ext, int maxit, double tol);
-/* pairs per slice of the last nmg_solve_slices (int[n_slices]). */
+/* pairs per slice of the last nmg_solve_slices (int[n_slices]; no
+ * reference counterpart). */
 int nmg_slice_counts(nmg_handle h, int *counts);
 
 /* Tuning/diagnostics of the last nmg_cheblannr / nmg_lanbounds call:
- * Lanczos steps, filter-operator applications, candidate count, seconds. */
+ * Lanczos steps, filter-operator applications, candidate count, seconds
+ * (no reference counterpart). */
 int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *ncand,
                     double *seconds);
 /* Convergence controls of nmg_cheblannr (defaults: ntest 30, cycle 20,
- * rtol 1e-9 on the filtered Ritz residual). */
+ * rtol 1e-9 on the filtered Ritz residual).  No reference counterpart: the
+ * reference passes only TOL = 1e-5 (src/mod_pevsl.f90:52, 122). */
 int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
 
 /* Residual target of nmg_cheblannr (default 5e-11; 0 = stop on the
@@ -260,7 +275,8 @@ int nmg_set_kernel_probe(nmg_handle h, int every);
 int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *samples,
                      double *bytes_per_launch);
 
-/* --- device-vector access (for tests and benchmarks) ------------------- */
+/* --- device-vector access (for tests and benchmarks; none of the entries
+ * below has a reference counterpart) ------------------------------------ */
 /* One application of the filter operator of the handle:
  * y = p((A B^-1 - cc)/dd) z (z-space of ChebLanNr) with pol; host vectors. */
 int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double *z, double *y);

@@ -79,7 +79,8 @@ def upload_with_layouts(P, layouts, tune):
     per wavefront), 'tiled:<cap>:<r>[:<threads>]' (LDS tiles of at most cap
     nodes, workgroups of 1024 or 256 threads), 'groups:<g>' (K = 1, row
     groups, g per wavefront), 'wide:<g>' (K >= 2, scalar mass, g lanes per
-    node row) or 'twide:<cap>:<g>' (the same on LDS tiles)."""
+    node row), 'twide:<cap>:<g>' (the same on LDS tiles) or 'pipe:<g>:<r>'
+    (K = 1, 3x3 blocks, g lanes walking r rows)."""
     from normalmodes_amd import driver, pevsl as pv
     if layouts == "auto":
         return driver.upload(P, tune=tune)
@@ -99,13 +100,16 @@ def upload_with_layouts(P, layouts, tune):
             pv.set_row_groups(mat, p[0])
         elif kind == "wide":
             pv.set_wide(mat, K, p[0])
+        elif kind == "pipe":
+            pv.set_row_pipeline(mat, p[0], p[1])
         else:
             pv.set_tiled_wide(mat, K, p[0], p[1])
         mv.tuned[f"{name}{K}"] = {"fixed": ":".join([kind] + [str(v) for v in p])}
     return mv
 
 
-_LAYOUT_ARGS = {"lanes": (1,), "rows": (1,), "tiled": (2, 3), "groups": (1,), "wide": (1,), "twide": (2,)}
+_LAYOUT_ARGS = {"lanes": (1,), "rows": (1,), "tiled": (2, 3), "groups": (1,), "wide": (1,), "twide": (2,),
+                "pipe": (2,)}
 
 
 def parse_layouts(spec):

@@ -349,6 +349,14 @@ int nmg_set_wide(nmg_handle mat, int K, int lanes);
  * Takes precedence over the other single-column layouts; the autotuner
  * (kind 3) sets it when it is the fastest. */
 int nmg_set_row_groups(nmg_handle mat, int per_wave);
+/* Single-column products of a 3x3-block matrix through pipelined lane
+ * groups (no reference counterpart): `lanes` lanes (16, 32 or 64) walk
+ * `rows` consecutive node rows (2, 4 or 8) one entry per lane, loading the
+ * columns and values of row j+1 before row j's x gathers, so each row waits
+ * on one memory latency instead of three in sequence.  rows = 0 drops it;
+ * nmg_set_kernel_map clears it.  The autotuner (kind 6) sets it when
+ * fastest. */
+int nmg_set_row_pipeline(nmg_handle mat, int lanes, int rows);
 /* LDS-tiled K-column products: rows cut into tiles whose referenced nodes
  * (at most max_nodes, <= 64 KB of x in LDS) are staged once per tile by a
  * workgroup of `threads` (256 or 1024; 0 = 1024); max_nodes = 0 removes the
@@ -364,9 +372,11 @@ int nmg_tiling_host(int64_t nb, const int64_t *rowptr, const int32_t *col, int m
  * capacities on a geometric ladder up to 64 KB, workgroups of 1024 or 256
  * threads; for K = 1 also row groups) and keeps the fastest.  kind: 0
  * lanes-per-row, 1 rows per wavefront, 2 tiled, 3 row groups, 4 wide,
- * 5 tiled wide (4 and 5: scalar mass, K >= 2); rows: lanes per row (kinds
- * 0, 4, 5), rows per wavefront (1, 2) or groups per wavefront (3);
- * cap: tile node capacity and threads: tile workgroup size (kind 2). */
+ * 5 tiled wide (4 and 5: scalar mass, K >= 2), 6 pipelined lanes-per-row
+ * (3x3 blocks, K = 1); rows: lanes per row (kinds 0, 4, 5), rows per
+ * wavefront (1, 2), groups per wavefront (3) or rows per lane group (6);
+ * cap: tile node capacity (kinds 2, 5) or lanes per group (6); threads:
+ * tile workgroup size (kind 2). */
 int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double *seconds, int *kind, int *rows,
                      int *cap, int *threads);
 

@@ -398,6 +398,7 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
     cp(S.bval, M->bval, S.value_count());
   }
   M->single_blocked = S.single_blocked;
+  M->pipe = S.pipe;
   M->xcd = S.xcd;
   M->nt = S.nt;
   M->tiled_vals = S.tiled_vals;
@@ -2734,6 +2735,7 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
     NMG_CHECK(group == 0 || group == 8 || group == 16 || group == 32 || group == 64, NMG_EINVAL,
               "group must be 0, 8, 16, 32 or 64");
     M->group = group;
+    M->pipe = 0;
     M->touch();
     if (group > 0 || K == 1) build_row_groups(*M, 0);  // the single-column route is set explicitly
     if (K > 0) {
@@ -2780,7 +2782,8 @@ static double time_launches(hipStream_t s, int reps, F&& f) {
 struct TuneResult {
   double seconds = 1e30;
   int kind = 0;  // 0 lanes-per-row (K = 1), 1 untiled block, 2 tiled block, 3 row groups (K = 1),
-                 // 4 wide (scalar mass, K >= 2), 5 tiled wide (scalar mass, K >= 2)
+                 // 4 wide (scalar mass, K >= 2), 5 tiled wide (scalar mass, K >= 2),
+                 // 6 pipelined lanes-per-row (BSR3, K = 1)
   int group = 0, rows = 1, cap = 0;
   int64_t ntiles = 0;
   int block = 1024;  // tile workgroup threads
@@ -2822,6 +2825,7 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   }
   if (K == 1) {
     M.single_blocked = false;
+    M.pipe = 0;
     for (int pw : {1, 2, 4, 8}) {
       build_row_groups(M, pw);
       consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 3, 0, pw, 0, 0);
@@ -2831,6 +2835,14 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
       M.group = g;
       consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 0, g, 1, 0, 0);
     }
+    if (M.layout == L_BSR3)
+      for (int g : {16, 32, 64})
+        for (int pr : {2, 4, 8}) {
+          M.group = g;
+          M.pipe = pr;
+          consider(time_launches(s, reps, [&] { k_spmv(M, x.p, y.p, s, fp32); }), 6, g, pr, 0, 0);
+        }
+    M.pipe = 0;
     M.group = 0;
   }
   for (int R : {1, 2, 4, 8}) {
@@ -2869,6 +2881,9 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   M.block_map[idx] = 0;
   if (best.kind == 0) {
     M.group = best.group;
+  } else if (best.kind == 6) {
+    M.group = best.group;
+    M.pipe = best.rows;
   } else if (best.kind == 3) {
     build_row_groups(M, best.rows);
   } else if (best.kind == 4) {
@@ -2902,7 +2917,7 @@ int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds,
     if (seconds) *seconds = r.seconds;
     if (kind) *kind = r.kind;
     if (rows) *rows = r.kind == 0 ? r.group : r.rows;
-    if (cap) *cap = r.cap;
+    if (cap) *cap = r.kind == 6 ? r.group : r.cap;
     if (threads) *threads = r.kind == 2 ? r.block : 0;
   });
 }
@@ -3018,6 +3033,22 @@ int nmg_set_wide(nmg_handle mat, int K, int lanes) {
               "lanes per row must be 0, 4, 8, 16 or 32");
     NMG_CHECK(lanes == 0 || M->layout == L_MASS3, NMG_EINVAL, "the wide layout applies to scalar-mass matrices");
     M->wide[idx] = lanes;
+    M->touch();
+  });
+}
+
+int nmg_set_row_pipeline(nmg_handle mat, int lanes, int rows) {
+  return guarded([&] {
+    DevMat* M = get<DevMat>(mat, K_MAT);
+    NMG_CHECK(rows == 0 || rows == 2 || rows == 4 || rows == 8, NMG_EINVAL, "rows per lane group must be 0, 2, 4 or 8");
+    NMG_CHECK(rows == 0 || lanes == 16 || lanes == 32 || lanes == 64, NMG_EINVAL, "lanes per group must be 16, 32 or 64");
+    NMG_CHECK(rows == 0 || M->layout == L_BSR3, NMG_EINVAL, "pipelined rows apply to 3x3-block matrices");
+    M->pipe = rows;
+    if (rows > 0) {  // the single-column products take the lanes-per-row route
+      M->group = lanes;
+      M->single_blocked = false;
+      build_row_groups(*M, 0);
+    }
     M->touch();
   });
 }

@@ -105,6 +105,7 @@ struct DevMat {
   int wide[5] = {0, 0, 0, 0, 0};
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
   RowGroups* rgroups = nullptr; // single-column products through the row-group kernels (first choice)
+  int pipe = 0;                 // BSR3 single-column rows per pipelined lane group (2, 4, 8; 0 = off)
   int xcd = 1;                  // XCD-contiguous workgroup order of the row kernels (0: plain order)
   int nt = 0;                   // non-temporal matrix loads in the row kernels
   // BSR3 value order: 0 planes (value q of entry k at q*nbnz + k); 1 tiles of

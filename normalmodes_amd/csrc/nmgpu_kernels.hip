@@ -161,6 +161,84 @@ __global__ __launch_bounds__(kBlock) void bsr3_rows(int64_t nb, int64_t nbnz,
   if (wr) epi.store(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2), pre);
 }
 
+// Single-column BSR3 rows walked by pipelined lane groups: G lanes take NR
+// consecutive node rows in order, one entry per lane.  The columns and
+// values of row j+1 are loaded before row j's x gathers are issued, so a row
+// waits on one memory latency (its gathers, with the next row's matrix loads
+// in flight beside them) instead of three in sequence (row offsets, then
+// columns and values, then x).  The NR+1 row offsets of the group are read
+// once, one per lane.  Entries of a row beyond the first G are finished
+// unpipelined.
+template <int G, int NR, class VT, class Epi>
+__global__ __launch_bounds__(kBlock) void bsr3_rows_pipe(int64_t nb, int64_t nbnz,
+                                                         const int64_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ci,
+                                                         const VT* __restrict__ bv,
+                                                         const double* __restrict__ x, Epi epi, int flags) {
+  static_assert(G > NR, "one lane per row offset of the group");
+  const int64_t t = xcd_block(flags) * kBlock + threadIdx.x;
+  const int64_t row0 = (t / G) * NR;
+  const int lane = threadIdx.x & (G - 1);
+  if (row0 >= nb) return;  // the whole lane group leaves together
+  const int nr = (int)(nb - row0 < NR ? nb - row0 : NR);
+  const int64_t off = lane <= nr ? rp[row0 + lane] : 0;  // rp[row0 .. row0 + nr], nb + 1 entries
+  int64_t e = __shfl(off, 1, G);
+  int64_t k = __shfl(off, 0, G) + lane;
+  int32_t c = 0;
+  double v[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) v[q] = 0.0;
+  if (k < e) {
+    c = ldm(ci + k, flags);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] = (double)ldm(bv + bidx(k, q, nbnz, flags), flags);
+  }
+  for (int j = 0; j < nr; ++j) {
+    const int64_t row = row0 + j;
+    typename Epi::Pre pre{};
+    if (lane < 3) pre = epi.load(3 * row + lane);
+    // row j+1 spans [e, e1); past the last row lane nr holds e, so e1 = e
+    const int64_t e1 = __shfl(off, j + 2 < nr ? j + 2 : nr, G);
+    const int64_t k1 = e + lane;
+    int32_t c1 = 0;
+    double v1[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v1[q] = 0.0;
+    if (k1 < e1) {
+      c1 = ldm(ci + k1, flags);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v1[q] = (double)ldm(bv + bidx(k1, q, nbnz, flags), flags);
+    }
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (k < e) {
+      const int64_t c3 = 3 * (int64_t)c;
+      const double x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
+      y0 = v[0] * x0 + v[1] * x1 + v[2] * x2;
+      y1 = v[3] * x0 + v[4] * x1 + v[5] * x2;
+      y2 = v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    for (int64_t kk = k + G; kk < e; kk += G) {
+      const int64_t c3 = 3 * (int64_t)ldm(ci + kk, flags);
+      double w[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) w[q] = (double)ldm(bv + bidx(kk, q, nbnz, flags), flags);
+      const double x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
+      y0 += w[0] * x0 + w[1] * x1 + w[2] * x2;
+      y1 += w[3] * x0 + w[4] * x1 + w[5] * x2;
+      y2 += w[6] * x0 + w[7] * x1 + w[8] * x2;
+    }
+    y0 = group_sum<G>(y0);
+    y1 = group_sum<G>(y1);
+    y2 = group_sum<G>(y2);
+    if (lane < 3) epi.store(3 * row + lane, lane == 0 ? y0 : (lane == 1 ? y1 : y2), pre);
+    k = k1;
+    e = e1;
+    c = c1;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] = v1[q];
+  }
+}
+
 template <int G, class VT, class Epi>
 __global__ __launch_bounds__(kBlock) void mass3_rows(int64_t nb, const int64_t* __restrict__ rp,
                                                      const int32_t* __restrict__ ci,
@@ -359,6 +437,16 @@ static int pick_group(double avg) {
   return 8;
 }
 
+// pipelined BSR3 rows: ceil(nb / NR) lane groups of G lanes
+template <int G, int NR, class VT, class Epi>
+static void launch_pipe(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
+  const int64_t groups = (M.nb + NR - 1) / NR;
+  const int64_t grid = (groups * G + kBlock - 1) / kBlock;
+  if (grid <= 0) return;
+  hipLaunchKernelGGL((bsr3_rows_pipe<G, NR, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nb, M.nbnz,
+                     M.browptr, M.bcol, v, x, epi, kflags(M));
+}
+
 template <int G, class VT, class Epi>
 static void launch_G(const DevMat& M, const VT* v, const double* x, Epi epi, hipStream_t s) {
   if (M.layout == L_CSR) {
@@ -368,6 +456,16 @@ static void launch_G(const DevMat& M, const VT* v, const double* x, Epi epi, hip
       hipLaunchKernelGGL((csr_rows<G, VT, Epi>), dim3((unsigned)grid), dim3(kBlock), 0, s, M.nrow,
                          M.rowptr, M.col, v, x, epi, kflags(M));
   } else {
+    if constexpr (G >= 16) {
+      if (M.layout == L_BSR3 && M.pipe > 0) {
+        switch (M.pipe) {
+          case 2: launch_pipe<G, 2>(M, v, x, epi, s); break;
+          case 4: launch_pipe<G, 4>(M, v, x, epi, s); break;
+          default: launch_pipe<G, 8>(M, v, x, epi, s); break;
+        }
+        return;
+      }
+    }
     const int64_t thr = M.nb * G;
     const int64_t grid = (thr + kBlock - 1) / kBlock;
     if (grid <= 0) return;

@@ -487,6 +487,13 @@ def set_wide(mat: Handle, K: int, lanes: int):
     _check(_lib().nmg_set_wide(_i64(mat), ctypes.c_int(K), ctypes.c_int(lanes)))
 
 
+def set_row_pipeline(mat: Handle, lanes: int, rows: int):
+    """Single-column products of a 3x3-block matrix through pipelined lane
+    groups: `lanes` lanes (16, 32, 64) walk `rows` consecutive node rows (2,
+    4, 8), loading row j+1's entries before row j's gathers; rows=0 drops it."""
+    _check(_lib().nmg_set_row_pipeline(_i64(mat), ctypes.c_int(lanes), ctypes.c_int(rows)))
+
+
 def set_row_groups(mat: Handle, per_wave: int):
     """Single-column products through row groups (per_wave groups per
     wavefront: 1, 2, 4 or 8; 0 drops them)."""
@@ -499,7 +506,8 @@ def csr_autotune(mat: Handle, K: int, reps: int = 10, fp32: bool = False):
     _check(_lib().nmg_csr_autotune(_i64(mat), ctypes.c_int(K), ctypes.c_int(reps), ctypes.c_int(int(fp32)),
                                    ctypes.byref(s), ctypes.byref(kind), ctypes.byref(rows), ctypes.byref(cap),
                                    ctypes.byref(thr)))
-    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups", "wide", "tiled_wide"][kind.value],
+    return dict(seconds=s.value, kind=["lanes_per_row", "rows_per_wave", "tiled", "row_groups", "wide", "tiled_wide",
+                                      "pipelined_rows"][kind.value],
                 rows=rows.value, cap=cap.value, threads=thr.value)
 
 

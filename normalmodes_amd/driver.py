@@ -316,6 +316,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         a, b = interval
     t = {}
     t0 = time.perf_counter()
+    pol_live = False
     h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every, block=block,
                      resid_target=resid_target, split=split, graphs=graphs, verbose=verbose,
                      split_filter=split_filter)
@@ -330,6 +331,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         xintv = np.array([a, b, lmin, lmax])
         t0 = time.perf_counter()
         pol = pv.pevsl_findpol(xintv, thr_int, thr_ext)       # :115
+        pol_live = True
         pinfo = pv.pol_info(pol)
         t["findpol"] = time.perf_counter() - t0
         nev = EVINT + 2
@@ -354,8 +356,11 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         t0 = time.perf_counter()
         lam, X = pv.pevsl_copy_result(h, mv.n, vectors)       # :124-130
         t["copy_result"] = time.perf_counter() - t0
+        pol_live = False
         pv.pevsl_freepol(pol)                                 # :215
     except BaseException:
+        if pol_live:
+            pv.pevsl_freepol(pol)
         pv.pevsl_finish(h)
         raise
     if not keep:

@@ -227,8 +227,8 @@ int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *
  * reference passes only TOL = 1e-5 (src/mod_pevsl.f90:52, 122). */
 int nmg_set_lanczos_controls(nmg_handle h, int ntest, int cycle, double rtol);
 
-/* Residual target of nmg_cheblannr (default 5e-11; 0 = stop on the
- * reference's trace test alone).  Once the trace test and the filtered Ritz
+/* Residual target of nmg_cheblannr (no reference counterpart; default
+ * 5e-11; 0 = stop on the reference's trace test alone).  Once the trace test and the filtered Ritz
  * residuals pass, the pairs are extracted and the iteration stops when every
  * ||A~x - lam B~x||_2/|lam| (x B~-normalised) is at or below the target, or
  * when that residual stops improving (its floor is set by the B-solve

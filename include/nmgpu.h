@@ -222,6 +222,14 @@ int nmg_slice_counts(nmg_handle h, int *counts);
  * (no reference counterpart). */
 int nmg_solve_stats(nmg_handle h, int *lan_steps, int64_t *filter_applies, int *ncand,
                     double *seconds);
+/* Outcome of the residual stop of the last nmg_cheblannr / nmg_solve_slices
+ * (no reference counterpart; the reference stops on its trace test alone,
+ * src/mod_pevsl.f90:122): target_met = 1 when every returned pair met the
+ * residual target (or the interval held no pair), 0 when the iteration
+ * stopped because the residual no longer improved; resid_max is the largest
+ * ||A~x - lam B~x||_2/|lam| of the returned pairs measured at that stop (0
+ * when the target is 0 and no residual was measured). */
+int nmg_solve_quality(nmg_handle h, int *target_met, double *resid_max);
 /* Convergence controls of nmg_cheblannr (defaults: ntest 30, cycle 20,
  * rtol 1e-9 on the filtered Ritz residual).  No reference counterpart: the
  * reference passes only TOL = 1e-5 (src/mod_pevsl.f90:52, 122). */

@@ -430,3 +430,584 @@ int nm_element_matrices(int64_t n_sel, int np, const double* nodes, const int32_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fluid-solid producer: the blocks Ad, B, E, ET, Ap of CGFSE3D_ISO
+// (src/mod_cg_create_matrix.f90:103-977) on the DoF layout of
+// matrixstruct_general (:1458-2033).
+//
+// Node status (src/mod_geometry.f90:283-312, 672-680): 0 solid, 1 fluid,
+// 2 fluid-solid vertex; P2 edge nodes 3/4/5 from the smaller status of their
+// two vertices.  Statuses 2 and 5 carry six displacement DoFs, the solid
+// triple first and the fluid triple second (:1467-1473); every status other
+// than 0 and 3 carries one pressure DoF (:1581-1613).  DoFs are numbered node
+// by node (vstt), pressures in node order (pstt).
+//
+// Rows of node v (adjacency = nodes sharing an element with v, sorted):
+//   pure fluid  Ad/B/E: fluid triples / component / pressure of every neighbour;
+//               Ap/ET: pressure / fluid triple of every neighbour.
+//   fluid-solid solid rows: solid triples of solid and fluid-solid neighbours,
+//               E: pressures of fluid-solid neighbours; fluid rows: fluid
+//               triples / pressures of neighbours with a pressure; Ap:
+//               pressures of those; ET: every displacement DoF of those.
+//   pure solid  Ad/B: solid triples of every neighbour; E: empty.
+// Because vstt and pstt increase with the node number, these lists come out
+// sorted when the adjacency is.
+//
+// Element terms (detJ, invJ, D_i and the least-squares gradients as in the
+// solid element above; rho_bar the element-average density):
+//   solid element: the CGE3D_ISO stiffness (gravity included) and the mass
+//     M rho_bar detJ on each component (:287-422).
+//   fluid element (max mu < TOL, :279): mass M rho_bar detJ (:468-469);
+//     coupling C_i = (M D_i - M diag(rho g_i/lam)) detJ on (u_i, p) and its
+//     transpose on (p, u_i) (:488-505, 612-633 without gravity); pressure
+//     block -detJ diag(lam^-1/2) M diag(lam^-1/2) (:638-639); with gravity
+//     the displacement blocks detJ (Gh_i M Gm_j + Gm_i M Gh_j)/2 with
+//     Gh = diag(g/|g|), Gm_j = diag(g_j/|g| N2 rho) and
+//     N2 = grad(rho).g/rho - |g|^2 rho/lam (0 where |g| < 1e-15) (:200-249,
+//     :472-528).
+//   faces of a fluid element (:643-706), with face mass MassF, outward unit
+//     normal n and sJac = |cross|/4 (src/mod_geometry.f90:1410-1426,
+//     2280-2303): a free-surface face subtracts MassF/(mean|g.n|)/(mean rho)
+//     sJac from the pressure block; an interior face with fewer than three
+//     fluid-solid vertices adds MassF (mean g.n)(mean rho) sJac n_j^2 to each
+//     displacement block j (these cancel between the two fluid elements of a
+//     face up to rounding).
+//   interface (a fluid element with exactly three fluid-solid vertices; the
+//     face opposite the fourth, :786-953): with gravity Ad's solid triples
+//     lose rho_f sJac (n_i MassF G_j + n_j G_i MassF)/2; ET(p_m, solid u_q of
+//     n) and E(solid u_p of m, p_n) lose n_q sJac MassF(n,m) and
+//     n_p sJac MassF(m,n).
+// Each row is gathered over its node's elements in ascending element order
+// (volume terms of an element before its interface terms), so the result is
+// deterministic and E = ET^T exactly.
+namespace {
+
+inline int fs_vnum(int8_t st) { return (st == 2 || st == 5) ? 6 : 3; }
+inline bool fs_hasp(int8_t st) { return st != 0 && st != 3; }
+
+struct FsElem {
+  int kind = 0;      // 0 solid, 1 fluid
+  double detJ = 0, rbar = 0;
+  int64_t off = 0;   // element matrix in the pool
+  int fcid = -1;     // interface face of a fluid element
+  double nrm[3] = {0, 0, 0}, sj = 0;
+  int64_t scm = -1;  // interface gravity block in the pool
+};
+
+// outward unit normals and sJac of the four faces (face f opposite vertex f)
+void face_normals(const double* X /*4x3*/, double n[4][3], double sj[4]) {
+  for (int f = 0; f < 4; ++f) {
+    const double* v[3];
+    int k = 0;
+    for (int j = 0; j < 4; ++j)
+      if (j != f) v[k++] = X + 3 * j;
+    double a[3], b[3];
+    for (int c = 0; c < 3; ++c) {
+      a[c] = v[1][c] - v[0][c];
+      b[c] = v[2][c] - v[0][c];
+    }
+    double m[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double len = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    sj[f] = len / 4.0;
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      m[c] /= len;
+      s += (X[3 * f + c] - v[0][c]) * m[c];
+    }
+    for (int c = 0; c < 3; ++c) n[f][c] = s > 0 ? -m[c] : m[c];
+  }
+}
+
+// least-squares gradient of a nodal scalar over one element
+void scalar_gradient(int n, const double* xn, const double* f, double grad[3]) {
+  double xm[3] = {0, 0, 0}, fm = 0.0;
+  for (int a = 0; a < n; ++a) {
+    for (int c = 0; c < 3; ++c) xm[c] += xn[a * 3 + c];
+    fm += f[a];
+  }
+  for (int c = 0; c < 3; ++c) xm[c] /= n;
+  fm /= n;
+  double N[9] = {0}, R[3] = {0, 0, 0};
+  for (int a = 0; a < n; ++a)
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) N[r * 3 + c] += (xn[a * 3 + r] - xm[r]) * (xn[a * 3 + c] - xm[c]);
+      R[r] += (xn[a * 3 + r] - xm[r]) * (f[a] - fm);
+    }
+  double Ni[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, d;
+  inv3(N, Ni, &d);
+  for (int r = 0; r < 3; ++r) grad[r] = Ni[r * 3 + 0] * R[0] + Ni[r * 3 + 1] * R[1] + Ni[r * 3 + 2] * R[2];
+}
+
+// Fluid element matrix FT ((3np+np)^2, displacement blocks then pressure).
+// Returns 0, 1 for a flat element, 2 for a free surface without gravity.
+int fluid_element(const Elem& E, const double* X, const double* lam, const double* rho, const double* g,
+                  const double* xn, int purefluid, int nfp, const double* massF, const int32_t* fmask,
+                  const int32_t* neigh, const int8_t* vst /*4 vertex statuses*/, double* FT, FsElem& out,
+                  double* SCM) {
+  const int n = E.np, L = 3 * n, LF = L + n;
+  double B[9], Bi[9], detB;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) B[r * 3 + c] = X[(r + 1) * 3 + c] - X[c];
+  if (!inv3(B, Bi, &detB)) return 1;
+  const double detJ = detB / 8.0;
+  double invJ[9];
+  for (int k = 0; k < 9; ++k) invJ[k] = 2.0 * Bi[k];
+  double D[3][kMaxNp * kMaxNp];
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < n * n; ++a)
+      D[i][a] = invJ[i * 3 + 0] * E.Dr[0 * n * n + a] + invJ[i * 3 + 1] * E.Dr[1 * n * n + a] +
+                invJ[i * 3 + 2] * E.Dr[2 * n * n + a];
+  double rs = 0.0;
+  for (int a = 0; a < n; ++a) rs += rho[a];
+  out.kind = 1;
+  out.detJ = detJ;
+  out.rbar = rs / n;
+  for (int a = 0; a < LF * LF; ++a) FT[a] = 0.0;
+  const bool grav = g != nullptr;
+  // gravity quantities per node
+  double gh[3][kMaxNp], gm[3][kMaxNp];
+  if (grav) {
+    double gr[3];
+    scalar_gradient(n, xn, rho, gr);
+    for (int a = 0; a < n; ++a) {
+      const double* ga = g + a * 3;
+      const double ng = std::sqrt(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+      double N2 = (gr[0] * ga[0] + gr[1] * ga[1] + gr[2] * ga[2]) / rho[a] - ng * ng / lam[a] * rho[a];
+      if (ng < 1e-15 || purefluid) N2 = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        gh[i][a] = ga[i] / std::max(ng, 1e-15);
+        gm[i][a] = gh[i][a] * N2 * rho[a];
+      }
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int a = 0; a < n; ++a)
+          for (int b = 0; b < n; ++b) {
+            const double m = E.M[a * n + b];
+            FT[(i * n + a) * LF + j * n + b] = detJ * ((gh[i][a] * m * gm[j][b] + gm[i][a] * m * gh[j][b]) * 0.5);
+          }
+  }
+  // coupling C_i = M D_i (- M diag(rho g_i / lam))
+  for (int i = 0; i < 3; ++i) {
+    double C[kMaxNp * kMaxNp];
+    matmul(n, E.M, D[i], C);
+    if (grav)
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) C[a * n + b] -= E.M[a * n + b] * (rho[b] / lam[b] * g[b * 3 + i]);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        FT[(i * n + a) * LF + L + b] = C[a * n + b] * detJ;
+        FT[(L + b) * LF + i * n + a] = C[a * n + b] * detJ;
+      }
+  }
+  // pressure block
+  double li[kMaxNp];
+  for (int a = 0; a < n; ++a) li[a] = 1.0 / std::sqrt(lam[a]);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) FT[(L + a) * LF + L + b] = -detJ * (li[a] * E.M[a * n + b] * li[b]);
+  // faces
+  double nf[4][3], sjf[4];
+  face_normals(X, nf, sjf);
+  for (int f = 0; f < 4; ++f) {
+    const int32_t* fm = fmask + f * nfp;
+    const double* MF = massF + f * nfp * nfp;
+    double srho = 0.0;
+    for (int a = 0; a < nfp; ++a) srho += rho[fm[a]];
+    srho /= nfp;
+    if (neigh[f] < 0) {
+      if (!grav) return 2;
+      double sg = 0.0;
+      for (int a = 0; a < nfp; ++a) {
+        double gn = 0.0;
+        for (int c = 0; c < 3; ++c) gn -= g[fm[a] * 3 + c] * nf[f][c];
+        sg += std::sqrt(gn * gn);
+      }
+      sg /= nfp;
+      for (int a = 0; a < nfp; ++a)
+        for (int b = 0; b < nfp; ++b) FT[(L + fm[a]) * LF + L + fm[b]] -= MF[a * nfp + b] / sg / srho * sjf[f];
+    } else {
+      int cnt = 0;
+      for (int j = 0; j < 4; ++j)
+        if (j != f && vst[j] == 2) cnt++;
+      if (cnt < 3 && grav) {
+        double sg = 0.0;
+        for (int a = 0; a < nfp; ++a)
+          for (int c = 0; c < 3; ++c) sg += g[fm[a] * 3 + c] * nf[f][c];
+        sg /= nfp;
+        for (int j = 0; j < 3; ++j)
+          for (int a = 0; a < nfp; ++a)
+            for (int b = 0; b < nfp; ++b)
+              FT[(j * n + fm[a]) * LF + j * n + fm[b]] += MF[a * nfp + b] * sg * srho * sjf[f] * (nf[f][j] * nf[f][j]);
+      }
+    }
+  }
+  // interface face
+  int cnt = 0, other = -1;
+  for (int j = 0; j < 4; ++j) {
+    if (vst[j] == 2)
+      cnt++;
+    else
+      other = j;
+  }
+  out.fcid = -1;
+  if (cnt == 3) {
+    const int f = other;
+    out.fcid = f;
+    for (int c = 0; c < 3; ++c) out.nrm[c] = nf[f][c];
+    out.sj = sjf[f];
+    if (grav && SCM) {
+      const int32_t* fm = fmask + f * nfp;
+      const double* MF = massF + f * nfp * nfp;
+      double rf = 0.0;
+      for (int a = 0; a < nfp; ++a) rf += rho[fm[a]];
+      rf /= nfp;
+      const int Ls = 3 * nfp;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int a = 0; a < nfp; ++a)
+            for (int b = 0; b < nfp; ++b) {
+              const double m = MF[a * nfp + b];
+              const double t = (sjf[f] * nf[f][i] * (m * g[fm[b] * 3 + j]) +
+                                sjf[f] * nf[f][j] * (g[fm[a] * 3 + i] * m)) * 0.5;
+              SCM[(i * nfp + a) * Ls + j * nfp + b] = t * rf;
+            }
+    }
+  }
+  return 0;
+}
+
+// position of column c in the sorted row [lo, hi) of col, or -1
+inline int64_t find_col(const int32_t* col, int64_t lo, int64_t hi, int64_t c) {
+  const int32_t* p = std::lower_bound(col + lo, col + hi, (int32_t)c);
+  return (p != col + hi && *p == c) ? (p - col) : -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// DoF layout from node statuses: vstt (first displacement DoF of each node),
+// pstt (pressure DoF or -1), and the totals.
+int nm_fs_layout(int64_t n_nodes, const int8_t* vstat, int64_t* vstt, int64_t* pstt, int64_t* n_dof,
+                 int64_t* n_p) {
+  int64_t d = 0, p = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    if (vstat[v] < 0 || vstat[v] > 5) return -(int)(1 + v);
+    vstt[v] = d;
+    d += fs_vnum(vstat[v]);
+    pstt[v] = fs_hasp(vstat[v]) ? p++ : -1;
+  }
+  *n_dof = d;
+  *n_p = p;
+  return 0;
+}
+
+// Row pointers of Ad, B, E (n_dof + 1 each), ET and Ap (n_p + 1 each).
+// Returns 0, or -(1+v) for a pure fluid node with a neighbour without a
+// pressure (the reference stops there, src/mod_cg_create_matrix.f90:1690-1694),
+// or -(1+n_nodes+v) for a six-DoF node without a pressure.
+int nm_fs_rowptr(int64_t n_nodes, const int64_t* adj_ptr, const int32_t* adj, const int8_t* vstat,
+                 const int64_t* vstt, const int64_t* pstt, int64_t* ad_ptr, int64_t* b_ptr, int64_t* e_ptr,
+                 int64_t* et_ptr, int64_t* ap_ptr) {
+  ad_ptr[0] = b_ptr[0] = e_ptr[0] = et_ptr[0] = ap_ptr[0] = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    const int64_t a0 = adj_ptr[v], deg = adj_ptr[v + 1] - a0;
+    const int8_t sv = vstat[v];
+    const int64_t r = vstt[v];
+    int64_t nS = 0, nF = 0, nFS = 0;
+    for (int64_t k = 0; k < deg; ++k) {
+      const int8_t sw = vstat[adj[a0 + k]];
+      if (!fs_hasp(sw) || fs_vnum(sw) == 6) nS++;
+      if (fs_hasp(sw)) nF++;
+      if (fs_vnum(sw) == 6) nFS++;
+    }
+    auto set3 = [&](int64_t row, int64_t na, int64_t nb, int64_t ne) {
+      for (int j = 0; j < 3; ++j) {
+        ad_ptr[row + j + 1] = ad_ptr[row + j] + na;
+        b_ptr[row + j + 1] = b_ptr[row + j] + nb;
+        e_ptr[row + j + 1] = e_ptr[row + j] + ne;
+      }
+    };
+    if (fs_hasp(sv)) {
+      const int64_t p = pstt[v];
+      if (fs_vnum(sv) == 3) {
+        if (nF != deg) return -(int)(1 + v);
+        ap_ptr[p + 1] = ap_ptr[p] + deg;
+        et_ptr[p + 1] = et_ptr[p] + 3 * deg;
+        set3(r, 3 * deg, deg, deg);
+      } else {
+        set3(r, 3 * nS, nS, nFS);
+        ap_ptr[p + 1] = ap_ptr[p] + nF;
+        et_ptr[p + 1] = et_ptr[p] + 3 * (nF + nFS);
+        set3(r + 3, 3 * nF, nF, nF);
+      }
+    } else {
+      if (fs_vnum(sv) == 6) return -(int)(1 + n_nodes + v);
+      set3(r, 3 * deg, deg, 0);
+    }
+  }
+  return 0;
+}
+
+// Values (and columns) of Ad, B, E, ET, Ap.  Inputs as nm_assemble_elastic,
+// plus: efluid[n_elem] (1 = fluid element), neigh[n_elem*4] (element across
+// face f, -1 on the boundary), purefluid, nfp, massF[4*nfp*nfp] and
+// fmask[4*nfp] of the reference faces, vstat/vstt/pstt and the row pointers
+// from nm_fs_layout / nm_fs_rowptr.  g may be null (no gravity).
+// stats[4]: fluid elements with an interface face, fluid elements with four
+// fluid-solid vertices (no interface term, as in the reference), fluid
+// elements, solid elements.
+// Returns 0; -(1+e) for a flat element e; -(1+n_elem+e) for a fluid element
+// on the free surface without gravity (the reference divides by mean|g.n|).
+int nm_assemble_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                   const int32_t* vslot, const double* lam, const double* mu, const double* rho, const double* g,
+                   const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM,
+                   const double* refD, const double* massF, const int32_t* fmask, const int64_t* adj_ptr,
+                   const int32_t* adj, const int8_t* vstat, const int64_t* vstt, const int64_t* pstt,
+                   const int64_t* ad_ptr, int32_t* ad_col, double* ad_val, const int64_t* b_ptr, int32_t* b_col,
+                   double* b_val, const int64_t* e_ptr, int32_t* e_col, double* e_val, const int64_t* et_ptr,
+                   int32_t* et_col, double* et_val, const int64_t* ap_ptr, int32_t* ap_col, double* ap_val,
+                   int64_t* stats) {
+  if (np > kMaxNp || nfp > kMaxNp) return -1;
+  Elem E{np, refM, refD};
+  const int L = 3 * np, LF = L + np, Ls = 3 * nfp;
+  std::vector<FsElem> info(n_elem);
+  int64_t pool = 0, spool = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    info[e].kind = efluid[e] ? 1 : 0;
+    info[e].off = pool;
+    pool += efluid[e] ? (int64_t)LF * LF : (int64_t)L * L;
+  }
+  std::vector<double> W((size_t)pool);
+  std::vector<int8_t> vs4((size_t)n_elem * 4);
+  for (int64_t e = 0; e < n_elem; ++e)
+    for (int k = 0; k < 4; ++k) vs4[e * 4 + k] = vstat[elems[e * np + vslot[k]]];
+  // interface blocks with gravity: count first
+  if (g)
+    for (int64_t e = 0; e < n_elem; ++e) {
+      if (!efluid[e]) continue;
+      int c = 0;
+      for (int k = 0; k < 4; ++k) c += vs4[e * 4 + k] == 2;
+      if (c == 3) {
+        info[e].scm = spool;
+        spool += (int64_t)Ls * Ls;
+      }
+    }
+  std::vector<double> SC((size_t)std::max<int64_t>(spool, 1));
+  int64_t bad = -1;
+  int bad_kind = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elem; ++e) {
+    double X[12], xn[kMaxNp * 3];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) X[k * 3 + c] = nodes[(int64_t)elems[e * np + vslot[k]] * 3 + c];
+    for (int a = 0; a < np; ++a)
+      for (int c = 0; c < 3; ++c) xn[a * 3 + c] = nodes[(int64_t)elems[e * np + a] * 3 + c];
+    const double* ge = g ? g + e * np * 3 : nullptr;
+    int rc = 0;
+    if (efluid[e]) {
+      rc = fluid_element(E, X, lam + e * np, rho + e * np, ge, xn, purefluid, nfp, massF, fmask, neigh + e * 4,
+                         &vs4[e * 4], &W[info[e].off], info[e], info[e].scm >= 0 ? &SC[info[e].scm] : nullptr);
+    } else {
+      double Me[kMaxNp * kMaxNp], detJ;
+      if (!element_matrices(E, X, lam + e * np, mu + e * np, rho + e * np, ge, xn, &W[info[e].off], Me, &detJ,
+                            nullptr))
+        rc = 1;
+      double rs = 0.0;
+      for (int a = 0; a < np; ++a) rs += rho[e * np + a];
+      info[e].detJ = detJ;
+      info[e].rbar = rs / np;
+    }
+    if (rc) {
+#pragma omp critical
+      {
+        bad = e;
+        bad_kind = rc;
+      }
+    }
+  }
+  if (bad >= 0) return bad_kind == 1 ? (int)(-(1 + bad)) : (int)(-(1 + n_elem + bad));
+  if (stats) {
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
+    for (int64_t e = 0; e < n_elem; ++e) {
+      if (!efluid[e]) {
+        stats[3]++;
+        continue;
+      }
+      stats[2]++;
+      int c = 0;
+      for (int k = 0; k < 4; ++k) c += vs4[e * 4 + k] == 2;
+      if (c == 3) stats[0]++;
+      if (c == 4) stats[1]++;
+    }
+  }
+  // node -> (element, slot)
+  std::vector<int64_t> n2e_ptr(n_nodes + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e)
+    for (int a = 0; a < np; ++a) n2e_ptr[elems[e * np + a] + 1]++;
+  for (int64_t v = 0; v < n_nodes; ++v) n2e_ptr[v + 1] += n2e_ptr[v];
+  std::vector<int64_t> n2e(n2e_ptr[n_nodes]);
+  std::vector<int32_t> n2s(n2e_ptr[n_nodes]);
+  {
+    std::vector<int64_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
+    for (int64_t e = 0; e < n_elem; ++e)
+      for (int a = 0; a < np; ++a) {
+        const int64_t v = elems[e * np + a];
+        n2e[fill[v]] = e;
+        n2s[fill[v]] = a;
+        fill[v]++;
+      }
+  }
+  auto fluid3 = [&](int64_t w) { return vstt[w] + fs_vnum(vstat[w]) - 3; };
+  int64_t miss = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : miss)
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    const int64_t a0 = adj_ptr[v], deg = adj_ptr[v + 1] - a0;
+    const int8_t sv = vstat[v];
+    const bool hp = fs_hasp(sv);
+    const int64_t r0 = vstt[v], rf = fluid3(v), pv = pstt[v];
+    // columns
+    {
+      // solid rows (pure solid or the first triple of a fluid-solid node)
+      if (!hp || fs_vnum(sv) == 6) {
+        for (int j = 0; j < 3; ++j) {
+          int64_t ia = ad_ptr[r0 + j], ib = b_ptr[r0 + j], ie = e_ptr[r0 + j];
+          for (int64_t k = 0; k < deg; ++k) {
+            const int32_t w = adj[a0 + k];
+            const int8_t sw = vstat[w];
+            if (hp && !(!fs_hasp(sw) || fs_vnum(sw) == 6)) continue;
+            for (int q = 0; q < 3; ++q) {
+              ad_col[ia] = (int32_t)(vstt[w] + q);
+              ad_val[ia++] = 0.0;
+            }
+            b_col[ib] = (int32_t)(vstt[w] + j);
+            b_val[ib++] = 0.0;
+            if (hp && fs_vnum(sw) == 6) {
+              e_col[ie] = (int32_t)pstt[w];
+              e_val[ie++] = 0.0;
+            }
+          }
+        }
+      }
+      if (hp) {
+        // fluid rows (the only triple of a pure fluid node)
+        for (int j = 0; j < 3; ++j) {
+          int64_t ia = ad_ptr[rf + j], ib = b_ptr[rf + j], ie = e_ptr[rf + j];
+          for (int64_t k = 0; k < deg; ++k) {
+            const int32_t w = adj[a0 + k];
+            if (!fs_hasp(vstat[w])) continue;
+            for (int q = 0; q < 3; ++q) {
+              ad_col[ia] = (int32_t)(fluid3(w) + q);
+              ad_val[ia++] = 0.0;
+            }
+            b_col[ib] = (int32_t)(fluid3(w) + j);
+            b_val[ib++] = 0.0;
+            e_col[ie] = (int32_t)pstt[w];
+            e_val[ie++] = 0.0;
+          }
+        }
+        int64_t ip = ap_ptr[pv], it = et_ptr[pv];
+        for (int64_t k = 0; k < deg; ++k) {
+          const int32_t w = adj[a0 + k];
+          if (!fs_hasp(vstat[w])) continue;
+          ap_col[ip] = (int32_t)pstt[w];
+          ap_val[ip++] = 0.0;
+          // pure fluid rows see the fluid triple only; fluid-solid rows every DoF
+          const int64_t first = fs_vnum(sv) == 6 ? vstt[w] : fluid3(w);
+          const int64_t last = vstt[w] + fs_vnum(vstat[w]);
+          for (int64_t c = first; c < last; ++c) {
+            et_col[it] = (int32_t)c;
+            et_val[it++] = 0.0;
+          }
+        }
+      }
+    }
+    // values, element by element in ascending order
+    for (int64_t t = n2e_ptr[v]; t < n2e_ptr[v + 1]; ++t) {
+      const int64_t e = n2e[t];
+      const int m = n2s[t];
+      const FsElem& I = info[e];
+      const double* K = &W[I.off];
+      const int32_t* el = elems + e * np;
+      if (I.kind == 0) {
+        for (int nn = 0; nn < np; ++nn) {
+          const int64_t w = el[nn];
+          for (int p = 0; p < 3; ++p) {
+            const int64_t row = r0 + p;
+            for (int q = 0; q < 3; ++q) {
+              const int64_t pos = find_col(ad_col, ad_ptr[row], ad_ptr[row + 1], vstt[w] + q);
+              if (pos < 0) { miss++; continue; }
+              ad_val[pos] += K[(p * np + m) * L + q * np + nn];
+            }
+            const int64_t pb = find_col(b_col, b_ptr[row], b_ptr[row + 1], vstt[w] + p);
+            if (pb < 0) { miss++; continue; }
+            b_val[pb] += refM[m * np + nn] * I.rbar * I.detJ;
+          }
+        }
+        continue;
+      }
+      // fluid element
+      for (int nn = 0; nn < np; ++nn) {
+        const int64_t w = el[nn];
+        const int64_t wf = fluid3(w);
+        for (int p = 0; p < 3; ++p) {
+          const int64_t row = rf + p;
+          for (int q = 0; q < 3; ++q) {
+            const int64_t pos = find_col(ad_col, ad_ptr[row], ad_ptr[row + 1], wf + q);
+            if (pos < 0) { miss++; continue; }
+            ad_val[pos] += K[(p * np + m) * LF + q * np + nn];
+          }
+          const int64_t pb = find_col(b_col, b_ptr[row], b_ptr[row + 1], wf + p);
+          const int64_t pe = find_col(e_col, e_ptr[row], e_ptr[row + 1], pstt[w]);
+          if (pb < 0 || pe < 0) { miss++; continue; }
+          b_val[pb] += refM[m * np + nn] * I.rbar * I.detJ;
+          e_val[pe] += K[(p * np + m) * LF + L + nn];
+        }
+        const int64_t pa = find_col(ap_col, ap_ptr[pv], ap_ptr[pv + 1], pstt[w]);
+        if (pa < 0) { miss++; continue; }
+        ap_val[pa] += K[(L + m) * LF + L + nn];
+        for (int q = 0; q < 3; ++q) {
+          const int64_t pt = find_col(et_col, et_ptr[pv], et_ptr[pv + 1], wf + q);
+          if (pt < 0) { miss++; continue; }
+          et_val[pt] += K[(L + m) * LF + q * np + nn];
+        }
+      }
+      // interface face of this fluid element
+      if (I.fcid >= 0) {
+        const int32_t* fm = fmask + I.fcid * nfp;
+        const double* MF = massF + I.fcid * nfp * nfp;
+        int mi = -1;
+        for (int a = 0; a < nfp; ++a)
+          if (fm[a] == m) mi = a;
+        if (mi < 0) continue;
+        for (int ni = 0; ni < nfp; ++ni) {
+          const int64_t w = el[fm[ni]];
+          if (I.scm >= 0) {
+            const double* S = &SC[I.scm];
+            for (int p = 0; p < 3; ++p)
+              for (int q = 0; q < 3; ++q) {
+                const int64_t pos = find_col(ad_col, ad_ptr[r0 + p], ad_ptr[r0 + p + 1], vstt[w] + q);
+                if (pos < 0) { miss++; continue; }
+                ad_val[pos] -= S[(p * nfp + mi) * Ls + q * nfp + ni];
+              }
+          }
+          for (int q = 0; q < 3; ++q) {
+            const int64_t pt = find_col(et_col, et_ptr[pv], et_ptr[pv + 1], vstt[w] + q);
+            if (pt < 0) { miss++; continue; }
+            et_val[pt] -= I.nrm[q] * I.sj * MF[ni * nfp + mi];
+          }
+          for (int p = 0; p < 3; ++p) {
+            const int64_t pe = find_col(e_col, e_ptr[r0 + p], e_ptr[r0 + p + 1], pstt[w]);
+            if (pe < 0) { miss++; continue; }
+            e_val[pe] -= I.nrm[p] * I.sj * MF[mi * nfp + ni];
+          }
+        }
+      }
+    }
+  }
+  if (miss) return -2;
+  return 0;
+}
+
+}  // extern "C"

@@ -555,6 +555,8 @@ struct Solver {
   int64_t filter_applies = 0;
   int ncand = 0;
   double seconds = 0;
+  int target_met = 1;       // last cheblannr: every pair met resid_target (1), or it stopped on stagnation (0)
+  double resid_best = 0.0;  // max ||A~x - lam B~x||/|lam| of the returned pairs at the residual stop (0: not checked)
 
   std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
@@ -1088,29 +1090,52 @@ static void tridiag_eig_device(Solver& S, int kdim, const std::vector<double>& a
 // Residual-based stop: after the trace test has passed, extract the pairs and
 // accept when max ||A~x - lam B~x||/|lam| <= resid_target.  The residual has
 // a floor set by the B-solve accuracy; when it has not improved by 10% over
-// the last max(60, kdim/4) Lanczos vectors the iteration stops with the best
-// pairs found.  Returns true when the iteration should stop.
+// the last max(60, kdim/4) Lanczos vectors the iteration stops, and the pairs
+// are extracted again at the Lanczos dimension that gave the best residual
+// (`reextract`).  When the converged candidates yield no pair inside the
+// interval (they are neighbours whose filter value reaches bar), the
+// iteration stops with no pairs, as ChebLanNr does.  S.target_met and
+// S.resid_best record the outcome.  Returns true when the iteration should
+// stop.
 static void do_residuals(Solver& S, double* res_ref, double* res2);
 struct ResidualTracker {
   double best = 1e300;
   int best_k = 0;
 };
 static bool residual_stop(Solver& S, int kdim, const double* dC, const std::vector<double>& theta, double bar,
-                          const double* xintv, ResidualTracker& rt) {
+                          const double* xintv, ResidualTracker& rt,
+                          const std::function<void(int)>& reextract) {
   if (S.resid_target <= 0.0) return true;
   extract_ritz(S, kdim, dC, theta, bar, xintv);
-  if (S.nev == 0) return false;
+  if (S.nev == 0) {
+    S.target_met = 1;
+    S.resid_best = 0.0;
+    return true;
+  }
   std::vector<double> r2(S.nev);
   do_residuals(S, nullptr, r2.data());
   double mx = 0.0;
   for (double v : r2) mx = std::max(mx, std::isfinite(v) ? v : 1e300);
-  if (mx <= S.resid_target) return true;
+  if (mx <= S.resid_target) {
+    S.target_met = 1;
+    S.resid_best = mx;
+    return true;
+  }
   if (mx < 0.9 * rt.best) {
     rt.best = mx;
     rt.best_k = kdim;
     return false;
   }
-  return kdim - rt.best_k >= std::max(60, kdim / 4);
+  if (kdim - rt.best_k < std::max(60, kdim / 4)) return false;
+  // stagnated: the pairs of the best dimension, unless the latest are better
+  if (mx > rt.best && rt.best_k > 0 && rt.best_k < kdim) {
+    reextract(rt.best_k);
+    S.resid_best = rt.best;
+  } else {
+    S.resid_best = mx;
+  }
+  S.target_met = 0;
+  return true;
 }
 
 // ChebLanNr's test at Lanczos dimension kdim: on the reference cadence
@@ -1161,7 +1186,14 @@ static int lanczos_test(Solver& S, int kdim, const PolyFilter& P, const double* 
   std::vector<double> th;
   DVec dC;
   tridiag_eig_device(S, kdim, L.al, L.be, dC, th);
-  return residual_stop(S, kdim, dC.p, th, P.bar, xintv, L.rt) ? 2 : 0;
+  auto reextract = [&](int k) {
+    std::vector<double> al, be, th2;
+    fetch_tridiag(S, k, al, be);
+    DVec dC2;
+    tridiag_eig_device(S, k, al, be, dC2, th2);
+    extract_ritz(S, k, dC2.p, th2, P.bar, xintv);
+  };
+  return residual_stop(S, kdim, dC.p, th, P.bar, xintv, L.rt, reextract) ? 2 : 0;
 }
 
 // extraction after the loop: eigenvectors of T on the device (rocSOLVER
@@ -1185,6 +1217,8 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
   S.free_basis();
   ensure_basis(S, std::min(maxit + 1, 64), 0, maxit + 1);
   S.filter_applies = 0;
+  S.target_met = 1;
+  S.resid_best = 0.0;
   DVec hw;
   lanczos_start(S);
   LanczosTest L;
@@ -1581,6 +1615,8 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
   S.free_basis();
   ensure_basis(S, std::min(maxcols, std::max(2 * K, 64)), 0, maxcols);
   S.filter_applies = 0;
+  S.target_met = 1;
+  S.resid_best = 0.0;
   // per-column filter coefficients (all columns share the interval here)
   DVec mu_dev;
   mu_dev.ensure((int64_t)(P.deg + 1) * K);
@@ -1687,28 +1723,50 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
         tr1 += w[i];
         cnt++;
       }
-    const bool conv = cnt > 0 && std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
-    tr0 = tr1;
-    if (!conv) continue;
-    // Ritz residuals ||R_{j+1} s_last|| of the candidates
-    std::vector<double> last((size_t)K * mm);
-    for (int i = 0; i < mm; ++i)
-      NMG_HIP(hipMemcpyAsync(last.data() + (size_t)i * K, dT.p + (size_t)i * mm + (mm - K), sizeof(double) * K,
-                             hipMemcpyDeviceToHost, S.s));
-    NMG_HIP(hipStreamSynchronize(S.s));
-    double rmax = 0.0;
-    for (int i = 0; i < mm; ++i) {
-      if (w[i] < P.bar) continue;
+    // Ritz residual ||R_{j+1} s_last|| of Ritz vector i of T
+    std::vector<double> last;
+    auto ritz_resid = [&](int i) {
       double nrm = 0.0;
       for (int r = 0; r < K; ++r) {
         double t = 0.0;
         for (int c = r; c < K; ++c) t += R[(size_t)r * K + c] * last[(size_t)i * K + c];
         nrm += t * t;
       }
-      rmax = std::max(rmax, std::sqrt(nrm));
+      return std::sqrt(nrm);
+    };
+    auto fetch_last = [&]() {
+      last.resize((size_t)K * mm);
+      for (int i = 0; i < mm; ++i)
+        NMG_HIP(hipMemcpyAsync(last.data() + (size_t)i * K, dT.p + (size_t)i * mm + (mm - K), sizeof(double) * K,
+                               hipMemcpyDeviceToHost, S.s));
+      NMG_HIP(hipStreamSynchronize(S.s));
+    };
+    if (cnt == 0) {
+      // no filtered Ritz value reaches bar: once the largest one has
+      // converged, the top of p's spectrum lies below bar and the interval
+      // holds no eigenvalue (the single-vector rule)
+      tr0 = 0.0;
+      fetch_last();
+      if (ritz_resid(mm - 1) <= S.rtol) converged = true;
+      continue;
     }
+    const bool conv = std::fabs(tr1 - tr0) < tol * std::fabs(tr1);
+    tr0 = tr1;
+    if (!conv) continue;
+    fetch_last();
+    double rmax = 0.0;
+    for (int i = 0; i < mm; ++i)
+      if (w[i] >= P.bar) rmax = std::max(rmax, ritz_resid(i));
     if (rmax > S.rtol) continue;
-    if (residual_stop(S, mm, dT.p, w, P.bar, xintv, rt)) {
+    auto reextract = [&](int k) {
+      std::vector<double> Tk((size_t)k * k), wk;
+      for (int c = 0; c < k; ++c)
+        for (int r = 0; r < k; ++r) Tk[(size_t)c * k + r] = Tm[(size_t)c * mm + r];
+      DVec dTk;
+      dense_eig(S, k, Tk, dTk, wk);
+      extract_ritz(S, k, dTk.p, wk, P.bar, xintv);
+    };
+    if (residual_stop(S, mm, dT.p, w, P.bar, xintv, rt, reextract)) {
       converged = true;
       stopped_with_result = S.resid_target > 0.0;
     }
@@ -1928,7 +1986,13 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
   S.nev = total;
   NMG_HIP(hipStreamSynchronize(S.s));
   S.filter_applies = 0;
-  for (int i = 0; i < ns; ++i) S.filter_applies += sub[i]->filter_applies;
+  S.target_met = 1;
+  S.resid_best = 0.0;
+  for (int i = 0; i < ns; ++i) {
+    S.filter_applies += sub[i]->filter_applies;
+    S.target_met = std::min(S.target_met, sub[i]->target_met);
+    S.resid_best = std::max(S.resid_best, sub[i]->resid_best);
+  }
   S.lan_steps = steps;
   S.seconds = now() - t0;
 }
@@ -2544,6 +2608,14 @@ int nmg_solve_stats(nmg_handle h, int* lan_steps, int64_t* filter_applies, int* 
     if (filter_applies) *filter_applies = S->filter_applies;
     if (ncand) *ncand = S->ncand;
     if (seconds) *seconds = S->seconds;
+  });
+}
+
+int nmg_solve_quality(nmg_handle h, int* target_met, double* resid_max) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    if (target_met) *target_met = S->target_met;
+    if (resid_max) *resid_max = S->resid_best;
   });
 }
 

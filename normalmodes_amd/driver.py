@@ -16,6 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import pevsl as pv
+from .assemble import FluidProblem
 
 __all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
@@ -103,27 +104,6 @@ class FluidSetup:
             pv.pevsl_chebiter_free(self.chebAp)
         for m in (self.matAd, self.matE, self.matET, self.matAp):
             pv.csr_free(m)
-
-
-@dataclass
-class FluidProblem:
-    """A fluid-solid pencil in the reference's blocks (CGM%Ad, %E, %ET, %Ap, %B):
-    the operator is Ad - E Ap^-1 ET with Ap negative definite."""
-    Ad: object
-    E: object
-    ET: object
-    Ap: object
-    B: object
-    porder: int = 2
-    name: str = ""
-
-    @property
-    def n(self) -> int:
-        return self.Ad.n_rows
-
-    @property
-    def n_p(self) -> int:
-        return self.Ap.n_rows
 
 
 def ap_chebiter_degree(porder: int, lmin: float, lmax: float) -> int:
@@ -242,6 +222,7 @@ class EigResult:
     slices: np.ndarray | None = None        # slice boundaries (apply_slices)
     slice_counts: np.ndarray | None = None  # pairs per slice
     dos_count: float | None = None          # KPM estimate of the count in the interval
+    target_met: bool = True                 # every pair met the residual target at the stop
 
     @property
     def freq_mhz(self):
@@ -341,6 +322,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
         pv.pevsl_cheblannr(h, xintv, maxit, tol, pol)         # :122
         t["cheblannr"] = time.perf_counter() - t0
         st = pv.solve_stats(h)
+        target_met, _ = pv.solve_quality(h)
         probe = {}
         if probe_every > 0:
             for key, which in (("chebstep", 0), ("filterstep", 1)):
@@ -370,7 +352,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                      res2=r2[order], lmin=lmin, lmax=lmax, xintv=xintv, deg=pinfo.deg,
                      bar=pinfo.bar, lan_steps=st["lan_steps"], filter_applies=st["filter_applies"],
                      ncand=st["ncand"], timings=t, lan_steps_bounds=lb_steps, probe=probe,
-                     handle=h if keep else 0)
+                     handle=h if keep else 0, target_met=target_met)
 
 
 def kpm_degree(xintv, n_slices: int, lo: int = 50, hi: int = 3000) -> int:
@@ -429,6 +411,7 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
         pv.solve_slices(h, sli, lmin, lmax, thr_int, thr_ext, maxit, tol)
         t["slices"] = time.perf_counter() - t0
         st = pv.solve_stats(h)
+        target_met, _ = pv.solve_quality(h)
         counts = pv.slice_counts(h, n_slices)
         probe = {}
         if probe_every > 0:
@@ -453,7 +436,7 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
                      lmin=lmin, lmax=lmax, xintv=xintv, deg=0, bar=float("nan"), lan_steps=st["lan_steps"],
                      filter_applies=st["filter_applies"], ncand=st["ncand"], timings=t,
                      lan_steps_bounds=lb_steps, probe=probe, handle=h if keep else 0, slices=sli,
-                     slice_counts=counts, dos_count=ecnt)
+                     slice_counts=counts, dos_count=ecnt, target_met=target_met)
 
 
 def solve_input(inputdir, basename, *, porder: int = 1, job: int = 1, lowfreq: float = 0.2, upfreq: float = 2.0,

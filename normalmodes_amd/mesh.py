@@ -89,9 +89,25 @@ def h_for_tets(n_tets: int, radius: float = 1.0) -> float:
     return float((vol * 6.74 / n_tets) ** (1.0 / 3.0))
 
 
+# interface spheres: sampled at INTERFACE_SPACING * h, lattice points kept at
+# least INTERFACE_BAND * h away (measured: with these values Delaunay keeps
+# every interface triangle as a tetrahedron face from ~1k tets on)
+INTERFACE_SPACING = 0.6
+INTERFACE_BAND = 0.7
+
+
 def ball_mesh(n_tets: int | None = None, *, h: float | None = None, porder: int = 2, seed: int = 1019,
-              jitter: float = 0.25, radius: float = 1.0, axes=(1.0, 1.0, 1.0)) -> TetMesh:
-    """Jittered-lattice Delaunay mesh of a ball (or, with ``axes``, an ellipsoid)."""
+              jitter: float = 0.25, radius: float = 1.0, axes=(1.0, 1.0, 1.0), interfaces=(),
+              scale: float = 1.0) -> TetMesh:
+    """Jittered-lattice Delaunay mesh of a ball (or, with ``axes``, an ellipsoid).
+
+    ``interfaces``: radii (fractions of ``radius``) of internal spheres, such
+    as the boundaries of a fluid outer core.  Points are placed on each
+    sphere and kept off a band around it, so that every tetrahedron lies on
+    one side: the interface is a surface of tetrahedron faces, as in the
+    reference's input meshes.  A mesh that fails this check raises.
+    ``scale`` multiplies the coordinates at the end (e.g. 6371 for a body in
+    km with the layer radii given as fractions)."""
     from scipy.spatial import Delaunay
 
     if h is None:
@@ -103,12 +119,22 @@ def ball_mesh(n_tets: int | None = None, *, h: float | None = None, porder: int 
     g = np.arange(-n, n + 1, dtype=np.float64) * h
     P = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
     P = P + rng.uniform(-jitter * h, jitter * h, size=P.shape)
-    P = P[np.linalg.norm(P, axis=1) < radius - 0.55 * h]
+    rP = np.linalg.norm(P, axis=1)
+    keep = rP < radius - 0.55 * h
+    radii = [float(f) * radius for f in interfaces]
+    for R in radii:
+        keep &= np.abs(rP - R) > INTERFACE_BAND * h
+    P = P[keep]
+    parts = [P]
+    for R in radii:
+        hs = INTERFACE_SPACING * h
+        parts.append(R * _fibonacci_sphere(max(12, int(round(4.0 * np.pi * R * R / (hs * hs * np.sqrt(3.0) / 2.0)))),
+                                           rng))
     ns = max(12, int(round(4.0 * np.pi * radius ** 2 / (h * h * np.sqrt(3.0) / 2.0))))
     S = radius * _fibonacci_sphere(ns, rng)
-    X = np.concatenate([P, S], axis=0)
+    X = np.concatenate(parts + [S], axis=0)
     on_surf = np.zeros(X.shape[0], dtype=bool)
-    on_surf[P.shape[0]:] = True
+    on_surf[X.shape[0] - S.shape[0]:] = True
 
     T = Delaunay(X, qhull_options="Qbb Qc Qz Q12").simplices.astype(np.int64)
     vol = _tet_volumes(X, T)
@@ -128,10 +154,25 @@ def ball_mesh(n_tets: int | None = None, *, h: float | None = None, porder: int 
     X = X[used]
     on_surf = on_surf[used]
     T = remap[T]
+    if radii:
+        _check_conforming(X, T, radii)
 
-    ax = np.asarray(axes, dtype=np.float64)
+    ax = np.asarray(axes, dtype=np.float64) * float(scale)
     X = X * ax[None, :]
-    return _assemble_mesh(X, T, on_surf, porder, seed=seed, h=h, reorder_elems=True)
+    return _assemble_mesh(X, T, on_surf, porder, seed=seed, h=h * float(scale), reorder_elems=True)
+
+
+def _check_conforming(X, T, radii):
+    """Every tetrahedron on one side of every interface sphere (vertices on
+    the sphere count for both sides)."""
+    rX = np.linalg.norm(X, axis=1)
+    rc = np.linalg.norm(X[T].mean(axis=1), axis=1)
+    for R in radii:
+        tol = 1e-9 * R
+        inside = rc < R
+        bad = (inside & (rX[T] > R + tol).any(axis=1)) | (~inside & (rX[T] < R - tol).any(axis=1))
+        if bad.any():
+            raise ValueError(f"{int(bad.sum())} tetrahedra cross the interface r = {R}: refine or change the seed")
 
 
 def _promote(X, T, on_surf, porder):

@@ -16,8 +16,15 @@ import numpy as np
 
 from .mesh import TetMesh
 
-__all__ = ["Layer", "RadialModel", "homogeneous", "three_layer", "three_layer_solid",
-           "element_node_values", "lame", "gaussian_random_field"]
+__all__ = ["Layer", "RadialModel", "homogeneous", "three_layer", "three_layer_solid", "fluid_core",
+           "element_node_values", "lame", "gaussian_random_field", "mass_within", "reference_gravity",
+           "GRAV_CONST"]
+
+# Newton's constant [m^3 kg^-1 s^-2] of the reference's model builder: the
+# demo gravity file (CONST_1L_3k.1_pod_1_potential_acceleration_true.dat,
+# a homogeneous rho = 5.51 ball) equals (4/3) pi G rho r with this value to
+# 3e-12, and differs by >= 4e-5 with any CODATA value.
+GRAV_CONST = 6.6723e-11
 
 
 @dataclass
@@ -36,10 +43,21 @@ class Layer:
 
 @dataclass
 class RadialModel:
+    """Layers over the normalised radius r/radius in [0, 1]; ``radius`` is
+    the body's radius in mesh units (km for the reference's models)."""
     layers: list[Layer]
     name: str = "radial"
     axes: tuple[float, float, float] = (1.0, 1.0, 1.0)
     meta: dict = field(default_factory=dict)
+    radius: float = 1.0
+
+    @property
+    def interfaces(self):
+        """Normalised radii where the layers meet."""
+        return tuple(L.r1 for L in self.layers[:-1])
+
+    def fluid_layers(self):
+        return [k for k, L in enumerate(self.layers) if max(L.vs) == 0.0]
 
     def layer_index(self, r):
         edges = np.array([L.r1 for L in self.layers[:-1]])
@@ -55,16 +73,28 @@ def _profile(r0, r1, lo, hi):
     return (lo + (hi - lo) * r0, lo + (hi - lo) * r1)
 
 
-def three_layer() -> RadialModel:
+def three_layer(radius: float = 1.0) -> RadialModel:
     """Solid inner core r<0.19, fluid outer core 0.19-0.55, solid mantle 0.55-1;
-    rho 13->4, vp 11->8, vs 3.5->6.5 linear in r (vs = 0 in the fluid)."""
+    rho 13->4, vp 11->8, vs 3.5->6.5 linear in r (vs = 0 in the fluid).
+    ``radius`` in mesh units (6371 for an Earth-sized body in km)."""
     L = []
     for r0, r1, fluid in ((0.0, 0.19, False), (0.19, 0.55, True), (0.55, 1.0, False)):
         rho = _profile(r0, r1, 13.0, 4.0)
         vp = _profile(r0, r1, 11.0, 8.0)
         vs = (0.0, 0.0) if fluid else _profile(r0, r1, 3.5, 6.5)
         L.append(Layer(r0, r1, rho, vp, vs))
-    return RadialModel(L, name="three_layer")
+    return RadialModel(L, name="three_layer", radius=float(radius))
+
+
+def fluid_core(r_core: float = 0.5, radius: float = 1.0) -> RadialModel:
+    """Two layers: fluid core r < r_core (vs = 0) under a solid mantle, with
+    the three-layer profiles of rho, vp and vs.  Meshes of ~1.5k tetrahedra
+    resolve both layers (the test body of the fluid-solid operator)."""
+    L = []
+    for r0, r1, fluid in ((0.0, r_core, True), (r_core, 1.0, False)):
+        vs = (0.0, 0.0) if fluid else _profile(r0, r1, 3.5, 6.5)
+        L.append(Layer(r0, r1, _profile(r0, r1, 13.0, 4.0), _profile(r0, r1, 11.0, 8.0), vs))
+    return RadialModel(L, name="fluid_core", radius=float(radius))
 
 
 def three_layer_solid() -> RadialModel:
@@ -83,7 +113,7 @@ def element_node_values(mesh: TetMesh, model: RadialModel, perturb=None):
     ``perturb`` is an optional callable (x[n,3]) -> (drho, dvp, dvs) relative
     perturbations applied multiplicatively at the nodes.
     """
-    ax = np.asarray(model.axes, dtype=np.float64)
+    ax = np.asarray(model.axes, dtype=np.float64) * model.radius
     X = mesh.nodes / ax[None, :]
     xe = X[mesh.elems]                       # (ne, np, 3)
     rn = np.linalg.norm(xe, axis=2)          # node radius
@@ -131,3 +161,40 @@ def gaussian_random_field(seed=2018, rms=0.02, corr_len=0.1, n_modes=256):
             out.append(rms * v)
         return tuple(out)
     return f
+
+
+def mass_within(model: RadialModel, r):
+    """Mass [kg] inside normalised radius r (array) of a radial model with
+    rho in g/cm^3 and ``radius`` in km: 4 pi int_0^r rho(s) s^2 ds, exact for
+    the piecewise-linear profiles."""
+    r = np.asarray(r, dtype=np.float64)
+    Rm = model.radius * 1.0e3
+    out = np.zeros_like(r)
+    for L in model.layers:
+        b = (L.rho[1] - L.rho[0]) / (L.r1 - L.r0)
+        a = L.rho[0] - b * L.r0
+        lo = L.r0
+        hi = np.clip(r, L.r0, L.r1)
+        prim = lambda s: a * s ** 3 / 3.0 + b * s ** 4 / 4.0  # noqa: E731
+        out += np.where(r > lo, prim(hi) - prim(lo), 0.0)
+    return 4.0 * np.pi * out * 1.0e3 * Rm ** 3   # g/cm^3 -> kg/m^3, normalised -> m
+
+
+def reference_gravity(mesh: TetMesh, model: RadialModel):
+    """Reference gravity at every element node, (n_elem, pNp, 3) in km/s^2:
+    g0 = -G M(r) r_hat / r^2 of the model's own spherical mass distribution
+    (the body's reference state).  The reference reads the same field in
+    m/s^2 from ``<base>_pod_<p>_potential_acceleration_true.dat`` and
+    divides by 1e3 (src/mod_cg_models.f90:366-429,
+    src/mod_cg_create_matrix.f90:194)."""
+    if tuple(model.axes) != (1.0, 1.0, 1.0):
+        raise ValueError("reference_gravity is for spherical bodies")
+    x = mesh.nodes[mesh.elems]                           # (ne, np, 3) mesh units (km)
+    r_km = np.linalg.norm(x, axis=2)
+    rn = r_km / model.radius
+    M = mass_within(model, rn)
+    r_m = np.maximum(r_km, 1e-300) * 1.0e3
+    gmag = GRAV_CONST * M / r_m ** 2                     # m/s^2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rhat = np.where(r_km[..., None] > 0, x / np.maximum(r_km, 1e-300)[..., None], 0.0)
+    return -(gmag / 1.0e3)[..., None] * rhat

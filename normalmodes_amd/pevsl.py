@@ -282,6 +282,14 @@ def solve_stats(h: Handle):
     return dict(lan_steps=ls.value, filter_applies=fa.value, ncand=nc.value, seconds=sec.value)
 
 
+def solve_quality(h: Handle):
+    """(target_met, resid_max) of the residual stop of the last solve."""
+    tm = ctypes.c_int()
+    rm = _d()
+    _check(_lib().nmg_solve_quality(_i64(h), ctypes.byref(tm), ctypes.byref(rm)))
+    return bool(tm.value), rm.value
+
+
 def kpmdos(h: Handle, mdeg: int, nvec: int, intv):
     """Trace moments mu[0..mdeg] of T_k((A~B~^-1 - cc)/dd) and the estimated
     eigenvalue count in [intv[0], intv[1]] (pEVSL kpmdos)."""

@@ -111,3 +111,42 @@ def reference_element(porder: int):
     Mf = np.array([[float(v) for v in row] for row in M])
     Df = np.array([[[float(v) for v in row] for row in Dd] for Dd in D])
     return nodes, Mf, Df
+
+
+@lru_cache(maxsize=4)
+def reference_faces(porder: int):
+    """Face node lists and face mass matrices of the reference tetrahedron.
+
+    Face f (0-based) is opposite vertex f: r+s+t = -1, r = -1, s = -1, t = -1
+    (src/mod_geometry.f90:2331-2347, 2389-2409).  ``fmask[f]`` lists its nodes
+    in increasing local order; ``massF[f]`` = int phi_a phi_b over the
+    reference triangle (-1,-1), (1,-1), (-1,1) in the face coordinates the
+    reference projects on ((s,t), (s,t), (r,t), (r,s); :2495-2513, where it is
+    formed as inv(V2D V2D^T) of an orthonormal basis).  Computed exactly in
+    rational arithmetic and rounded once.  Returns (fmask[4, nfp] int32,
+    massF[4, nfp, nfp] float64)."""
+    if porder not in (1, 2):
+        raise ValueError("pOrder must be 1 or 2")
+    lat = _lattice(porder)                      # (xi, eta, zeta) = (r+1, s+1, t+1)/2
+    tests = (lambda p: p[0] + p[1] + p[2] == 1, lambda p: p[0] == 0, lambda p: p[1] == 0, lambda p: p[2] == 0)
+    coords = ((1, 2), (1, 2), (0, 2), (0, 1))
+    mons = [(a, b) for a in range(porder + 1) for b in range(porder + 1 - a)]
+
+    def tri_int(a, b):
+        # int over the unit triangle of xi^a eta^b; the reference triangle has 4x the area
+        return Fraction(factorial(a) * factorial(b), factorial(a + b + 2))
+
+    G = [[4 * tri_int(ma[0] + mb[0], ma[1] + mb[1]) for mb in mons] for ma in mons]
+    fmask, mass = [], []
+    for f in range(4):
+        idx = [i for i, p in enumerate(lat) if tests[f](p)]
+        pts = [(lat[i][coords[f][0]], lat[i][coords[f][1]]) for i in idx]
+        n = len(pts)
+        V = [[x ** a * y ** b for (a, b) in mons] for (x, y) in pts]
+        eye = [[Fraction(int(i == j)) for j in range(n)] for i in range(n)]
+        C = _solve_exact(V, eye)
+        M = [[sum(C[a][i] * G[a][b] * C[b][j] for a in range(n) for b in range(n)) for j in range(n)]
+             for i in range(n)]
+        fmask.append(idx)
+        mass.append([[float(v) for v in row] for row in M])
+    return np.array(fmask, dtype=np.int32), np.array(mass, dtype=np.float64)

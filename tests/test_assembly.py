@@ -110,9 +110,16 @@ def test_c1_lowest_multiplet_near_lamb_toroidal():
     assert 0 < mult.mean() / x ** 2 - 1 < 0.01              # FE bound from above
 
 
-def test_fluid_elements_rejected():
-    from normalmodes_amd.assemble import assemble
+def test_fluid_body_needs_conforming_interfaces():
+    """A layered body with a fluid layer assembles into the fluid-solid blocks
+    when its mesh has the layer boundaries as tetrahedron faces; on a mesh
+    cut across them the reference's node statuses leave empty rows, and the
+    producer refuses it."""
+    from normalmodes_amd.assemble import FluidProblem, assemble
     from normalmodes_amd.mesh import ball_mesh
     from normalmodes_amd.models import three_layer
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         assemble(ball_mesh(600, porder=2), three_layer())
+    m = three_layer()
+    Q = assemble(ball_mesh(12000, porder=1, interfaces=m.interfaces), m)
+    assert isinstance(Q, FluidProblem) and Q.n_p > 0

@@ -1,10 +1,14 @@
-"""Fluid-solid Schur operator (sparsefsAV) on the device.
+"""Fluid-solid Schur operator (sparsefsAV, src/mod_matvec.f90:98-222, 498-520)
+on the device.
 
-The finite-element producer assembles solids only, so the pencil is synthetic
-but has the reference's block structure (src/mod_matvec.f90:98-222): Ad
-(solid stiffness), Ap = -(pressure mass) negative definite, ET coupling
-pressure to displacements on the fluid nodes' stencils, E = ET^T.  The
-oracle is the dense Schur complement D (Ad - E Ap^-1 ET) D."""
+Two kinds of pencils: synthetic ones with the reference's block structure
+(Ad, Ap = -(pressure mass) negative definite, ET coupling pressure to the
+fluid nodes' stencils, E = ET^T), and the assembled fluid-solid bodies of the
+producer (CGFSE3D_ISO).  The reference for both is the dense *same operator*
+the device applies: D (Ad + E Dp q(Ap~) Dp ET) D, where q is the Chebyshev
+polynomial of the Ap~ solve (degree 100 at P2, 25 at P1) on the device's own
+Lanczos bounds of Ap~ = Dp (-Ap) Dp, in the closed form pinned in
+tests/test_oracle_pins.py (1 - t q(t) = T_{c+1}((th - t)/de) / T_{c+1}(th/de))."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -43,6 +47,32 @@ def dense_schur(Q):
     ET = Q.ET.to_scipy().toarray()
     Ap = Q.Ap.to_scipy().toarray()
     return Ad - E @ np.linalg.solve(Ap, ET)
+
+
+def cheb_poly_values(t, lmin, lmax, deg):
+    """q(t) of the Chebyshev semi-iteration after ``deg`` products on
+    [lmin, lmax]: 1 - t q(t) = T_{deg+1}((th - t)/de) / T_{deg+1}(th/de)."""
+    th, de = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    k = deg + 1
+    T = np.polynomial.chebyshev.chebval
+    R = T((th - t) / de, np.eye(k + 1)[k]) / T(th / de, np.eye(k + 1)[k])
+    return (1.0 - R) / t
+
+
+def same_operator(Q, mv):
+    """The unscaled operator the device applies, Ad + E Dp q(Ap~) Dp ET,
+    symmetrised, with q on the device's bounds and degree of the Ap~ solve."""
+    Ad = Q.Ad.to_scipy().toarray()
+    E = Q.E.to_scipy().toarray()
+    ET = Q.ET.to_scipy().toarray()
+    Ap = Q.Ap.to_scipy().toarray()
+    dp = 1.0 / np.sqrt(-np.diag(Ap))
+    Apt = -Ap * dp[:, None] * dp[None, :]
+    th, U = np.linalg.eigh(0.5 * (Apt + Apt.T))
+    q = cheb_poly_values(th, mv.fs.aplmin, mv.fs.aplmax, mv.fs.apdeg)
+    C = (U * q[None, :]) @ U.T
+    S = Ad + (E * dp[None, :]) @ C @ (dp[:, None] * ET)
+    return 0.5 * (S + S.T)
 
 
 def test_synthetic_structure(small_p2):
@@ -106,8 +136,12 @@ def test_fs_eigenpairs_match_dense(fs_setup, block):
     ref = w[(w >= a) & (w <= b)]
     r = apply_pevsl(mv, interval=(a, b), block=block)
     assert r.lam.size == ref.size
-    assert np.max(np.abs(r.lam - ref) / ref) < 1e-10
+    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
     assert np.all(r.res2 <= 1e-10)
+    # the same-operator pencil has the same eigenvalues (q(Ap~) = Ap~^-1 to
+    # rounding at degree 100 on this pencil)
+    w2 = sl.eigh(same_operator(Q, mv), B, eigvals_only=True)
+    assert np.max(np.abs(w2[(w >= a) & (w <= b)] - ref) / ref) < 1e-13
 
 
 @pytest.mark.gpu
@@ -120,4 +154,91 @@ def test_fs_slices(fs_setup):
     a, b = 0.5 * (nz[2] + nz[3]), 0.5 * (nz[17] + nz[18])
     ref = w[(w >= a) & (w <= b)]
     r = apply_slices(mv, interval=(a, b), n_slices=2, slicer="uniform")
-    assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-10
+    assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-12
+
+
+# ---------------------------------------------------------------- assembled bodies
+ASSEMBLED = {
+    # name: (model, n_tets, pOrder, job)
+    "core-p2-job1": ("fluid_core", 900, 2, 1),
+    "core-p2-job2": ("fluid_core", 900, 2, 2),
+    "3layer-p1-job2": ("three_layer", 9000, 1, 2),
+}
+
+
+@pytest.fixture(scope="module", params=list(ASSEMBLED), ids=list(ASSEMBLED))
+def assembled(request):
+    from normalmodes_amd import models
+    from normalmodes_amd.assemble import assemble
+    from normalmodes_amd.driver import setupmatvec_fs
+    from normalmodes_amd.mesh import ball_mesh
+    name, nt, porder, job = ASSEMBLED[request.param]
+    m = models.fluid_core(0.5, radius=6371.0) if name == "fluid_core" else models.three_layer(6371.0)
+    me = ball_mesh(nt, porder=porder, interfaces=m.interfaces, scale=m.radius)
+    Q = assemble(me, m, job=job)
+    print(f"assembled {request.param}: {Q.n} DoF, {Q.n_p} pressures", flush=True)
+    mv = setupmatvec_fs(Q, bdeg="auto")
+    S = same_operator(Q, mv)
+    B = Q.B.to_scipy().toarray()
+    w = sl.eigh(S, B, eigvals_only=True)
+    yield request.param, Q, mv, S, B, w
+    mv.free()
+
+
+def _seismic_interval(w, n_modes=20):
+    """An interval over the first ~n_modes eigenvalues above the zero-energy
+    and core-mode cluster, with its ends in gaps."""
+    f = np.sign(w) * np.sqrt(np.abs(w))
+    top = f.max()
+    i0 = int(np.argmax(f > 4e-3 * top))                # first mode above the cluster
+    i1 = min(i0 + n_modes, w.size - 2)
+    g = np.diff(w)
+    j0 = i0 + int(np.argmax(g[i0:i0 + 6]))              # widest gap among the first few
+    j1 = i1 + int(np.argmax(g[i1:i1 + 8]))
+    a = 0.5 * (w[j0] + w[j0 + 1])
+    b = 0.5 * (w[j1] + w[j1 + 1])
+    return a, b
+
+
+@pytest.mark.gpu
+def test_assembled_operator_matches_same_operator(assembled):
+    from normalmodes_amd import pevsl as pv
+    from normalmodes_amd.driver import _open_solver
+    name, Q, mv, S, B, w = assembled
+    assert mv.fs.apdeg == (100 if Q.porder == 2 else 25)
+    St = S * mv.d[:, None] * mv.d[None, :]
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(mv.n)
+    h = _open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
+    try:
+        y = pv.apply(h, 0, x)
+    finally:
+        pv.pevsl_finish(h)
+    ref = St @ x
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref), name
+
+
+@pytest.mark.gpu
+def test_assembled_eigenpairs_match_same_operator(assembled):
+    """All eigenpairs in an interval of ~20 seismic modes, single-vector
+    ChebLanNr, against the dense eigenvalues of the same operator at 1e-12;
+    the residual gate recomputed from the returned vectors."""
+    from normalmodes_amd.driver import apply_pevsl
+    name, Q, mv, S, B, w = assembled
+    a, b = _seismic_interval(w)
+    ref = w[(w >= a) & (w <= b)]
+    assert ref.size >= 10
+    print(f"{name}: interval [{a:.6e}, {b:.6e}], {ref.size} eigenvalues", flush=True)
+    r = apply_pevsl(mv, interval=(a, b), block=4)
+    assert r.lam.size == ref.size, (name, r.lam.size, ref.size)
+    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12, name
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+    # recomputed residual ||S~x - lam B~x|| / |lam| of the scaled pencil
+    St = S * mv.d[:, None] * mv.d[None, :]
+    Bt = B * mv.d[:, None] * mv.d[None, :]
+    X = r.X
+    R = St @ X.T - (Bt @ X.T) * r.lam[None, :]
+    xb = np.einsum("ij,ji->i", X, Bt @ X.T)
+    r2 = np.linalg.norm(R, axis=0) / np.sqrt(xb) / np.abs(r.lam)
+    assert np.all(r2 <= 1e-10), r2
+    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + 1e-14)

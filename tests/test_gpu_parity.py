@@ -17,6 +17,34 @@ def _rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
+def _recomputed_residuals(P, lam, X):
+    """Both residual definitions of the scaled pencil recomputed with scipy
+    from copied-out eigenvectors (rows of X, scaled coordinates x~):
+    the reference's sqrt(sum r^2 / N)/|lam| (src/mod_pevsl.f90:137-162) and
+    ||A~x - lam B~x||_2 / (|lam| sqrt(x^T B~ x))."""
+    import scipy.sparse as sp
+    A, B = P.A.to_scipy(), P.B.to_scipy()
+    D = sp.diags(1.0 / np.sqrt(B.diagonal()))
+    At, Bt = (D @ A @ D).tocsr(), (D @ B @ D).tocsr()
+    R = (At @ X.T) - (Bt @ X.T) * lam[None, :]
+    xb = np.einsum("ij,ji->i", X, Bt @ X.T)
+    al = np.where(np.abs(lam) > 0, np.abs(lam), 1.0)
+    nr = np.linalg.norm(R, axis=0)
+    return nr / np.sqrt(P.n) / al, nr / np.sqrt(xb) / al
+
+
+def _check_gate(P, r, gate=1e-10):
+    """The device's nmg_residuals agree with the recomputation, and the
+    recomputed 2-norm residual meets the gate."""
+    r1, r2 = _recomputed_residuals(P, r.lam, r.X)
+    assert np.all(r2 <= gate), r2
+    # a residual near 1e-13 is the difference of terms of order |lam|: its
+    # own rounding (~1e-15 relative to |lam|) is the absolute floor
+    floor = 1e-14
+    assert np.all(np.abs(r.res2 - r2) <= 1e-3 * r2 + floor), (r.res2, r2)
+    assert np.all(np.abs(r.res_ref - r1) <= 1e-3 * r1 + floor / np.sqrt(P.n)), (r.res_ref, r1)
+
+
 def test_spmv_layouts(small_p2):
     pv = _pv()
     P = small_p2
@@ -256,6 +284,7 @@ def test_full_path_residual_gate(small_p2, dense_eigs):
         assert r.lam.size == ref.size
         assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
         assert np.all(r.res2 <= 1e-10), r.res2
+        _check_gate(small_p2, r)
         # eigenvectors: B~-orthonormal
         X = r.X
         BX = np.stack([orc.Pencil(small_p2.A, small_p2.B, bdeg=1, bounds_B=(1, 2)).apply_b(x) for x in X])
@@ -319,12 +348,13 @@ def test_c1_matches_shift_invert_golden():
     assert P.n == g["n_dof"] and P.A.nnz == g["nnz_A"]
     mv = setupmatvec(P, bdeg="auto")
     try:
-        r = apply_pevsl(mv, interval=(a, b), vectors=False)
+        r = apply_pevsl(mv, interval=(a, b))
     finally:
         mv.free()
     assert r.lam.size == ref.size == g["n_in_interval"]
     assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
-    assert np.all(r.res2 <= 1e-10)
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+    _check_gate(P, r)
 
 
 @pytest.mark.parametrize("block", [2, 4, 8, 16])
@@ -342,6 +372,7 @@ def test_block_lanczos_matches_dense(small_p2, dense_eigs, block):
     assert r.lam.size == ref.size
     assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
     assert np.all(r.res2 <= 1e-10), r.res2
+    _check_gate(small_p2, r)
 
 
 def test_block_products_match_single_column(small_p2):
@@ -384,9 +415,10 @@ def test_fp32_bsolve_eigenvalues_match_fp64(small_p2, dense_eigs):
     assert np.all(r32.res2 < 1e-4)
 
 
-def test_empty_interval_stops_early(small_p2, dense_eigs):
+@pytest.mark.parametrize("block", [1, 8])
+def test_empty_interval_stops_early(small_p2, dense_eigs, block):
     """An interval inside a spectral gap: no pair, and the stop comes from the
-    converged largest Ritz value, not from maxit."""
+    converged largest Ritz value, not from maxit (single-vector and block)."""
     from normalmodes_amd.driver import setupmatvec, apply_pevsl
     w = dense_eigs(small_p2)
     nz = w[w > 1.0]
@@ -394,10 +426,30 @@ def test_empty_interval_stops_early(small_p2, dense_eigs):
     a, b = nz[g] + 0.3 * (nz[g + 1] - nz[g]), nz[g] + 0.7 * (nz[g + 1] - nz[g])
     mv = setupmatvec(small_p2, bdeg="auto")
     try:
-        r = apply_pevsl(mv, interval=(a, b), vectors=False)
+        r = apply_pevsl(mv, interval=(a, b), vectors=False, block=block)
     finally:
         mv.free()
-    assert r.lam.size == 0 and r.lan_steps < 200
+    assert r.lam.size == 0 and r.lan_steps < 200 * block
+
+
+@pytest.mark.parametrize("block", [1, 8])
+def test_interval_beside_one_eigenvalue_stops(small_p2, dense_eigs, block):
+    """An interval that starts just above an isolated eigenvalue: that
+    eigenvalue's filtered Ritz value may reach the bar and converge while its
+    Rayleigh quotient lies outside the interval.  The solve must stop with
+    no pair, not run on to maxit."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    w = dense_eigs(small_p2)
+    nz = w[w > 1.0]
+    g = int(np.argmax(np.diff(nz[:30])))
+    gap = nz[g + 1] - nz[g]
+    a, b = nz[g] + 0.01 * gap, nz[g] + 0.4 * gap
+    mv = setupmatvec(small_p2, bdeg="auto")
+    try:
+        r = apply_pevsl(mv, interval=(a, b), vectors=False, block=block)
+    finally:
+        mv.free()
+    assert r.lam.size == 0 and r.lan_steps < 200 * block and r.target_met
 
 
 def test_tall_skinny_products():

@@ -264,3 +264,40 @@ def test_cheblannr_empty_interval_stops():
     a, b = nz[g] + 0.3 * (nz[g + 1] - nz[g]), nz[g] + 0.7 * (nz[g + 1] - nz[g])
     lam, X, info = orc.solve_interval(P.A, P.B, (a, b), bdeg=58, return_info=True)
     assert lam.size == 0 and info["nsteps"] < 200
+
+
+# ------------------------------------------------------------ residual check
+def test_residuals_closed_form(small_p2):
+    """or_residuals (src/mod_pevsl.f90:137-162) on vectors whose residual is
+    known exactly: x = x_j + eps x_k from two B~-orthonormal eigenvectors of
+    the scaled pencil gives A~x - lam_j B~x = eps (lam_k - lam_j) B~x_k, so
+        ref definition: eps |lam_k - lam_j| ||B~x_k|| / sqrt(N) / |lam_j|
+        2-norm gate:    eps |lam_k - lam_j| ||B~x_k|| / sqrt(1 + eps^2) / |lam_j|.
+    A dropped /N, a missing B~-normalisation or a residual taken with
+    lam B x instead of lam B~x fails one of the two."""
+    A, B = small_p2.A.to_scipy(), small_p2.B.to_scipy()
+    d = 1.0 / np.sqrt(B.diagonal())
+    D = sp.diags(d)
+    At = (D @ A @ D).toarray()
+    Bt = (D @ B @ D).toarray()
+    w, V = sl.eigh(At, Bt)
+    j, k = 20, 27
+    P = orc.Pencil(small_p2.A, small_p2.B, bdeg=5, bounds_B=(0.2, 5.0))
+    n = P.n
+    for eps in (0.0, 1e-9, 1e-4, 0.3):
+        x = V[:, j] + eps * V[:, k]
+        r1, r2 = P.residuals(np.array([w[j]]), x[None, :])
+        nb = np.linalg.norm(Bt @ V[:, k])
+        want_ref = eps * abs(w[k] - w[j]) * nb / np.sqrt(n) / abs(w[j])
+        want_2 = eps * abs(w[k] - w[j]) * nb / np.sqrt(1.0 + eps * eps) / abs(w[j])
+        # the dense eigenvector itself leaves a residual of ~1e-13 (rounding)
+        floor = 3e-13
+        assert abs(r1[0] - want_ref) <= 1e-10 * want_ref + floor, (eps, r1, want_ref)
+        assert abs(r2[0] - want_2) <= 1e-10 * want_2 + floor, (eps, r2, want_2)
+    # several pairs at once, and a zero eigenvalue is divided by 1, not 0
+    X = V[:, [0, j, k]].T.copy()
+    r1, r2 = P.residuals(w[[0, j, k]], X)
+    assert np.all(r1[1:] < 1e-12) and np.all(r2[1:] < 1e-12)
+    r1z, r2z = P.residuals(np.array([0.0]), V[:, [j]].T.copy())
+    want = np.linalg.norm(At @ V[:, j])
+    assert abs(r2z[0] - want) < 1e-10 * want and abs(r1z[0] - want / np.sqrt(n)) < 1e-10 * want

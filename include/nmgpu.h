@@ -121,6 +121,16 @@ int nmg_set_seed(nmg_handle h, uint64_t seed);
 int nmg_setup_chebiter(double lmin, double lmax, int deg, nmg_handle mat, nmg_handle *cheb);
 /* pevsl_chebiter_f90(type, b, x, cheb)  src/mod_matvec.f90:480, 512 */
 int nmg_chebiter(int type, const double *b, double *x, nmg_handle cheb);
+/* The same solve on device-resident vectors without allocating: no
+ * reference counterpart; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  b_dev and x_dev are device pointers of nrow doubles on the current
+ * device, stream a hipStream_t (NULL: the null stream); the call is enqueued
+ * on the stream and returns without synchronizing.  The second iterate lives
+ * in a fixed 32 MiB device array per device, so nrow <= 4194304 (larger
+ * matrices are rejected with NMG_EINVAL).  Every call on one device shares
+ * that array: calls that overlap on one device race. */
+int nmg_chebiter_noalloc(int type, const double *b_dev, double *x_dev, nmg_handle cheb, void *stream);
 /* pEVSL_chebiterstatsprint_f90(cheb)  src/mod_pevsl.f90:132-135 */
 int nmg_chebiter_stats(nmg_handle cheb, int64_t *ncalls, int64_t *nmatvec, double *seconds);
 /* releases a B-solve (no reference counterpart: chebB lives until the end

@@ -728,6 +728,28 @@ static Solver::GraphKey graph_key(const Solver& S, const Cheb& C, int K, int par
 //   x_1 = b/th,  x_{k+1} = x_{k-1} + w_{k+1}(x_k - x_{k-1} + (b - B x_k)/th),
 //   w_1 = 2, w_{k+1} = 1/(1 - w_k/(4 s^2)), th = (lmax+lmin)/2, s = th/de.
 // The result lands in `out`; `tmp` is a second buffer.
+// The Chebyshev semi-iteration as one fused launch per step: x_1 = b/th,
+// x_0 = 0 (cleared here), x_{k+1} from the fused product + update; the
+// iterates alternate between out and tmp so that the last lands in out.
+// With `probe`, the step launches are bracketed by its events.
+static void cheb_steps(const Cheb& C, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
+                       KernelProbe* probe) {
+  const int64_t n = C.mat->nrow;
+  const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
+  double* xk = (C.deg % 2 == 0) ? out : tmp;
+  double* xm = (C.deg % 2 == 0) ? tmp : out;
+  k_scale_copy(n, 1.0 / th, b, xk, s);
+  k_fill(n, 0.0, xm, s);
+  if (probe) probe->before(s);
+  double w = 2.0;
+  for (int k = 1; k <= C.deg; ++k) {
+    w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    std::swap(xk, xm);
+  }
+  if (probe) probe->after(s);
+}
+
 static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
                        bool fp32, KernelProbe* probe = nullptr) {
   const int64_t n = C.mat->nrow;
@@ -743,22 +765,10 @@ static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipSt
     }
     return;
   }
-  const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
-  double* xk = (C.deg % 2 == 0) ? out : tmp;
-  double* xm = (C.deg % 2 == 0) ? tmp : out;
-  k_scale_copy(n, 1.0 / th, b, xk, s);
-  k_fill(n, 0.0, xm, s);
   // a sampled solve is bracketed as a whole (its deg back-to-back step
   // launches), so the event overhead is not charged to a single launch
   const bool pr = probe && probe->armed();
-  if (pr) probe->before(s);
-  double w = 2.0;
-  for (int k = 1; k <= C.deg; ++k) {
-    w = 1.0 / (1.0 - w / (4.0 * sg * sg));
-    k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
-    std::swap(xk, xm);
-  }
-  if (pr) probe->after(s);
+  cheb_steps(C, b, out, tmp, s, fp32, pr ? probe : nullptr);
   if (probe) {
     probe->bytes = step_bytes;
     probe->launches = C.deg;
@@ -2382,6 +2392,50 @@ int nmg_chebiter(int type, const double* b, double* x, nmg_handle cheb) {
     NMG_HIP(hipMemcpyAsync(x, d1.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     NMG_HIP(hipStreamSynchronize(s));
     C->seconds += now() - t0;
+    C->ncalls++;
+    C->nmatvec += C->deg;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free Chebyshev solve.  Its one scratch vector, the second iterate
+// of the recurrence, is a fixed device array of kScratchDoubles doubles (32 MiB
+// on every device the library runs on, held until the process ends).  Host
+// code never passes the array's name to a kernel: the name is a host-side
+// stand-in; its device address is looked up once per device with
+// hipGetSymbolAddress and kept in scratch_ptr.
+constexpr int64_t kScratchDoubles = (int64_t)1 << 22;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static double* scratch_on_current_device() {
+  static std::mutex mu;
+  static double* scratch_ptr[kMaxDevices] = {};
+  int dev = -1;
+  NMG_HIP(hipGetDevice(&dev));
+  NMG_CHECK(dev >= 0 && dev < kMaxDevices, NMG_EHIP, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    NMG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NMG_CHECK(p != nullptr, NMG_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return scratch_ptr[dev];
+}
+
+int nmg_chebiter_noalloc(int type, const double* b, double* x, nmg_handle cheb, void* stream) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(type == 1 || type == 2, NMG_EINVAL, "chebiter type must be 1 or 2");
+    NMG_CHECK(b && x, NMG_EINVAL, "null vector");
+    NMG_CHECK(b != x, NMG_EINVAL, "b and x must be different device vectors");
+    const int64_t n = C->mat->nrow;
+    NMG_CHECK(n <= kScratchDoubles, NMG_EINVAL,
+              "nmg_chebiter_noalloc: the matrix has more rows than the scratch array holds (4194304)");
+    double* tmp = scratch_on_current_device();
+    cheb_steps(*C, b, x, tmp, static_cast<hipStream_t>(stream), false, nullptr);
+    NMG_HIP(hipGetLastError());
     C->ncalls++;
     C->nmatvec += C->deg;
   });

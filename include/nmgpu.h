@@ -281,6 +281,20 @@ int nmg_set_block_size(nmg_handle h, int block);
  * reference counterpart). */
 int nmg_set_split_streams(nmg_handle h, int parts);
 
+/* MVFunc-shaped callbacks, void (*)(double *x, double *y, void *data) as
+ * pEVSL's PEVSL_SETAMV_F90 / PEVSL_SETBMV_F90 take them
+ * (src/mod_pevsl.f90:69, 77-80), standing for the reference's sparseAV,
+ * sparseBV and solveBV (src/mod_matvec.f90:445-483): y = A~x (the DAD or
+ * fluid-solid operator set on the solver), y = B~x, y ~ B~^-1 x (the
+ * solver's Chebyshev B-solve).  x and y are host vectors of the solver's n;
+ * data points at the solver's nmg_handle.  An unmodified pEVSL caller can
+ * hand these to its own Lanczos driver.  They return nothing, as MVFunc
+ * does: on failure y is filled with NaN and nmg_last_error() holds the
+ * message. */
+void nmg_amv(double *x, double *y, void *data);
+void nmg_bmv(double *x, double *y, void *data);
+void nmg_bsol(double *x, double *y, void *data);
+
 /* Kernel timing probe (no reference counterpart): with every > 0, every
  * `every`-th B-solve (which = 0: its deg back-to-back fused Chebyshev step
  * launches, bracketed as a whole) and every `every`-th fused filter step

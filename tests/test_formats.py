@@ -151,3 +151,25 @@ def test_const3k_job2_adjoint(const3k):
         assert np.max(np.abs(g[k] - ref[k])) < 1e-10 * np.abs(ref[k]).max(), k
     s = g[1] @ P.lam.mean(axis=1) + g[2] @ P.mu.mean(axis=1) + g[0] @ P.rho.mean(axis=1)
     assert np.max(np.abs(s) / r.lam) < 1e-9
+
+
+def test_reference_gravity_matches_demo_file(const3k):
+    """The gravity generator on the demo's homogeneous ball (rho 5.51,
+    R 6371 km) reproduces the demo's own gravity file, which the reference's
+    model builder wrote, at every element node (m/s^2 there, km/s^2 here)."""
+    from normalmodes_amd.models import RadialModel, Layer, reference_gravity
+    M = const3k
+    model = RadialModel([Layer(0.0, 1.0, (5.51, 5.51), (10.0, 10.0), (5.77, 5.77))], radius=6371.0)
+    g = reference_gravity(M.mesh, model)
+    # surface nodes sit up to 3e-6 km outside R = 6371, where the generator
+    # switches to G M / r^2: a 1e-9 relative difference there
+    assert np.max(np.abs(g * 1e3 - M.g0)) <= 3e-9 * np.abs(M.g0).max()
+    inner = np.linalg.norm(M.mesh.nodes[M.mesh.elems], axis=2) < 6371.0
+    assert np.max(np.abs(g * 1e3 - M.g0)[inner]) <= 1e-11 * np.abs(M.g0).max()
+
+
+def test_face_neighbours_match_demo_file(const3k):
+    """Element adjacency across faces (face f opposite local vertex f, -1 on
+    the boundary) equals the demo's _neigh.dat."""
+    from normalmodes_amd.assemble import face_neighbours
+    assert np.array_equal(face_neighbours(const3k.mesh), const3k.neigh)

@@ -301,3 +301,29 @@ def test_residuals_closed_form(small_p2):
     r1z, r2z = P.residuals(np.array([0.0]), V[:, [j]].T.copy())
     want = np.linalg.norm(At @ V[:, j])
     assert abs(r2z[0] - want) < 1e-10 * want and abs(r1z[0] - want / np.sqrt(n)) < 1e-10 * want
+
+
+def test_shift_invert_eigs_closed_form():
+    """1D P1 finite elements on n interior nodes, h = 1: K = tridiag(-1, 2, -1),
+    M = tridiag(1, 4, 1)/6, eigenvalues 6 (1 - cos t)/(2 + cos t) with
+    t = k pi/(n+1).  Near the bottom of this spectrum (lam_max/lam ~ 5e4)
+    the shift-invert reference stays within a few 1e-13 relative; LAPACK's
+    dense generalized solver, whose absolute error is ~eps ||K||, is several
+    times worse; higher up both are near eps."""
+    import scipy.sparse as sp
+    n = 800
+    K = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+    M = (sp.diags([np.ones(n - 1), 4 * np.ones(n), np.ones(n - 1)], [-1, 0, 1]) / 6.0).tocsr()
+    t = np.arange(1, n + 1) * np.pi / (n + 1)
+    exact = 12.0 * np.sin(0.5 * t) ** 2 / (2.0 + np.cos(t))    # 1 - cos t without cancellation
+    dense = sl.eigh(K.toarray(), M.toarray(), eigvals_only=True)
+    for (i0, i1), tol in (((3, 30), 5e-13), ((30, 60), 2e-14)):
+        a, b = 0.5 * (exact[i0 - 1] + exact[i0]), 0.5 * (exact[i1] + exact[i1 + 1])
+        ref = exact[(exact >= a) & (exact <= b)]
+        w = orc.shift_invert_eigs(K, M, a, b)
+        assert w.size == ref.size == i1 - i0 + 1
+        err = np.max(np.abs(w - ref) / ref)
+        assert err < tol, (i0, err)
+        wd = orc.shift_invert_eigs(K.toarray(), M.toarray(), a, b)
+        assert np.max(np.abs(wd - ref) / ref) < tol
+        assert np.max(np.abs(dense[i0:i1 + 1] - ref) / ref) > 3 * err

@@ -295,14 +295,13 @@ void nmg_amv(double *x, double *y, void *data);
 void nmg_bmv(double *x, double *y, void *data);
 void nmg_bsol(double *x, double *y, void *data);
 
-/* Kernel timing probe (no reference counterpart): with every > 0, every
- * `every`-th B-solve (which = 0: its deg back-to-back fused Chebyshev step
- * launches, bracketed as a whole) and every `every`-th fused filter step
- * launch (which = 1) are bracketed by HIP events on the solver's stream.
- * nmg_kernel_probe returns the mean duration of one launch (bracket time /
- * launches in it, so dispatch gaps between back-to-back launches are
- * included), the sample count and the algorithmic bytes of one launch
- * (matrix + vector streams). */
+/* Kernel timing probe (no reference counterpart): with every > 0, in every
+ * `every`-th B-solve the middle fused Chebyshev step launch (which = 0; a
+ * persistent or graph-replayed solve is bracketed whole) and every
+ * `every`-th fused filter step launch (which = 1) are bracketed by HIP
+ * events on the stream they run on.  nmg_kernel_probe returns the mean
+ * duration of one launch (bracket time / launches in it), the sample count
+ * and the algorithmic bytes of one launch (matrix + vector streams). */
 int nmg_set_kernel_probe(nmg_handle h, int every);
 int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *samples,
                      double *bytes_per_launch);
@@ -331,6 +330,23 @@ int nmg_gemm_tn(int64_t n, int m, int kc, const double *A, const double *B, doub
  * seconds per C = A^T B with A n x m and B n x kc filled on the device;
  * bytes: its algorithmic HBM bytes 8 (n m + n kc + m kc). */
 int nmg_bench_gemm_tn(int64_t n, int m, int kc, int reps, double *seconds, double *bytes);
+/* Dense products of the block recurrence and the Rayleigh-Ritz extraction
+ * (no reference counterpart; pEVSL's ChebLanNr does them with BLAS inside
+ * pEVSL_CHEBLANNR_F90, src/mod_pevsl.f90:122), on host arrays (column-major,
+ * leading dimensions n, m and n): op 0 C (n x K) = alpha A (n x m) B (m x K)
+ * + beta C; op 1 C (m x K) = A^T B with A n x m, B n x K.  kernel 1 runs the
+ * hand-written FP64 MFMA kernels (v_mfma_f64_16x16x4f64) the solver uses,
+ * kernel 0 rocBLAS DGEMM (op 0) or the FMA split reduction (op 1).
+ * nmg_bench_dense times one product on device-resident data and returns its
+ * algorithmic bytes and flops. */
+int nmg_dense_product(int op, int kernel, int64_t n, int m, int K, double alpha, const double *A,
+                      const double *B, double beta, double *C);
+int nmg_bench_dense(int op, int kernel, int64_t n, int m, int K, int reps, double *seconds, double *bytes,
+                    double *flops);
+/* Which kernels a solver's dense products run on (no reference
+ * counterpart): 1 (default) the MFMA kernels, 0 rocBLAS DGEMM and the FMA
+ * split reduction. */
+int nmg_set_dense_kernels(nmg_handle h, int mfma);
 /* A plain copy kernel of n doubles, `reps` times (calibration of HBM byte
  * counters: it moves 16 n bytes per launch); mean seconds per launch. */
 int nmg_bench_copy(int64_t n, int reps, double *seconds);

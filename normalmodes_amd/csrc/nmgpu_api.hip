@@ -339,10 +339,12 @@ void build_row_groups(DevMat& M, int per_wave) {
 void build_tiling(DevMat& M, int K, int max_nodes, int rows_per_wave, int block, int wide) {
   M.touch();
   int idx = -1;
-  for (int i = 0; i < 5; ++i)
+  for (int i = 0; i < 7; ++i)
     if ((1 << i) == K) idx = i;
-  NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+  NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8, 16, 32 or 64");
   delete M.tiling[idx];
+  M.tiling[idx] = nullptr;
+  NMG_CHECK(max_nodes <= 0 || K <= 16, NMG_EINVAL, "LDS tiles are built for K <= 16");
   M.tiling[idx] = nullptr;
   if (max_nodes <= 0 || M.layout == L_CSR) return;
   NMG_CHECK((int64_t)M.h_browptr.size() == M.nb + 1, NMG_ESTATE, "no host pattern for tiling");
@@ -381,8 +383,8 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
   M->h_browptr = S.h_browptr;
   M->h_bcol = S.h_bcol;
   M->group = S.group;
-  for (int i = 0; i < 5; ++i) M->block_map[i] = S.block_map[i];
-  for (int i = 0; i < 5; ++i) M->wide[i] = S.wide[i];
+  for (int i = 0; i < 7; ++i) M->block_map[i] = S.block_map[i];
+  for (int i = 0; i < 7; ++i) M->wide[i] = S.wide[i];
   auto cp = [&](auto* src, auto*& dst, size_t count) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
     if (!src) return;
@@ -559,6 +561,7 @@ struct Solver {
   int target_met = 1;       // last cheblannr: every pair met resid_target (1), or it stopped on stagnation (0)
   double resid_best = 0.0;  // max ||A~x - lam B~x||/|lam| of the returned pairs at the residual stop (0: not checked)
   DVec mvx, mvy;            // staging of the MVFunc-shaped callbacks (nmg_amv / nmg_bmv / nmg_bsol)
+  int mfma = 1;             // dense products: 1 hand-written FP64 MFMA kernels, 0 rocBLAS DGEMM + FMA split reduction
 
   std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
@@ -733,7 +736,7 @@ static Solver::GraphKey graph_key(const Solver& S, const Cheb& C, int K, int par
 // The Chebyshev semi-iteration as one fused launch per step: x_1 = b/th,
 // x_0 = 0 (cleared here), x_{k+1} from the fused product + update; the
 // iterates alternate between out and tmp so that the last lands in out.
-// With `probe`, the step launches are bracketed by its events.
+// With `probe`, the middle step's launch is bracketed by its events.
 static void cheb_steps(const Cheb& C, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
                        KernelProbe* probe) {
   const int64_t n = C.mat->nrow;
@@ -742,14 +745,15 @@ static void cheb_steps(const Cheb& C, const double* b, double* out, double* tmp,
   double* xm = (C.deg % 2 == 0) ? tmp : out;
   k_scale_copy(n, 1.0 / th, b, xk, s);
   k_fill(n, 0.0, xm, s);
-  if (probe) probe->before(s);
+  const int mid = (C.deg + 1) / 2;
   double w = 2.0;
   for (int k = 1; k <= C.deg; ++k) {
     w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    if (probe && k == mid) probe->before(s);
     k_cheb_step(*C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    if (probe && k == mid) probe->after(s);
     std::swap(xk, xm);
   }
-  if (probe) probe->after(s);
 }
 
 static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
@@ -767,13 +771,12 @@ static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipSt
     }
     return;
   }
-  // a sampled solve is bracketed as a whole (its deg back-to-back step
-  // launches), so the event overhead is not charged to a single launch
+  // in a sampled solve one step launch (the middle one) is bracketed
   const bool pr = probe && probe->armed();
   cheb_steps(C, b, out, tmp, s, fp32, pr ? probe : nullptr);
   if (probe) {
     probe->bytes = step_bytes;
-    probe->launches = C.deg;
+    probe->launches = 1;
   }
 }
 
@@ -992,6 +995,26 @@ static void gemm_tn_any(Solver& S, int64_t n, int m, int kc, const double* A, in
   }
 }
 
+// Dense products of the block recurrence and the extraction, on the device:
+// C (n x K) = alpha A (n x m) B (m x K) + beta C and C (m x K) = A^T B.
+static void gemm(Solver& S, rocblas_operation ta, rocblas_operation tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc, const char* what);
+static void dense_nn(Solver& S, int64_t n, int K, int m, double alpha, const double* A, int64_t lda,
+                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, const char* what) {
+  if (S.mfma) k_gemm_nn_mfma(n, K, m, alpha, A, lda, B, ldb, beta, C, ldc, S.s);
+  else gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, m, alpha, A, lda, B, ldb, beta, C, ldc, what);
+}
+static void dense_tn(Solver& S, int64_t n, int m, int K, const double* A, int64_t lda, const double* B,
+                     int64_t ldb, double* C, int64_t ldc) {
+  if (!S.mfma) return gemm_tn_any(S, n, m, K, A, lda, B, ldb, C, ldc);
+  for (int c0 = 0; c0 < K; c0 += 64) {
+    const int w = std::min(64, K - c0);
+    S.gw.ensure((int64_t)gemm_tn_mfma_work_size(n, m, w));
+    k_gemm_tn_mfma(n, m, w, A, lda, B + (int64_t)c0 * ldb, ldb, C + (int64_t)c0 * ldc, ldc, S.gw.p, S.s);
+  }
+}
+
 // Ritz vectors of the candidates (theta >= bar) from the Lanczos basis V(:, 0:kdim)
 // and the eigenvectors dC (kdim x kdim, device) of the projected matrix, then
 // Rayleigh-Ritz of (A~, B~) on their span; keeps the pairs inside xintv[0..1].
@@ -1017,10 +1040,7 @@ static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vecto
       NMG_HIP(hipMemcpyAsync(Ssel.p + (size_t)j * kdim, dC + (size_t)sel[j] * kdim,
                              sizeof(double) * kdim, hipMemcpyDeviceToDevice, S.s));
     Y.ensure(n * nc);
-    const double one = 1.0, zero = 0.0;
-    NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, nc,
-                            kdim, &one, S.V, (rocblas_int)n, Ssel.p, kdim, &zero, Y.p,
-                            (rocblas_int)n), "rocblas_dgemm (Ritz vectors) failed");
+    dense_nn(S, n, nc, kdim, 1.0, S.V, n, Ssel.p, kdim, 0.0, Y.p, n, "rocblas_dgemm (Ritz vectors) failed");
     // Rayleigh-Ritz of (A~, B~) on span(Y)
     AY.ensure(n * nc);
     BY.ensure(n * nc);
@@ -1030,8 +1050,8 @@ static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vecto
     }
     GA.ensure((int64_t)nc * nc);
     GB.ensure((int64_t)nc * nc);
-    gemm_tn_any(S, n, nc, nc, Y.p, n, AY.p, n, GA.p, nc);  // Y^T A Y
-    gemm_tn_any(S, n, nc, nc, Y.p, n, BY.p, n, GB.p, nc);  // Y^T B Y
+    dense_tn(S, n, nc, nc, Y.p, n, AY.p, n, GA.p, nc);  // Y^T A Y
+    dense_tn(S, n, nc, nc, Y.p, n, BY.p, n, GB.p, nc);  // Y^T B Y
     // symmetrise (the products agree to rounding)
     {
       std::vector<double> ga((size_t)nc * nc), gb((size_t)nc * nc);
@@ -1068,9 +1088,7 @@ static void extract_ritz(Solver& S, int kdim, const double* dC, const std::vecto
         NMG_HIP(hipMemcpyAsync(Wsel.p + (size_t)j * nc, GA.p + (size_t)keep[j] * nc, sizeof(double) * nc,
                                hipMemcpyDeviceToDevice, S.s));
       S.X = dmalloc<double>((size_t)n * nev);
-      NMG_LIB(rocblas_dgemm(S.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, nev,
-                              nc, &one, Y.p, (rocblas_int)n, Wsel.p, nc, &zero, S.X,
-                              (rocblas_int)n), "rocblas_dgemm (eigenvectors) failed");
+      dense_nn(S, n, nev, nc, 1.0, Y.p, n, Wsel.p, nc, 0.0, S.X, n, "rocblas_dgemm (eigenvectors) failed");
       for (int j = 0; j < nev; ++j) S.lam.push_back(rmu[keep[j]]);
       S.nev = nev;
     }
@@ -1270,7 +1288,8 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
   if (K == 1 && !w32) return cheb_solve(C, b, out, tmp, s, fp32, probe);
   const int64_t nk = C.mat->nrow * K;
   const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
-  if (const PersistPlan* P = C.persistent(fp32)) {
+  // the persistent form is built for K <= 16
+  if (const PersistPlan* P = K <= 16 ? C.persistent(fp32) : nullptr) {
     const bool pr = probe && probe->armed();
     if (pr) probe->before(s);
     k_cheb_persistent(K, *C.mat, *P, b, out, tmp, C.deg, C.lmin, C.lmax, s);
@@ -1288,18 +1307,19 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
     float* fm = w32 + 2 * nk;
     k_cheb32_start(nk, 1.0 / th, b, b32, fk, fm, s);
     const bool pr = probe && probe->armed();
-    if (pr) probe->before(s);
+    const int mid = (C.deg + 1) / 2;
     double w = 2.0;
     for (int k = 1; k <= C.deg; ++k) {
       w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+      if (pr && k == mid) probe->before(s);
       k_block_cheb_step32(K, *C.mat, b32, fk, fm, w, 1.0 / th, s, fp32);
+      if (pr && k == mid) probe->after(s);
       std::swap(fk, fm);
     }
-    if (pr) probe->after(s);
     k_f32_to_f64(nk, fk, out, s);
     if (probe) {
       probe->bytes = C.mat->matrix_bytes(fp32) + 4.0 * 4.0 * (double)nk;
-      probe->launches = C.deg;
+      probe->launches = 1;
     }
     return;
   }
@@ -1308,17 +1328,18 @@ static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp
   k_scale_copy(nk, 1.0 / th, b, xk, s);
   k_fill(nk, 0.0, xm, s);
   const bool pr = probe && probe->armed();
-  if (pr) probe->before(s);
+  const int mid = (C.deg + 1) / 2;
   double w = 2.0;
   for (int k = 1; k <= C.deg; ++k) {
     w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+    if (pr && k == mid) probe->before(s);
     k_block_cheb_step(K, *C.mat, b, xk, xm, w, 1.0 / th, s, fp32);
+    if (pr && k == mid) probe->after(s);
     std::swap(xk, xm);
   }
-  if (pr) probe->after(s);
   if (probe) {
     probe->bytes = step_bytes;
-    probe->launches = C.deg;
+    probe->launches = 1;
   }
 }
 
@@ -1571,7 +1592,7 @@ static void block_borth(Solver& S, int K, double* V, double* Z, std::vector<doub
   Rinv.ensure((int64_t)K * K);
   T.ensure(n * K);
   S.gw.ensure((int64_t)gemm_tn_work_size(n, K, K));
-  k_gemm_tn(n, K, K, V, n, Z, n, G.p, K, S.gw.p, S.s);  // block Gram V^T Z
+  dense_tn(S, n, K, K, V, n, Z, n, G.p, K);  // block Gram V^T Z
   std::vector<double> g((size_t)K * K), Ri;
   NMG_HIP(hipMemcpyAsync(g.data(), G.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
   NMG_HIP(hipStreamSynchronize(S.s));
@@ -1583,12 +1604,10 @@ static void block_borth(Solver& S, int K, double* V, double* Z, std::vector<doub
   for (int i = 0; i < K; ++i)
     for (int j = 0; j < K; ++j) ric[(size_t)j * K + i] = Ri[(size_t)i * K + j];
   NMG_HIP(hipMemcpyAsync(Rinv.p, ric.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, S.s));
-  gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, 1.0, V, n, Rinv.p, K, 0.0, T.p, n,
-       "V R^-1");
+  dense_nn(S, n, K, K, 1.0, V, n, Rinv.p, K, 0.0, T.p, n, "V R^-1");
   NMG_HIP(hipMemcpyAsync(V, T.p, sizeof(double) * n * K, hipMemcpyDeviceToDevice, S.s));
   if (Z != V) {
-    gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, 1.0, Z, n, Rinv.p, K, 0.0, T.p, n,
-         "Z R^-1");
+    dense_nn(S, n, K, K, 1.0, Z, n, Rinv.p, K, 0.0, T.p, n, "Z R^-1");
     NMG_HIP(hipMemcpyAsync(Z, T.p, sizeof(double) * n * K, hipMemcpyDeviceToDevice, S.s));
   }
   NMG_HIP(hipStreamSynchronize(S.s));
@@ -1673,27 +1692,23 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
     Ad.ensure((int64_t)K * K);
     Rd.ensure((int64_t)K * K);
     if (j > 0) {
-      std::vector<double> rc((size_t)K * K);  // column-major R_j
+      // column-major R_j^T: element (r, c) = R_j[c][r], and R_j is row-major
+      std::vector<double> rt((size_t)K * K);
       for (int r = 0; r < K; ++r)
-        for (int c = 0; c < K; ++c) rc[(size_t)c * K + r] = Rblk[j - 1][(size_t)r * K + c];
-      NMG_HIP(hipMemcpyAsync(Rd.p, rc.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, S.s));
-      gemm(S, rocblas_operation_none, rocblas_operation_transpose, n, K, K, -1.0,
-           S.Z + (size_t)(j - 1) * K * n, n, Rd.p, K, 1.0, W, n, "W -= Z_{j-1} R_j^T");
+        for (int c = 0; c < K; ++c) rt[(size_t)c * K + r] = Rblk[j - 1][(size_t)c * K + r];
+      NMG_HIP(hipMemcpyAsync(Rd.p, rt.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, S.s));
+      dense_nn(S, n, K, K, -1.0, S.Z + (size_t)(j - 1) * K * n, n, Rd.p, K, 1.0, W, n, "W -= Z_{j-1} R_j^T");
     }
-    S.gw.ensure((int64_t)gemm_tn_work_size(n, K, K));
-    k_gemm_tn(n, K, K, Vj, n, W, n, Ad.p, K, S.gw.p, S.s);  // A_j = V_j^T W
+    dense_tn(S, n, K, K, Vj, n, W, n, Ad.p, K);  // A_j = V_j^T W
     std::vector<double> a((size_t)K * K);
     NMG_HIP(hipMemcpyAsync(a.data(), Ad.p, sizeof(double) * K * K, hipMemcpyDeviceToHost, S.s));
-    gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, K, -1.0, Zj, n, Ad.p, K, 1.0, W, n,
-         "W -= Z_j A_j");
+    dense_nn(S, n, K, K, -1.0, Zj, n, Ad.p, K, 1.0, W, n, "W -= Z_j A_j");
     // full re-orthogonalisation, classical Gram-Schmidt twice, as GEMMs
     const int64_t m = (int64_t)(j + 1) * K;
     Hd.ensure(m * K);
     for (int pass = 0; pass < 2; ++pass) {
-      S.gw.ensure((int64_t)gemm_tn_work_size(n, (int)m, K));
-      k_gemm_tn(n, (int)m, K, S.V, n, W, n, Hd.p, m, S.gw.p, S.s);  // H = V^T W
-      gemm(S, rocblas_operation_none, rocblas_operation_none, n, K, m, -1.0, S.Z, n, Hd.p, m, 1.0, W, n,
-           "W -= Z H");
+      dense_tn(S, n, (int)m, K, S.V, n, W, n, Hd.p, m);  // H = V^T W
+      dense_nn(S, n, K, (int)m, -1.0, S.Z, n, Hd.p, m, 1.0, W, n, "W -= Z H");
     }
     // V_{j+1} = B^-1 W, B-orthonormalise
     if (S.geneig) {
@@ -1821,7 +1836,7 @@ static void do_cheblannr_block(Solver& S, const double* xintv, int maxit, double
 // B~ are read once per product for all slices.  All filters share cc and dd
 // (the centre and half-width of [lmin, lmax]).
 static int block_width_for(int ns) {
-  for (int K : {1, 2, 4, 8, 16})
+  for (int K : {1, 2, 4, 8, 16, 32, 64})
     if (K >= ns) return K;
   return -1;
 }
@@ -1830,7 +1845,7 @@ static void do_solve_slices(Solver& S, int ns, const double* sli, double lmin, d
                             double thr_ext, int maxit, double tol) {
   NMG_CHECK(S.n > 0, NMG_ESTATE, "problem size not set");
   const int K = block_width_for(ns);
-  NMG_CHECK(ns >= 1 && K > 0, NMG_EINVAL, "between 1 and 16 slices");
+  NMG_CHECK(ns >= 1 && K > 0, NMG_EINVAL, "between 1 and 64 slices");
   for (int i = 0; i < ns; ++i) NMG_CHECK(sli[i + 1] > sli[i], NMG_EINVAL, "slice boundaries must increase");
   const double t0 = now();
   const int64_t n = S.n;
@@ -2629,7 +2644,7 @@ int nmg_spslicer(double* sli, const double* mu, int mdeg, const double* intv, in
   return guarded([&] {
     NMG_CHECK(sli && mu && intv && mdeg >= 1, NMG_EINVAL, "bad argument");
     NMG_CHECK(intv[3] > intv[2] && intv[1] > intv[0], NMG_EINVAL, "bad interval");
-    NMG_CHECK(n_slices >= 1 && n_slices <= 16, NMG_EINVAL, "between 1 and 16 slices");
+    NMG_CHECK(n_slices >= 1 && n_slices <= 64, NMG_EINVAL, "between 1 and 64 slices");
     const int rc = dos_slicer(mdeg, mu, intv[2], intv[3], intv[0], intv[1], n_slices, sli);
     NMG_CHECK(rc == NMG_OK, NMG_ENUM, "the density estimate gives no increasing slice boundaries");
   });
@@ -2719,8 +2734,8 @@ int nmg_set_graphs(nmg_handle h, int on) {
 int nmg_set_block_size(nmg_handle h, int block) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
-    NMG_CHECK(block == 1 || block == 2 || block == 4 || block == 8 || block == 16, NMG_EINVAL,
-              "block size must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(block == 1 || block == 2 || block == 4 || block == 8 || block == 16 || block == 32 || block == 64,
+              NMG_EINVAL, "block size must be 1, 2, 4, 8, 16, 32 or 64");
     S->block = block;
   });
 }
@@ -2846,7 +2861,8 @@ int nmg_apply_block(nmg_handle h, int which, int K, const double* X, double* Y) 
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(X && Y, NMG_EINVAL, "null block");
     NMG_CHECK(which >= 0 && which <= 2, NMG_EINVAL, "which must be 0 (A), 1 (B) or 2 (B-solve)");
-    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64, NMG_EINVAL,
+              "K must be 1, 2, 4, 8, 16, 32 or 64");
     const int64_t n = S->n;
     setup_work(*S);
     for (int i = 0; i < 6; ++i) S->bw[i].ensure(n * K);
@@ -2905,9 +2921,9 @@ int nmg_set_kernel_map(nmg_handle mat, int group, int K, int rows_per_wave) {
     if (group > 0 || K == 1) build_row_groups(*M, 0);  // the single-column route is set explicitly
     if (K > 0) {
       int idx = -1;
-      for (int i = 0; i < 5; ++i)
+      for (int i = 0; i < 7; ++i)
         if ((1 << i) == K) idx = i;
-      NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+      NMG_CHECK(idx >= 0, NMG_EINVAL, "K must be 1, 2, 4, 8, 16, 32 or 64");
       NMG_CHECK(rows_per_wave == 0 || rows_per_wave == 1 || rows_per_wave == 2 || rows_per_wave == 4 ||
                     rows_per_wave == 8,
                 NMG_EINVAL, "rows per wavefront must be 0, 1, 2, 4 or 8");
@@ -2981,7 +2997,7 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   // another width leaves it as it is)
   build_tiling(M, K, 0, 1);
   M.wide[idx] = 0;
-  if (M.layout == L_MASS3 && K >= 2) {
+  if (M.layout == L_MASS3 && K >= 2 && K <= 16) {
     for (int G : {4, 8, 16, 32}) {
       M.wide[idx] = G;
       consider(time_launches(s, reps, [&] { k_block_spmv(K, M, x.p, y.p, s, fp32); }), 4, 0, G, 0, 0);
@@ -3019,11 +3035,11 @@ static TuneResult tune_matrix(DevMat& M, int K, int reps, bool fp32, hipStream_t
   // row (a tile must hold one row) up to 64 KB of LDS per workgroup
   int64_t longest = 0;
   for (int64_t r = 0; r < M.nb; ++r) longest = std::max(longest, M.h_browptr[r + 1] - M.h_browptr[r]);
-  const int limit = std::min(65535, tile_cap_limit(K));
+  const int limit = K <= 16 ? std::min(65535, tile_cap_limit(K)) : 0;  // no LDS tiles beyond K = 16
   std::vector<int> caps;
   for (double f = 1.0; f * (double)longest <= (double)limit * 1.0001; f *= 1.28)
     caps.push_back((int)std::ceil(f * (double)longest));
-  if (caps.empty() || caps.back() < limit) caps.push_back(limit);
+  if (limit > 0 && (caps.empty() || caps.back() < limit)) caps.push_back(limit);
   for (int cap : caps) {
     if (cap < longest || cap > limit) continue;
     for (int R : {1, 2, 4, 8}) {
@@ -3073,7 +3089,8 @@ int nmg_csr_autotune(nmg_handle mat, int K, int reps, int fp32, double* seconds,
                      int* cap, int* threads) {
   return guarded([&] {
     DevMat* M = get<DevMat>(mat, K_MAT);
-    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64, NMG_EINVAL,
+              "K must be 1, 2, 4, 8, 16, 32 or 64");
     NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
     hipStream_t s = default_stream();
     if (fp32 && !M->bval32) make_fp32(*M, s);
@@ -3091,7 +3108,8 @@ int nmg_bench_block_chebstep(nmg_handle cheb, int K, int reps, int fp32, double*
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);
     NMG_CHECK(reps >= 1, NMG_EINVAL, "reps must be >= 1");
-    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, NMG_EINVAL, "K must be 1, 2, 4, 8 or 16");
+    NMG_CHECK(K == 1 || K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64, NMG_EINVAL,
+              "K must be 1, 2, 4, 8, 16, 32 or 64");
     NMG_CHECK(fp32 >= 0 && fp32 <= 2, NMG_EINVAL, "fp32 must be 0 (FP64), 1 (FP32 values) or 2 (and FP32 vectors)");
     DevMat* M = C->mat;
     hipStream_t s = default_stream();
@@ -3288,6 +3306,98 @@ int nmg_bench_gemm_tn(int64_t n, int m, int kc, int reps, double* seconds, doubl
     const double sec = time_launches(S.s, reps, [&] { gemm_tn_any(S, n, m, kc, dA.p, n, dB.p, n, dC.p, m); });
     if (seconds) *seconds = sec;
     if (bytes) *bytes = 8.0 * ((double)n * m + (double)n * kc + (double)m * kc);
+  });
+}
+
+// Dense products on host arrays (tests) and their timing: op 0
+// C (n x K) = alpha A (n x m) B (m x K) + beta C, op 1 C (m x K) = A^T B
+// (A n x m, B n x K); kernel 1 = the MFMA kernels, 0 = rocBLAS DGEMM (op 0)
+// or the FMA split reduction (op 1).
+}  // extern "C"
+
+template <class F>
+static void with_dense_solver(int kernel, F&& f) {
+  Solver S;
+  S.s = default_stream();
+  S.borrowed = true;
+  S.mfma = kernel ? 1 : 0;
+  struct Blas {
+    rocblas_handle h = nullptr;
+    ~Blas() {
+      if (h) rocblas_destroy_handle(h);
+    }
+  } bh;
+  if (!kernel) {
+    NMG_LIB(rocblas_create_handle(&bh.h), "rocblas handle");
+    NMG_LIB(rocblas_set_stream(bh.h, S.s), "rocblas stream");
+    S.blas = bh.h;
+  }
+  f(S);
+  S.blas = nullptr;
+}
+
+extern "C" {
+
+int nmg_dense_product(int op, int kernel, int64_t n, int m, int K, double alpha, const double* A, const double* B,
+                      double beta, double* C) {
+  return guarded([&] {
+    NMG_CHECK(op == 0 || op == 1, NMG_EINVAL, "op must be 0 (A B) or 1 (A^T B)");
+    NMG_CHECK(n > 0 && m > 0 && K > 0 && A && B && C, NMG_EINVAL, "bad argument");
+    NMG_CHECK(op == 0 || (alpha == 1.0 && beta == 0.0), NMG_EINVAL, "A^T B takes alpha 1, beta 0");
+    with_dense_solver(kernel, [&](Solver& S) {
+      const int64_t nc = op == 0 ? n : m;
+      DVec dA, dB, dC;
+      dA.ensure(n * m);
+      dB.ensure(op == 0 ? (int64_t)m * K : n * K);
+      dC.ensure(nc * K);
+      NMG_HIP(hipMemcpyAsync(dA.p, A, sizeof(double) * n * m, hipMemcpyHostToDevice, S.s));
+      NMG_HIP(hipMemcpyAsync(dB.p, B, sizeof(double) * (op == 0 ? (int64_t)m * K : n * K), hipMemcpyHostToDevice,
+                             S.s));
+      if (op == 0) {
+        NMG_HIP(hipMemcpyAsync(dC.p, C, sizeof(double) * nc * K, hipMemcpyHostToDevice, S.s));
+        dense_nn(S, n, K, m, alpha, dA.p, n, dB.p, m, beta, dC.p, n, "rocblas_dgemm failed");
+      } else {
+        dense_tn(S, n, m, K, dA.p, n, dB.p, n, dC.p, m);
+      }
+      NMG_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * nc * K, hipMemcpyDeviceToHost, S.s));
+      NMG_HIP(hipStreamSynchronize(S.s));
+    });
+  });
+}
+
+int nmg_bench_dense(int op, int kernel, int64_t n, int m, int K, int reps, double* seconds, double* bytes,
+                    double* flops) {
+  return guarded([&] {
+    NMG_CHECK(op == 0 || op == 1, NMG_EINVAL, "op must be 0 (A B) or 1 (A^T B)");
+    NMG_CHECK(n > 0 && m > 0 && K > 0 && reps >= 1, NMG_EINVAL, "bad argument");
+    with_dense_solver(kernel, [&](Solver& S) {
+      const int64_t nc = op == 0 ? n : m;
+      DVec dA, dB, dC;
+      dA.ensure(n * m);
+      dB.ensure(op == 0 ? (int64_t)m * K : n * K);
+      dC.ensure(nc * K);
+      k_fill(n * m, 0.5, dA.p, S.s);
+      k_fill(op == 0 ? (int64_t)m * K : n * K, 0.25, dB.p, S.s);
+      k_fill(nc * K, 0.0, dC.p, S.s);
+      const double sec = time_launches(S.s, reps, [&] {
+        if (op == 0) dense_nn(S, n, K, m, -1.0, dA.p, n, dB.p, m, 1.0, dC.p, n, "rocblas_dgemm failed");
+        else dense_tn(S, n, m, K, dA.p, n, dB.p, n, dC.p, m);
+      });
+      if (seconds) *seconds = sec;
+      // algorithmic bytes: A once, B once, C read and written (op 0) or written (op 1)
+      if (bytes)
+        *bytes = op == 0 ? 8.0 * ((double)n * m + (double)m * K + 2.0 * n * K)
+                         : 8.0 * ((double)n * m + (double)n * K + (double)m * K);
+      if (flops) *flops = 2.0 * (double)n * m * K;
+    });
+  });
+}
+
+int nmg_set_dense_kernels(nmg_handle h, int mfma) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    NMG_CHECK(mfma == 0 || mfma == 1, NMG_EINVAL, "mode must be 0 or 1");
+    S->mfma = mfma;
   });
 }
 

@@ -98,11 +98,11 @@ struct DevMat {
   // kernel mapping knobs (0 = default): lanes per row of the single-column
   // products; rows per wavefront of the k-column products for K = 1,2,4,8,16
   int group = 0;
-  int block_map[5] = {0, 0, 0, 0, 0};
-  Tiling* tiling[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1,2,4,8,16
+  int block_map[7] = {0, 0, 0, 0, 0, 0, 0};  // per K = 1,2,4,8,16,32,64
+  Tiling* tiling[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // per K = 1..64
   // lanes per node row of the wide k-column scalar-mass kernel (one entry
   // per lane, K columns in registers), per K = 1,2,4,8,16; 0 = not used
-  int wide[5] = {0, 0, 0, 0, 0};
+  int wide[7] = {0, 0, 0, 0, 0, 0, 0};
   bool single_blocked = false;  // single-column products through the K = 1 block kernels
   RowGroups* rgroups = nullptr; // single-column products through the row-group kernels (first choice)
   int pipe = 0;                 // BSR3 single-column rows per pipelined lane group (2, 4, 8; 0 = off)
@@ -248,6 +248,15 @@ size_t dot_work_size();
 size_t gemm_tn_work_size(int64_t n, int m, int K);
 void k_gemm_tn(int64_t n, int m, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
                int64_t ldc, double* work, hipStream_t s);
+// FP64 MFMA products (nmgpu_mfma.hip), column-major operands:
+//   k_gemm_nn_mfma: C (n x K) = alpha A (n x m) B (m x K) + beta C
+//   k_gemm_tn_mfma: C (m x K) = A^T B with A n x m, B n x K (K <= 64), split
+//                   over chunks of rows; work: gemm_tn_mfma_work_size doubles
+void k_gemm_nn_mfma(int64_t n, int K, int m, double alpha, const double* A, int64_t lda, const double* B,
+                    int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+size_t gemm_tn_mfma_work_size(int64_t n, int m, int K);
+void k_gemm_tn_mfma(int64_t n, int m, int K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                    int64_t ldc, double* work, hipStream_t s);
 size_t gemv_work_size(int64_t n, int k);
 
 // ---------------------------------------------------------------------------

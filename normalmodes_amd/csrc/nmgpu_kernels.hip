@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void block_dot_final_k(const double* part, 
 }
 size_t block_dot_work_size(int K) { return (size_t)K * kDotBlocks; }
 void k_block_dots(int64_t n, int K, const double* x, const double* y, double* out, double* work, hipStream_t s) {
-  if (K < 1 || K > 16 || (kBlock % K) != 0) throw Error{NMG_EINVAL, "block dots: K must divide the block size"};
+  if (K < 1 || K > 64 || (kBlock % K) != 0) throw Error{NMG_EINVAL, "block dots: K must divide the block size"};
   hipLaunchKernelGGL(block_dot_part_k, dim3(kDotBlocks), dim3(kBlock), 0, s, n, K, x, y, work);
   hipLaunchKernelGGL(block_dot_final_k, dim3(K), dim3(kBlock), 0, s, work, out);
   NMG_HIP(hipGetLastError());
@@ -1343,7 +1343,7 @@ static void launch_wide_G(const DevMat& M, const VT* v, const XT* x, Epi epi, hi
 // 16-byte load (K >= 2 for FP64 vectors, K >= 4 for FP32)
 template <int K, class XT>
 constexpr bool wide_ok() {
-  return K >= 2 && (3 * K) % (16 / (int)sizeof(XT)) == 0;
+  return K >= 2 && K <= 16 && (3 * K) % (16 / (int)sizeof(XT)) == 0;
 }
 
 template <int K, class VT, class Epi, class XT>
@@ -1369,7 +1369,7 @@ static bool launch_wide(const DevMat& M, const VT* v, const XT* x, Epi epi, hipS
 // rows per wavefront of the k-column kernels: chosen per (matrix, K) by the
 // autotuner (tune_block_map); 0 means "not tuned" and uses R = 1
 static int block_rows(const DevMat& M, int K) {
-  for (int i = 0; i < 5; ++i)
+  for (int i = 0; i < 7; ++i)
     if ((1 << i) == K) return M.block_map[i] > 0 ? M.block_map[i] : 1;
   return 1;
 }
@@ -1434,7 +1434,9 @@ static void dispatch_block(const DevMat& M, const XT* x, Epi epi, hipStream_t s,
     case 4: { constexpr int KK = 4; CALL; break; }                              \
     case 8: { constexpr int KK = 8; CALL; break; }                              \
     case 16: { constexpr int KK = 16; CALL; break; }                            \
-    default: throw Error{NMG_EINVAL, "block width must be 1, 2, 4, 8 or 16"};  \
+    case 32: { constexpr int KK = 32; CALL; break; }                            \
+    case 64: { constexpr int KK = 64; CALL; break; }                            \
+    default: throw Error{NMG_EINVAL, "block width must be 1, 2, 4, 8, 16, 32 or 64"};  \
   }
 
 void k_filter_step(const DevMat& A, const double* x, const double* vk, double* vkm, double* y,
@@ -1760,6 +1762,10 @@ static void launch_tiled_R(const DevMat& M, const Tiling& T, const VT* v, const 
 
 template <int K, class VT, class Epi, class XT>
 bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_t s) {
+  // LDS tiles hold 3K values per node: built for K <= 16 only
+  if constexpr (K > 16) {
+    return false;
+  } else {
   int idx = 0;
   while ((1 << idx) != K) ++idx;
   const Tiling* T = M.tiling[idx];
@@ -1775,6 +1781,7 @@ bool launch_tiled(const DevMat& M, const VT* v, const XT* x, Epi epi, hipStream_
   else launch_tiled_R<K, 1>(M, *T, v, x, epi, s);
   NMG_HIP(hipGetLastError());
   return true;
+  }
 }
 
 }  // namespace nmg

@@ -115,11 +115,12 @@ def ap_chebiter_degree(porder: int, lmin: float, lmax: float) -> int:
 
 
 def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bounds=(1000, 2000, 1e-12),
-                   ap_bounds=(2000, 3000, 1e-12)) -> MatvecSetup:
+                   ap_bounds=(2000, 3000, 1e-12), tune=()) -> MatvecSetup:
     """setupmatvec for a body with fluid regions (src/mod_matvec.f90:24-222):
     B scaled and its Chebyshev solve as in the solid case; Ad, E, ET uploaded;
     Ap negated and scaled (Apdiagscaling), its bounds by LanBounds and its
-    Chebyshev solve."""
+    Chebyshev solve.  ``tune``: block widths for which the layouts of B and
+    Ad are autotuned on the device."""
     t0 = time.perf_counter()
     n = problem.n
     matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
@@ -129,6 +130,11 @@ def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bo
                     matET=pv.csr_create(problem.ET.rowptr, problem.ET.col, problem.ET.val, n_cols=n),
                     matAp=pv.csr_create(problem.Ap.rowptr, problem.Ap.col, problem.Ap.val), dp=None)
     fs.dp = pv.csr_diagscale(fs.matAp, -1.0)                          # :137-140, 345-441
+    tuned = {}
+    for K in tune:
+        for key, mat in (("B", matB), ("Ad", fs.matAd)):
+            if pv.csr_info(mat)["layout"] != "csr":
+                tuned[f"{key}{K}"] = pv.csr_autotune(mat, K, 10)
     hp = pv.pevsl_start(0)                                            # :152
     try:
         pv.pevsl_setprobsizes(hp, problem.n_p, problem.n_p, -1)
@@ -141,6 +147,7 @@ def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bo
     fs.chebAp = pv.pevsl_setup_chebiter(fs.aplmin, fs.aplmax, fs.apdeg, fs.matAp)   # :174
     mv = MatvecSetup(n=n, matA=0, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
                      porder=problem.porder, timings=dict(upload=time.perf_counter() - t0), fs=fs)
+    mv.tuned = tuned
     return setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds)
 
 
@@ -242,7 +249,7 @@ def split_parts(block: int, split="auto") -> int:
 
 
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split="auto", graphs=False, verbose=0, split_filter=True):
+                 split="auto", graphs=False, verbose=0, split_filter=True, dense_kernels=1):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -269,6 +276,8 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
                 pv.set_split_filter(h, False)
         if graphs:
             pv.set_graphs(h, True)
+        if dense_kernels != 1:
+            pv.set_dense_kernels(h, bool(dense_kernels))
         if verbose:
             pv.set_verbose(h, verbose)
         if resid_target is None and precision != pv.FP64:
@@ -289,8 +298,10 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
                 keep: bool = False, split="auto", graphs: bool = False, verbose: int = 0,
-                split_filter: bool = True) -> EigResult:
-    """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator."""
+                split_filter: bool = True, dense_kernels: int = 1) -> EigResult:
+    """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator.
+    ``dense_kernels``: 1 the FP64 MFMA kernels for the block recurrence and the
+    extraction's dense products, 0 rocBLAS / the FMA split reduction."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
     else:
@@ -300,7 +311,7 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
     pol_live = False
     h = _open_solver(mv, precision=precision, seed=seed, rtol=rtol, probe_every=probe_every, block=block,
                      resid_target=resid_target, split=split, graphs=graphs, verbose=verbose,
-                     split_filter=split_filter)
+                     split_filter=split_filter, dense_kernels=dense_kernels)
     try:
         t["setup"] = time.perf_counter() - t0
         t0 = time.perf_counter()

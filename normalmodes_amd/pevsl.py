@@ -431,6 +431,35 @@ def gemm_tn(A, B):
     return C
 
 
+def dense_product(op: int, kernel: int, A, B, C=None, alpha=1.0, beta=0.0):
+    """op 0: alpha A B + beta C (A n x m, B m x K); op 1: A^T B (A n x m,
+    B n x K).  kernel 1: FP64 MFMA kernels; 0: rocBLAS / FMA split reduction."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    n, m = A.shape
+    K = B.shape[1]
+    if op == 0:
+        out = np.asfortranarray(np.zeros((n, K)) if C is None else C, dtype=np.float64).copy(order="F")
+    else:
+        out = np.zeros((m, K), order="F")
+    _check(_lib().nmg_dense_product(ctypes.c_int(op), ctypes.c_int(kernel), _i64(n), ctypes.c_int(m),
+                                    ctypes.c_int(K), _d(alpha), _p(A, _f64p), _p(B, _f64p), _d(beta),
+                                    _p(out, _f64p)))
+    return out
+
+
+def bench_dense(op: int, kernel: int, n: int, m: int, K: int, reps: int = 10):
+    """(seconds per product, algorithmic bytes, flops)."""
+    sec, b, f = _d(), _d(), _d()
+    _check(_lib().nmg_bench_dense(ctypes.c_int(op), ctypes.c_int(kernel), _i64(n), ctypes.c_int(m), ctypes.c_int(K),
+                                  ctypes.c_int(reps), ctypes.byref(sec), ctypes.byref(b), ctypes.byref(f)))
+    return sec.value, b.value, f.value
+
+
+def set_dense_kernels(h: Handle, mfma: bool = True):
+    _check(_lib().nmg_set_dense_kernels(_i64(h), ctypes.c_int(int(mfma))))
+
+
 def bench_copy(n: int, reps: int = 10) -> float:
     """Mean seconds of a plain copy kernel of n doubles (16 n bytes)."""
     sec = _d()

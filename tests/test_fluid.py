@@ -241,7 +241,7 @@ def _band(w, center, n_modes=16):
 def _check_pairs(name, mv, S, B, r, ref, tol):
     assert r.lam.size == ref.size, (name, r.lam.size, ref.size)
     err = np.abs(r.lam - ref) / ref
-    print(f"{name}: {ref.size} pairs, max rel err vs shift-invert {err.max():.2e} (tolerance {tol.min():.1e}"
+    print(f"{name}: {ref.size} pairs, max rel err {err.max():.2e} (tolerance {tol.min():.1e}"
           f"..{tol.max():.1e})", flush=True)
     assert np.all(err <= tol), name
     assert np.all(r.res2 <= 1e-10) and r.target_met
@@ -253,7 +253,10 @@ def _check_pairs(name, mv, S, B, r, ref, tol):
     xb = np.einsum("ij,ji->i", X, Bt @ X.T)
     r2 = np.linalg.norm(R, axis=0) / np.sqrt(xb) / np.abs(r.lam)
     assert np.all(r2 <= 1e-10), r2
-    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + 1e-14)
+    # two evaluations of the same residual differ by its own rounding, about
+    # eps * lam_max / |lam| (the product S~x carries eps * ||S~|| ||x||)
+    floor = 0.1 * np.finfo(float).eps * r.lmax / np.abs(r.lam)
+    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + np.maximum(floor, 1e-14))
 
 
 @pytest.mark.gpu
@@ -268,7 +271,7 @@ def test_assembled_eigenpairs_match_same_operator(assembled):
     ref = orc.shift_invert_eigs(S, B, a, b)
     assert ref.size == w[(w >= a) & (w <= b)].size >= 10
     assert np.finfo(float).eps * w[-1] / ref.min() < 4e-13
-    r = apply_pevsl(mv, interval=(a, b), block=4)
+    r = apply_pevsl(mv, interval=(a, b), block=4, verbose=20)
     _check_pairs(name + " mid-band", mv, S, B, r, ref, np.full(ref.size, 1e-12))
 
 
@@ -290,7 +293,7 @@ def test_assembled_seismic_band(assembled):
     ref = w[(w >= a) & (w <= b)]
     assert ref.size >= 10
     print(f"{name}: seismic band [{a:.6e}, {b:.6e}], {ref.size} eigenvalues", flush=True)
-    r = apply_pevsl(mv, interval=(a, b), block=4)
+    r = apply_pevsl(mv, interval=(a, b), block=4, verbose=20)
     tol = np.maximum(1e-12, 8.0 * np.finfo(float).eps * w[-1] / ref)
     _check_pairs(name + " seismic band", mv, S, B, r, ref, tol)
     assert np.all(r.res2 <= 1e-10) and r.target_met
@@ -302,4 +305,7 @@ def test_assembled_seismic_band(assembled):
     xb = np.einsum("ij,ji->i", X, Bt @ X.T)
     r2 = np.linalg.norm(R, axis=0) / np.sqrt(xb) / np.abs(r.lam)
     assert np.all(r2 <= 1e-10), r2
-    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + 1e-14)
+    # two evaluations of the same residual differ by its own rounding, about
+    # eps * lam_max / |lam| (the product S~x carries eps * ||S~|| ||x||)
+    floor = 0.1 * np.finfo(float).eps * r.lmax / np.abs(r.lam)
+    assert np.all(np.abs(r2 - r.res2) <= 1e-3 * r2 + np.maximum(floor, 1e-14))

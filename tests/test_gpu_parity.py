@@ -357,7 +357,7 @@ def test_c1_matches_shift_invert_golden():
     _check_gate(P, r)
 
 
-@pytest.mark.parametrize("block", [2, 4, 8, 16])
+@pytest.mark.parametrize("block", [2, 4, 8, 16, 32, 64])
 def test_block_lanczos_matches_dense(small_p2, dense_eigs, block):
     """The block filtered Lanczos (same filter, block B-solve, GEMM
     re-orthogonalisation) finds the same eigenpairs as the dense solver."""

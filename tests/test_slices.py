@@ -114,7 +114,7 @@ def test_kpm_moments_vs_oracle(small_p2, mv_small):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("slicer,ns,b", [("uniform", 3, 12.0), ("kpm", 2, 30.0), ("kpm", 4, 60.0),
-                                         ("uniform", 1, 12.0)])
+                                         ("uniform", 1, 12.0), ("kpm", 20, 200.0), ("uniform", 40, 400.0)])
 def test_slices_union_equals_single_interval(small_p2, mv_small, dense_eigs, slicer, ns, b):
     """(Narrow slices at the bottom of a wide spectrum need high filter
     degrees, so the KPM cases use wider bands.)"""

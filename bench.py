@@ -165,14 +165,19 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
 
 
 def reorth_rate(n, m, k, reps=10):
-    """The block re-orthogonalisation product V^T W (V n x m, W n x k):
-    FP64 rate against the FP64 peak and its HBM rate against 8 TB/s."""
+    """The block re-orthogonalisation products on the FP64 matrix cores
+    (hand-written v_mfma_f64_16x16x4f64 kernels): H = V^T W and W -= Z H
+    (V, Z n x m, W n x k).  Rates against the FP64 peak (MFMA utilisation)
+    and against 8 TB/s, with the rocBLAS / FMA kernels they replace."""
     from normalmodes_amd import pevsl as pv
-    s, b = pv.bench_gemm_tn(n, m, k, reps)
-    flops = 2.0 * n * m * k
-    return dict(n=n, m=m, k=k, us=round(s * 1e6, 2), tflops=round(flops / s / 1e12, 2),
-                frac_fp64_peak=round(flops / s / 1e12 / FP64_PEAK_TFLOPS, 4),
-                gbs=round(b / s / 1e9, 1), frac_hbm=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
+    out = dict(n=n, m=m, k=k)
+    for op, name in ((1, "vtw"), (0, "w_minus_zh")):
+        for kernel, tag in ((1, "mfma"), (0, "library")):
+            s, b, f = pv.bench_dense(op, kernel, n, m, k, reps)
+            out[f"{name}_{tag}"] = dict(us=round(s * 1e6, 2), tflops=round(f / s / 1e12, 2),
+                                        frac_fp64_peak=round(f / s / 1e12 / FP64_PEAK_TFLOPS, 4),
+                                        gbs=round(b / s / 1e9, 1), frac_hbm=round(b / s / 1e9 / HBM_PEAK_GBS, 3))
+    return out
 
 
 def spmv_rates(name, reps=50):
@@ -198,6 +203,74 @@ def spmv_rates(name, reps=50):
     finally:
         mv.free()
     out["reorth_gemm"] = reorth_rate(P.n, 400, 8)
+    out["reorth_gemm_k64"] = reorth_rate(P.n, 512, 64, reps=5)
+    return out
+
+
+# the 2M-DoF three-layer ball with a fluid outer core (JOB=2, Earth-sized,
+# radius 6371 km) and the band of its ~200 lowest seismic modes in mHz
+C2_BAND_MHZ = (0.25, 1.3)
+
+
+def c2_components(reps=20):
+    """Measured pieces of the 2M-DoF fluid-solid solve: assembly, set-up,
+    LanBounds, FindPol's degree for the ~200-mode band, one timed filter
+    application (single vector, FP64) and the rates of every product and
+    Chebyshev step of the operator.  A whole solve of this body takes
+    Lanczos steps x filter degree x the per-degree time; it is reported as
+    such, not as a measured headline."""
+    from normalmodes_amd import driver, pevsl as pv
+    from normalmodes_amd.workloads import build_problem
+    t0 = time.perf_counter()
+    Q = build_problem("c2")
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", tune=(1,))
+    t_setup = time.perf_counter() - t0
+    out = dict(workload="c2: three-layer ball (radius 6371 km; solid inner core r<0.19, fluid outer core "
+                        "0.19-0.55, solid mantle), JOB=2 reference gravity, P2, seeded ~450k-tet mesh",
+               n_dof=Q.n, n_pressure=Q.n_p, n_tets=Q.mesh.n_elem,
+               nnz={k: getattr(Q, k).nnz for k in ("Ad", "B", "E", "ET", "Ap")},
+               build_s=round(t_build, 2), setup_s=round(t_setup, 2), bsolve_degree=mv.bdeg,
+               ap_degree=mv.fs.apdeg, kernel_layouts=mv.tuned)
+    try:
+        rates = {}
+        for key, mat in (("Ad", mv.fs.matAd), ("B", mv.matB), ("Ap", mv.fs.matAp), ("E", mv.fs.matE),
+                         ("ET", mv.fs.matET)):
+            sec, b = pv.bench_spmv(mat, reps, False)
+            rates[key] = dict(layout=pv.csr_info(mat)["layout"], us=round(sec * 1e6, 2),
+                              gbs=round(b / sec / 1e9, 1), frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
+        for key, cheb in (("B_chebstep", mv.chebB), ("Ap_chebstep", mv.fs.chebAp)):
+            sec, b = pv.bench_chebstep(cheb, reps, False)
+            rates[key] = dict(us=round(sec * 1e6, 2), gbs=round(b / sec / 1e9, 1),
+                              frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
+        out["rates"] = rates
+        h = driver._open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
+        try:
+            t0 = time.perf_counter()
+            lmin, lmax = pv.pevsl_lanbounds(h, 3000, 5000, 1e-5)
+            out["lanbounds"] = dict(seconds=round(time.perf_counter() - t0, 2), lmin=lmin, lmax=lmax,
+                                    steps=pv.solve_stats(h)["lan_steps"])
+            a, b_ = driver.freq_to_lambda(C2_BAND_MHZ[0]), driver.freq_to_lambda(C2_BAND_MHZ[1])
+            pol = pv.pevsl_findpol(np.array([a, b_, lmin, lmax]), 0.8, 0.7)
+            try:
+                deg = pv.pol_info(pol).deg
+                z = np.random.default_rng(7).standard_normal(mv.n)
+                pv.filter_apply(h, pol, z)
+                t0 = time.perf_counter()
+                pv.filter_apply(h, pol, z)
+                tf = time.perf_counter() - t0
+            finally:
+                pv.pevsl_freepol(pol)
+            out["filter"] = dict(band_mhz=list(C2_BAND_MHZ), interval=[a, b_], degree=deg,
+                                 seconds_per_application=round(tf, 3),
+                                 ms_per_degree=round(tf / deg * 1e3, 3),
+                                 note="one Lanczos step of the single-vector solve applies the filter once; "
+                                      "host copies of one vector included")
+        finally:
+            pv.pevsl_finish(h)
+    finally:
+        mv.free()
     return out
 
 
@@ -227,11 +300,48 @@ def cpu_baseline(P, mv, r, seconds_budget):
     # step (already counted in the GPU's filter_applies) -> per-degree time
     degrees = r.deg * r.filter_applies + r.lan_steps_bounds
     est = per_deg * degrees
-    return dict(value=round(est, 2), unit="s", cores=threads, kind="port",
+    # CPU SpMV rates of the same matrices (oracle CSR product, all threads)
+    x = np.random.default_rng(1).standard_normal(P.n)
+    rates = {}
+    for key, M in (("A", P.A), ("B", P.B)):
+        orc.spmv(M, x)
+        reps = 20
+        t0 = orc.wtime()
+        for _ in range(reps):
+            orc.spmv(M, x)
+        sec = (orc.wtime() - t0) / reps
+        byt = M.nnz * 12.0 + (M.n_rows + 1) * 8.0 + 2 * 8.0 * M.n_rows
+        rates[key] = dict(us=round(sec * 1e6, 1), gbs=round(byt / sec / 1e9, 2))
+    return dict(value=round(est, 2), unit="s", cores=threads, kind="port", cpu_model=cpu_model(),
+                host_threads=os.cpu_count(), spmv_csr_fp64=rates,
                 sample=(f"oracle filter of degree {m} (of {pol.deg}) with B-solve degree {mv.bdeg} on the "
                         f"same {P.n}-DoF operator: {t_s:.1f} s; scaled by the {degrees} filter degrees "
                         f"(ChebLanNr {r.filter_applies} applications x {r.deg} + {r.lan_steps_bounds} "
                         f"LanBounds steps) of the GPU solve"))
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_full_solve(P, interval, bdeg):
+    """One complete single-vector solve of the CPU restatement (oracle) on the
+    same operator and interval: the measurement the bench's scaled estimate
+    is checked against."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    t0 = orc.wtime()
+    lam, _, info = orc.solve_interval(P.A, P.B, interval, bdeg=bdeg, return_info=True)
+    sec = orc.wtime() - t0
+    return dict(seconds=round(sec, 2), nev=int(lam.size), cores=orc.num_threads(), cpu_model=cpu_model(),
+                info={k: (v if isinstance(v, (int, float)) else str(v)) for k, v in info.items()
+                      if k in ("nsteps", "deg", "lmin", "lmax", "lanbounds_steps", "ncand")})
 
 
 def main():
@@ -252,12 +362,17 @@ def main():
     ap.add_argument("--single-compare", type=int, default=1,
                     help="with --block > 1, also time one single-vector (block 1) solve (0 skips)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
+    ap.add_argument("--cpu-full", type=int, default=0,
+                    help="also time one complete single-vector solve of the CPU restatement (several minutes)")
     ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
     ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="copy kernel + the --spmv-workload product rates only, --steps launches each (counter runs)")
     ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint of the solved pairs (0 skips)")
+    ap.add_argument("--c2", type=int, default=1,
+                    help="measure the pieces of the 2M-DoF fluid-solid (c2) solve: set-up, bounds, filter degree, "
+                         "one filter application and the product rates (0 skips)")
     ap.add_argument("--layouts", default="auto",
                     help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
                          "'A:<g>,B:<g>' (g in 8,16,32,64), e.g. for profiler runs, where timing-based tuning "
@@ -418,6 +533,12 @@ def main():
             log(f"[bench] spmv rates: {line['spmv_hbm']}")
         except Exception as e:  # report, but do not lose the headline line
             line["spmv_hbm"] = {"error": str(e)}
+    if args.c2:
+        try:
+            line["c2"] = c2_components()
+            log(f"[bench] c2 pieces: {line['c2']}")
+        except Exception as e:
+            line["c2"] = {"error": str(e)}
     if args.cpu_seconds > 0:
         try:
             # the CPU restatement runs the reference's single-vector ChebLanNr:
@@ -427,6 +548,12 @@ def main():
             log(f"[bench] cpu baseline: {line['cpu_baseline']}")
         except Exception as e:
             line["cpu_baseline"] = {"error": str(e)}
+    if args.cpu_full:
+        try:
+            line["cpu_full_solve"] = cpu_full_solve(P, interval, mv.bdeg)
+            log(f"[bench] cpu full solve: {line['cpu_full_solve']}")
+        except Exception as e:
+            line["cpu_full_solve"] = {"error": str(e)}
     mv.free()
     print(json.dumps(line), flush=True)
 

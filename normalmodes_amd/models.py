@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .mesh import TetMesh
+from .mesh import TetMesh, element_radius
 
 __all__ = ["Layer", "RadialModel", "homogeneous", "three_layer", "three_layer_solid", "fluid_core",
            "element_node_values", "lame", "gaussian_random_field", "mass_within", "reference_gravity",
@@ -117,7 +117,7 @@ def element_node_values(mesh: TetMesh, model: RadialModel, perturb=None):
     X = mesh.nodes / ax[None, :]
     xe = X[mesh.elems]                       # (ne, np, 3)
     rn = np.linalg.norm(xe, axis=2)          # node radius
-    rc = np.linalg.norm(X[mesh.tets].mean(axis=1), axis=1)
+    rc = element_radius(X, np.asarray(mesh.tets, dtype=np.int64))
     li = model.layer_index(rc)
     rho = np.empty_like(rn)
     vp = np.empty_like(rn)

@@ -2,20 +2,24 @@
 
 c1   homogeneous solid unit ball (rho=1, vp=1.73, vs=1), ~12k P2 tets, ~52k DoF
 c2s  three-layer radial ball, all layers solid (rho 13->4, vp 11->8,
-     vs 3.5->6.5 linear in r), ~450k P2 tets, ~2M DoF
+     vs 3.5->6.5 linear in r), ~466k P2 tets, ~1.9M DoF
 c2   the same profiles with a fluid outer core (0.19 < r < 0.55, vs = 0),
      Earth-sized (radius 6371 km, so that self-gravity matters), JOB=2 with
      the model's own reference gravity g0(r) = G M(r)/r^2, interfaces as
-     tetrahedron faces, ~450k P2 tets: 2.02M displacement DoF (six at the
-     fluid-solid nodes) and 128k pressures
-c2r  the same body at ~12k tets (~70k DoF), for tests
+     tetrahedron faces, ~466k P2 tets: ~1.97M displacement DoF (six at the
+     fluid-solid nodes) and ~100k pressures
+c2r  the same body at ~13k tets (~63k DoF), for tests
+c4   the same body refined to ~2M tets (~8.6M DoF), for the batched slices
 tiny homogeneous ball, ~300 P2 tets (tests and smoke)
+c1 and tiny use the jittered-lattice Delaunay mesh; the layered bodies use
+the isosurface-stuffed BCC mesh (no slivers: dihedral angles 18-149 degrees,
+so the largest eigenvalue stays at the lattice spacing's scale).
 Mesh seed 1019, Lanczos start-vector seed 7.
 """
 from __future__ import annotations
 
 from .assemble import assemble
-from .mesh import ball_mesh
+from .mesh import ball_mesh, stuffed_ball_mesh
 from .models import homogeneous, three_layer, three_layer_solid
 
 MESH_SEED = 1019
@@ -27,6 +31,7 @@ SPECS = {
     "c2s": dict(n_tets=450000, model="three_layer_solid"),
     "c2": dict(n_tets=450000, model="three_layer_fluid", job=2),
     "c2r": dict(n_tets=12000, model="three_layer_fluid", job=2),
+    "c4": dict(n_tets=2000000, model="three_layer_fluid", job=2),
 }
 
 
@@ -34,12 +39,14 @@ def build_problem(name: str, porder: int = 2, seed: int = MESH_SEED):
     spec = SPECS[name]
     if spec["model"] == "three_layer_fluid":
         model = three_layer(6371.0)
-        mesh = ball_mesh(spec["n_tets"], porder=porder, seed=seed, interfaces=model.interfaces,
-                         scale=model.radius)
+        mesh = stuffed_ball_mesh(spec["n_tets"], porder=porder, seed=seed, interfaces=model.interfaces,
+                                 scale=model.radius)
         return assemble(mesh, model, job=spec.get("job", 1), name=name)
+    if spec["model"] == "three_layer_solid":
+        mesh = stuffed_ball_mesh(spec["n_tets"], porder=porder, seed=seed)
+        return assemble(mesh, three_layer_solid(), name=name)
     mesh = ball_mesh(spec["n_tets"], porder=porder, seed=seed)
-    model = homogeneous() if spec["model"] == "homogeneous" else three_layer_solid()
-    return assemble(mesh, model, name=name)
+    return assemble(mesh, homogeneous(), name=name)
 
 # Eigenvalue intervals (lambda = omega^2 in the model's units).
 #   c1: a = lambda_7 / 2 (above the six rigid-body zero modes) and b at the

@@ -37,7 +37,7 @@ enum {
 };
 
 /* precision of the Chebyshev filter (nmg_set_precision) */
-enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1, NMG_FP32_BSOLVE = 2 };
+enum { NMG_FP64 = 0, NMG_FP32_FILTER = 1, NMG_FP32_BSOLVE = 2, NMG_FP32_FILTER_A = 3 };
 
 /* Error message of the last failing call on this thread, library version,
  * and the device's name, CU count and HBM size (no reference counterpart:
@@ -110,7 +110,12 @@ int nmg_set_geneig(nmg_handle h);
  * NMG_FP32_BSOLVE: as NMG_FP32_FILTER, and the k-column B-solves of the
  * block and slice paths also hold b and their iterates in FP32 (each
  * product and update formed in FP64, rounded once on store): half the
- * vector bytes of every Chebyshev step.  Call after nmg_set_bsol_chebiter. */
+ * vector bytes of every Chebyshev step.
+ * NMG_FP32_FILTER_A: as NMG_FP32_FILTER, and the filter's stiffness products
+ * (A~, or Ad, E, ET and the Ap~ solve of the fluid-solid operator) read FP32
+ * values with FP64 accumulation and FP64 vectors; Rayleigh-Ritz and the
+ * residual check stay FP64.  Call after nmg_set_bsol_chebiter and the
+ * operator (nmg_set_amv*). */
 int nmg_set_precision(nmg_handle h, int mode);
 /* Lanczos start vector seed (pEVSL draws a random vinit). */
 int nmg_set_seed(nmg_handle h, uint64_t seed);
@@ -311,6 +316,13 @@ int nmg_kernel_probe(nmg_handle h, int which, double *mean_seconds, int64_t *sam
 /* One application of the filter operator of the handle:
  * y = p((A B^-1 - cc)/dd) z (z-space of ChebLanNr) with pol; host vectors. */
 int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double *z, double *y);
+
+/* Mean seconds of one application of the filter to K device-resident
+ * columns (K = 1, 2, 4, .., 64), with the solver's current precision, stream
+ * split and operator; timed over `reps` applications after one warm-up.
+ * No counterpart in the reference: a measurement entry (per-degree cost of
+ * the filter at a given block width). */
+int nmg_bench_filter(nmg_handle h, nmg_handle pol, int K, int reps, double *seconds);
 /* Operators of the handle on host vectors: which = 0 A, 1 B, 2 B-solve. */
 int nmg_apply(nmg_handle h, int which, const double *x, double *y);
 /* The same on K host columns at once (X, Y: K blocks of n, column c at c*n)

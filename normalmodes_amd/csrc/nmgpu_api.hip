@@ -513,6 +513,10 @@ struct FVec {
 //   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))],  C_p ~ Ap~^-1 by Chebyshev
 // iteration, Ap~ = Dp (-Ap) Dp (Apdiagscaling, src/mod_matvec.f90:345-441);
 // i.e. D (Ad - E Ap^-1 ET) D, the displacement Schur complement.
+struct FsWork {
+  DVec t[3], p[3];  // n*K and np*K
+  DVec aw;          // operator output of a split filter chain (n*K)
+};
 struct FsOp {
   DevMat* Ad = nullptr;  // n x n
   DevMat* E = nullptr;   // n x np
@@ -520,7 +524,7 @@ struct FsOp {
   Cheb* cp = nullptr;    // Chebyshev solve with Ap~ (np x np)
   int64_t n = 0, np = 0;
   DVec d, dp;
-  DVec t[3], p[3];  // work: n*K and np*K
+  FsWork part[4];  // per concurrent chain (part 0: the solver's stream)
 };
 
 struct Solver {
@@ -630,9 +634,11 @@ struct Solver {
   }
   bool fp32() const { return precision != NMG_FP64; }                // FP32 values in the B-solve
   bool fp32v() const { return precision == NMG_FP32_BSOLVE; }        // and FP32 vectors (k-column solves)
+  bool fp32a() const { return precision == NMG_FP32_FILTER_A; }      // and FP32 stiffness values in the filter
 };
 
 static void apply_A_block(Solver& S, int K, const double* x, double* y);
+static void apply_fs(FsOp& F, int K, const double* x, double* y, hipStream_t s, FsWork& W, bool f32);
 static void apply_A(Solver& S, const double* x, double* y);
 
 // grow the Lanczos basis to hold `need` columns (copying the first `keep`)
@@ -827,12 +833,12 @@ static void filter_apply(Solver& S, const PolyFilter& P, const double* z, double
       in = b1;
     }
     if (S.fs) {
-      apply_A(S, in, S.w[7].p);
+      apply_fs(*S.fs, 1, in, S.w[7].p, S.s, S.fs->part[0], S.fp32a());
       k_filter_update(n, 1, S.w[7].p, vk, vkm, y, nullptr, P.mu[k], t, P.cc, k == 1, S.s);
     } else {
       const bool pr = S.probe[1].armed();
       if (pr) S.probe[1].before(S.s);
-      k_filter_step(S.A(), in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, false);
+      k_filter_step(S.A(), in, vk, vkm, y, t, P.cc, P.mu[k], k == 1, S.s, S.fp32a());
       if (pr) S.probe[1].after(S.s);
     }
     std::swap(vk, vkm);
@@ -1404,29 +1410,34 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
 
 // A~ applied to K interleaved columns (K = 1: one vector): the sparse product,
 // or the fluid-solid Schur operator
+// the fluid-solid Schur operator on K interleaved columns, on stream s with
+// the work buffers W (one set per concurrent chain)
+static void apply_fs(FsOp& F, int K, const double* x, double* y, hipStream_t s, FsWork& W, bool f32) {
+  for (int i = 0; i < 3; ++i) {
+    W.t[i].ensure(F.n * K);
+    W.p[i].ensure(F.np * K);
+  }
+  double *v0 = W.t[0].p, *w0 = W.t[1].p, *w1 = W.t[2].p;
+  double *x0 = W.p[0].p, *y0 = W.p[1].p, *tp = W.p[2].p;
+  k_rowmul(F.n, K, F.d.p, x, v0, s);        // v0 = D v
+  k_block_spmv(K, *F.Ad, v0, w0, s, f32);   // w0 = Ad v0
+  k_block_spmv(K, *F.ET, v0, x0, s, f32);   // x0 = ET v0
+  k_rowmul(F.np, K, F.dp.p, x0, tp, s);     // x1 = Dp x0
+  block_cheb(*F.cp, K, tp, y0, x0, s, f32, nullptr);  // y0 ~ Ap~^-1 x1
+  k_rowmul(F.np, K, F.dp.p, y0, tp, s);     // y1 = Dp y0
+  k_block_spmv(K, *F.E, tp, w1, s, f32);    // w1 = E y1
+  k_add_rowmul(F.n, K, F.d.p, w0, w1, y, s);  // w = D (w0 + w1)
+  F.cp->ncalls += K;
+  F.cp->nmatvec += (int64_t)F.cp->deg * K;
+}
+
 static void apply_A_block(Solver& S, int K, const double* x, double* y) {
   if (!S.fs) {
     if (K == 1) k_spmv(S.A(), x, y, S.s);
     else k_block_spmv(K, S.A(), x, y, S.s);
     return;
   }
-  FsOp& F = *S.fs;
-  for (int i = 0; i < 3; ++i) {
-    F.t[i].ensure(F.n * K);
-    F.p[i].ensure(F.np * K);
-  }
-  double *v0 = F.t[0].p, *w0 = F.t[1].p, *w1 = F.t[2].p;
-  double *x0 = F.p[0].p, *y0 = F.p[1].p, *tp = F.p[2].p;
-  k_rowmul(F.n, K, F.d.p, x, v0, S.s);        // v0 = D v
-  k_block_spmv(K, *F.Ad, v0, w0, S.s);        // w0 = Ad v0
-  k_block_spmv(K, *F.ET, v0, x0, S.s);        // x0 = ET v0
-  k_rowmul(F.np, K, F.dp.p, x0, tp, S.s);     // x1 = Dp x0
-  block_cheb(*F.cp, K, tp, y0, x0, S.s, false, nullptr);  // y0 ~ Ap~^-1 x1
-  k_rowmul(F.np, K, F.dp.p, y0, tp, S.s);     // y1 = Dp y0
-  k_block_spmv(K, *F.E, tp, w1, S.s);         // w1 = E y1
-  k_add_rowmul(F.n, K, F.d.p, w0, w1, y, S.s);  // w = D (w0 + w1)
-  F.cp->ncalls += K;
-  F.cp->nmatvec += (int64_t)F.cp->deg * K;
+  apply_fs(*S.fs, K, x, y, S.s, S.fs->part[0], false);
 }
 static void apply_A(Solver& S, const double* x, double* y) { apply_A_block(S, 1, x, y); }
 
@@ -1437,13 +1448,13 @@ static void block_filter_step(Solver& S, int K, const double* in, const double* 
                               const double* mu_k, double t, double cc, bool first) {
   if (S.fs) {
     S.aw.ensure(S.n * K);
-    apply_A_block(S, K, in, S.aw.p);
+    apply_fs(*S.fs, K, in, S.aw.p, S.s, S.fs->part[0], S.fp32a());
     k_filter_update(S.n, K, S.aw.p, vk, vkm, y, mu_k, 0.0, t, cc, first, S.s);
     return;
   }
   const bool pr = S.probe[1].armed();
   if (pr) S.probe[1].before(S.s);
-  k_block_filter_step(K, S.A(), in, vk, vkm, y, mu_k, t, cc, first, S.s, false);
+  k_block_filter_step(K, S.A(), in, vk, vkm, y, mu_k, t, cc, first, S.s, S.fp32a());
   if (pr) S.probe[1].after(S.s);
 }
 
@@ -1452,7 +1463,9 @@ static void block_filter_step(Solver& S, int K, const double* in, const double* 
 // own stream; 1 when the filter is not split
 static int filter_parts(const Solver& S, int K) {
   const int P = S.split;
-  return (P > 1 && K % P == 0 && K / P >= 2 && !S.borrowed && !S.fs && S.geneig) ? P : 1;
+  // fluid-solid chains may be single columns: the latency-bound Ap~ solve of
+  // one chain then runs beside the bandwidth-bound B~ solve of another
+  return (P > 1 && K % P == 0 && (K / P >= 2 || S.fs) && !S.borrowed && S.geneig) ? P : 1;
 }
 
 // block_filter with the columns split into P independent chains: the chains
@@ -1502,10 +1515,19 @@ static void block_filter_split(Solver& S, int K, int P, const PolyFilter& Pf, co
       double* b2 = S.pf[q][5].p;
       block_cheb(C, H, vk[q], b1, b2, sq[q], S.fp32(), q == 0 ? &S.probe[0] : nullptr,
                  v32 ? S.hf[q].p : nullptr);
+      if (S.fs) {
+        FsWork& W = S.fs->part[q];
+        W.aw.ensure(nh);
+        apply_fs(*S.fs, H, b1, W.aw.p, sq[q], W, S.fp32a());
+        k_filter_update(S.n, H, W.aw.p, vk[q], vkm[q], yq[q], mq[q] + (size_t)k * H, 0.0, t, Pf.cc, k == 1,
+                        sq[q]);
+        std::swap(vk[q], vkm[q]);
+        continue;
+      }
       const bool pr = q == 0 && S.probe[1].armed();
       if (pr) S.probe[1].before(sq[q]);
       k_block_filter_step(H, S.A(), b1, vk[q], vkm[q], yq[q], mq[q] + (size_t)k * H, t, Pf.cc, k == 1, sq[q],
-                          false);
+                          S.fp32a());
       if (pr) S.probe[1].after(sq[q]);
       std::swap(vk[q], vkm[q]);
     }
@@ -2346,16 +2368,27 @@ int nmg_set_geneig(nmg_handle h) {
 int nmg_set_precision(nmg_handle h, int mode) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
-    NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER || mode == NMG_FP32_BSOLVE, NMG_EINVAL,
-              "unknown precision");
+    NMG_CHECK(mode == NMG_FP64 || mode == NMG_FP32_FILTER || mode == NMG_FP32_BSOLVE || mode == NMG_FP32_FILTER_A,
+              NMG_EINVAL, "unknown precision");
+    if (mode != NMG_FP64) NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "set the B-solve before the precision");
+    if (mode == NMG_FP32_FILTER_A)
+      NMG_CHECK(S->fs || S->adad || S->amat, NMG_ESTATE, "set the operator before the precision");
     S->precision = mode;
     S->drop_graphs();
     if (mode != NMG_FP64) {
-      // FP32 values in the Chebyshev B-solve only: B~ is well conditioned
-      // (kappa ~ 15 at P2), while FP32 rounding of the stiffness would be
-      // amplified by lambda_max/lambda on the low modes
-      NMG_CHECK(S->bsol != nullptr, NMG_ESTATE, "set the B-solve before the precision");
+      // B~ is well conditioned (kappa ~ 15 at P2): its FP32 values cost the
+      // B-solve ~1e-7 relative.  FP32 stiffness values (NMG_FP32_FILTER_A)
+      // perturb the low modes by ~eps32 lambda_max / lambda instead
       make_fp32(*S->bsol->mat, S->s);
+      if (mode == NMG_FP32_FILTER_A) {
+        if (S->fs) {
+          for (DevMat* M : {S->fs->Ad, S->fs->E, S->fs->ET, S->fs->cp->mat})
+            if (!(M->val32 || M->bval32)) make_fp32(*M, S->s);
+        } else {
+          DevMat& A = const_cast<DevMat&>(S->A());
+          if (!(A.val32 || A.bval32)) make_fp32(A, S->s);
+        }
+      }
       NMG_HIP(hipStreamSynchronize(S->s));
     }
   });
@@ -2789,6 +2822,46 @@ int nmg_filter_apply(nmg_handle h, nmg_handle pol, const double* z, double* y) {
     filter_apply(*S, *P, dz.p, dy.p);
     NMG_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->s));
     NMG_HIP(hipStreamSynchronize(S->s));
+  });
+}
+
+int nmg_bench_filter(nmg_handle h, nmg_handle pol, int K, int reps, double* seconds) {
+  return guarded([&] {
+    Solver* S = get<Solver>(h, K_SOLVER);
+    PolyFilter* P = get<PolyFilter>(pol, K_POL);
+    NMG_CHECK(seconds != nullptr, NMG_EINVAL, "null output");
+    NMG_CHECK(K >= 1 && K <= 64 && (K & (K - 1)) == 0 && reps >= 1, NMG_EINVAL, "K must be 1..64, a power of 2");
+    setup_work(*S);
+    const int64_t nk = S->n * K;
+    DVec dz, dy, mu;
+    dz.ensure(nk);
+    dy.ensure(nk);
+    mu.ensure((int64_t)(P->deg + 1) * K);
+    {
+      std::vector<double> m((size_t)(P->deg + 1) * K), z((size_t)nk);
+      for (int k = 0; k <= P->deg; ++k)
+        for (int c = 0; c < K; ++c) m[(size_t)k * K + c] = P->mu[k];
+      uint64_t st = 0x9E3779B97F4A7C15ULL;
+      for (int64_t i = 0; i < nk; ++i) {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        z[(size_t)i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+      }
+      NMG_HIP(hipMemcpyAsync(mu.p, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, S->s));
+      NMG_HIP(hipMemcpyAsync(dz.p, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice, S->s));
+      NMG_HIP(hipStreamSynchronize(S->s));
+    }
+    auto once = [&] {
+      if (K == 1) filter_apply(*S, *P, dz.p, dy.p);
+      else block_filter(*S, K, *P, mu.p, dz.p, dy.p);
+    };
+    once();  // warm: work buffers, streams, FP32 copies
+    NMG_HIP(hipStreamSynchronize(S->s));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) once();
+    NMG_HIP(hipStreamSynchronize(S->s));
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
   });
 }
 

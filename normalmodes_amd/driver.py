@@ -18,7 +18,7 @@ import numpy as np
 from . import pevsl as pv
 from .assemble import FluidProblem
 
-__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "plan_slice_waves", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
@@ -106,21 +106,31 @@ class FluidSetup:
             pv.csr_free(m)
 
 
-def ap_chebiter_degree(porder: int, lmin: float, lmax: float) -> int:
-    """Degree of the Chebyshev solve with Ap~ (src/mod_matvec.f90:166-171):
-    P2: max(min(int(lmax/lmin), 15)*3, 100); P1: 25."""
+def ap_chebiter_degree(porder: int, lmin: float, lmax: float, policy="reference", eps=1e-13) -> int:
+    """Degree of the Chebyshev solve with Ap~.
+
+    ``reference`` (src/mod_matvec.f90:166-171): P2 max(min(int(lmax/lmin),
+    15)*3, 100), P1 25.  ``auto``: the smallest degree whose error bound
+    2 rho^c stays below eps (as for the B-solve).  An integer selects it."""
+    if isinstance(policy, int):
+        return policy
+    if policy == "auto":
+        return chebiter_degree(porder, lmin, lmax, "auto", eps)
+    if policy != "reference":
+        raise ValueError(policy)
     if porder == 2:
         return max(min(int(lmax / lmin), 15) * 3, 100)
     return 25
 
 
 def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bounds=(1000, 2000, 1e-12),
-                   ap_bounds=(2000, 3000, 1e-12), tune=()) -> MatvecSetup:
+                   ap_bounds=(2000, 3000, 1e-12), tune=(), apdeg="reference") -> MatvecSetup:
     """setupmatvec for a body with fluid regions (src/mod_matvec.f90:24-222):
     B scaled and its Chebyshev solve as in the solid case; Ad, E, ET uploaded;
     Ap negated and scaled (Apdiagscaling), its bounds by LanBounds and its
-    Chebyshev solve.  ``tune``: block widths for which the layouts of B and
-    Ad are autotuned on the device."""
+    Chebyshev solve (degree by ``apdeg``, see ``ap_chebiter_degree``).
+    ``tune``: block widths for which the layouts of B and Ad are autotuned on
+    the device."""
     t0 = time.perf_counter()
     n = problem.n
     matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
@@ -143,7 +153,7 @@ def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bo
         fs.aplmin, fs.aplmax = pv.pevsl_lanbounds(hp, *ap_bounds)     # :160-162
     finally:
         pv.pevsl_finish(hp)
-    fs.apdeg = ap_chebiter_degree(problem.porder, fs.aplmin, fs.aplmax)
+    fs.apdeg = ap_chebiter_degree(problem.porder, fs.aplmin, fs.aplmax, apdeg)
     fs.chebAp = pv.pevsl_setup_chebiter(fs.aplmin, fs.aplmax, fs.apdeg, fs.matAp)   # :174
     mv = MatvecSetup(n=n, matA=0, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
                      porder=problem.porder, timings=dict(upload=time.perf_counter() - t0), fs=fs)
@@ -230,6 +240,7 @@ class EigResult:
     slice_counts: np.ndarray | None = None  # pairs per slice
     dos_count: float | None = None          # KPM estimate of the count in the interval
     target_met: bool = True                 # every pair met the residual target at the stop
+    waves: list | None = None               # slice ranges solved together (apply_slices with a budget)
 
     @property
     def freq_mhz(self):
@@ -377,18 +388,43 @@ def kpm_degree(xintv, n_slices: int, lo: int = 50, hi: int = 3000) -> int:
     return int(min(hi, max(lo, math.ceil(2.0 * math.pi * n_slices / max(dth, 1e-12)))))
 
 
+def plan_slice_waves(counts, n: int, budget_bytes: float, *, vectors_per_pair: float = 3.0, extra: int = 40,
+                     max_width: int = 64):
+    """Consecutive slices grouped into waves whose Lanczos bases fit a memory
+    budget.  A slice holding c pairs is taken to build about
+    vectors_per_pair * c + extra basis vectors, each kept twice (V and
+    Z = B~V, FP64: 16 n bytes).  A wave holds at most ``max_width`` slices;
+    a slice whose own basis exceeds the budget forms a wave by itself.
+    Returns [(first, last + 1), ...] over the slice indices."""
+    per = 16.0 * n
+    need = [(vectors_per_pair * float(c) + extra) * per for c in counts]
+    waves, i = [], 0
+    while i < len(need):
+        j, tot = i, 0.0
+        while j < len(need) and j - i < max_width and (j == i or tot + need[j] <= budget_bytes):
+            tot += need[j]
+            j += 1
+        waves.append((i, j))
+        i = j
+    return waves
+
+
 def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm", kpm_deg: int | None = None,
                  kpm_nvec: int = 16, tol: float = 1e-5, thr_int: float = 0.8, thr_ext: float = 0.7,
                  lanbounds=(3000, 5000, 1e-5), precision: int = pv.FP64, seed: int = 7, rtol: float = 1e-9,
                  vectors: bool = True, residual_check: bool = True, probe_every: int = 0,
                  resid_target: float | None = None, keep: bool = False, split="auto",
-                 verbose: int = 0) -> EigResult:
+                 verbose: int = 0, basis_budget: float | None = None) -> EigResult:
     """The interval split into n_slices spectrum slices, solved in lockstep.
 
     Slice boundaries: ``kpm`` places them at equal counts of the kernel
     polynomial density of states (pEVSL kpmdos + spslicer); ``uniform``
     splits the interval evenly.  Each slice runs ChebLanNr with its own
-    FindPol filter; every sparse product is shared by all slices.
+    FindPol filter; every sparse product is shared by the slices of a wave.
+
+    ``basis_budget`` (bytes): the slices run in waves of consecutive slices
+    whose estimated Lanczos bases fit it (``plan_slice_waves``, counts from
+    the KPM density of states); without it all slices form one wave.
     """
     a, b = interval
     t = {}
@@ -407,47 +443,73 @@ def apply_slices(mv: MatvecSetup, *, interval, n_slices: int, slicer: str = "kpm
         xintv = np.array([a, b, lmin, lmax])
         t0 = time.perf_counter()
         ecnt = None
-        if slicer == "kpm":
+        mu = None
+        if slicer == "kpm" or basis_budget is not None:
             mdeg = kpm_deg or kpm_degree(xintv, n_slices)
             mu, ecnt = pv.kpmdos(h, mdeg, kpm_nvec, xintv)
+        if slicer == "kpm":
             sli = pv.spslicer(mu, xintv, n_slices)
         elif slicer == "uniform":
             sli = np.linspace(a, b, n_slices + 1)
         else:
             raise ValueError(slicer)
+        waves = [(0, n_slices)]
+        if basis_budget is not None:
+            est = [max(0.0, pv.dos_count(mu, [sli[i], sli[i + 1], lmin, lmax])) for i in range(n_slices)]
+            waves = plan_slice_waves(est, mv.n, basis_budget)
         t["slicer"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         nev = EVINT + 2
         maxit = 3 * max(4 * nev, 2000)
-        pv.solve_slices(h, sli, lmin, lmax, thr_int, thr_ext, maxit, tol)
-        t["slices"] = time.perf_counter() - t0
-        st = pv.solve_stats(h)
-        target_met, _ = pv.solve_quality(h)
-        counts = pv.slice_counts(h, n_slices)
+        lams, Xs, r1s, r2s, counts = [], [], [], [], []
+        steps = applies = ncand = 0
+        target_met = True
         probe = {}
-        if probe_every > 0:
-            for key, which in (("chebstep", 0), ("filterstep", 1)):
-                sec, ns, by = pv.kernel_probe(h, which)
-                probe[key] = dict(seconds=sec, samples=ns, bytes=by)
-        t0 = time.perf_counter()
-        if residual_check:
-            r1, r2 = pv.residuals(h)
-        else:
-            k = pv.pevsl_get_nev(h)
-            r1 = r2 = np.full(k, np.nan)
-        t["residual"] = time.perf_counter() - t0
-        lam, X = pv.pevsl_copy_result(h, mv.n, vectors)
+        t["residual"] = 0.0
+        for (i0, i1) in waves:
+            if len(waves) > 1:
+                wv = 1 << max(0, (i1 - i0 - 1).bit_length())
+                pv.set_split_streams(h, split_parts(wv, split))
+            pv.solve_slices(h, sli[i0:i1 + 1], lmin, lmax, thr_int, thr_ext, maxit, tol)
+            st = pv.solve_stats(h)
+            steps += st["lan_steps"]
+            applies += st["filter_applies"]
+            ncand += st["ncand"]
+            target_met = target_met and pv.solve_quality(h)[0]
+            counts.append(pv.slice_counts(h, i1 - i0))
+            if probe_every > 0 and not probe:
+                for key, which in (("chebstep", 0), ("filterstep", 1)):
+                    sec, ns, by = pv.kernel_probe(h, which)
+                    probe[key] = dict(seconds=sec, samples=ns, bytes=by)
+            t1 = time.perf_counter()
+            if residual_check:
+                r1, r2 = pv.residuals(h)
+            else:
+                k = pv.pevsl_get_nev(h)
+                r1 = r2 = np.full(k, np.nan)
+            t["residual"] += time.perf_counter() - t1
+            lam, X = pv.pevsl_copy_result(h, mv.n, vectors)
+            lams.append(lam)
+            Xs.append(X)
+            r1s.append(r1)
+            r2s.append(r2)
+        t["slices"] = time.perf_counter() - t0 - t["residual"]
     except BaseException:
         pv.pevsl_finish(h)
         raise
     if not keep:
         pv.pevsl_finish(h)
+    lam = np.concatenate(lams)
+    X = None if not vectors else np.concatenate(Xs, axis=0)
+    r1 = np.concatenate(r1s)
+    r2 = np.concatenate(r2s)
     order = np.argsort(lam, kind="stable")
     return EigResult(lam=lam[order], X=None if X is None else X[order], res_ref=r1[order], res2=r2[order],
-                     lmin=lmin, lmax=lmax, xintv=xintv, deg=0, bar=float("nan"), lan_steps=st["lan_steps"],
-                     filter_applies=st["filter_applies"], ncand=st["ncand"], timings=t,
+                     lmin=lmin, lmax=lmax, xintv=xintv, deg=0, bar=float("nan"), lan_steps=steps,
+                     filter_applies=applies, ncand=ncand, timings=t,
                      lan_steps_bounds=lb_steps, probe=probe, handle=h if keep else 0, slices=sli,
-                     slice_counts=counts, dos_count=ecnt, target_met=target_met)
+                     slice_counts=np.concatenate(counts), dos_count=ecnt, target_met=target_met,
+                     waves=waves)
 
 
 def solve_input(inputdir, basename, *, porder: int = 1, job: int = 1, lowfreq: float = 0.2, upfreq: float = 2.0,

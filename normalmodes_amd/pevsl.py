@@ -23,11 +23,12 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "LAYOUTS"]
+           "bench_bsolve", "bench_filter", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
 FP32_BSOLVE = 2   # FP32 values and FP32 vectors in the k-column B-solves
+FP32_FILTER_A = 3  # FP32 values of B~ in the B-solve and of the stiffness in the filter products
 LAYOUTS = {0: "csr", 1: "bsr3", 2: "mass3"}
 
 _i64 = ctypes.c_int64
@@ -388,6 +389,13 @@ def apply(h: Handle, which: int, x):
     y = np.empty_like(x)
     _check(_lib().nmg_apply(_i64(h), ctypes.c_int(which), _p(x, _f64p), _p(y, _f64p)))
     return y
+
+
+def bench_filter(h: Handle, pol: Handle, K: int = 1, reps: int = 3) -> float:
+    """Mean seconds of one filter application to K device-resident columns."""
+    sec = _d()
+    _check(_lib().nmg_bench_filter(_i64(h), _i64(pol), ctypes.c_int(K), ctypes.c_int(reps), ctypes.byref(sec)))
+    return sec.value
 
 
 def apply_block(h: Handle, which: int, X):

@@ -42,6 +42,17 @@ def test_ap_degree_is_100_for_p2():
     assert driver.ap_chebiter_degree(1, 1.0, 50.0) == 25
 
 
+def test_ap_degree_auto_meets_its_bound():
+    """apdeg="auto": the smallest c with 2 rho^c <= 1e-13 for the Ap~ bounds."""
+    for lo, hi in ((0.28, 4.35), (0.1, 9.0), (1.0, 1.5)):
+        c = driver.ap_chebiter_degree(2, lo, hi, "auto")
+        rho = (math.sqrt(hi / lo) - 1) / (math.sqrt(hi / lo) + 1)
+        assert 2 * rho ** c <= 1e-13 < 2 * rho ** (c - 1)
+    assert driver.ap_chebiter_degree(2, 0.28, 4.35, 77) == 77
+    with pytest.raises(ValueError):
+        driver.ap_chebiter_degree(2, 0.28, 4.35, "fast")
+
+
 def test_interval_conversion_keeps_the_reference_single_precision():
     """lowfreq/upfreq and PI are default REALs in the reference
     (src/mod_para.f90:51-52, src/mod_pevsl.f90:43): the interval differs from
@@ -66,3 +77,28 @@ def test_kpm_degree_grows_with_the_slices_and_stays_bounded():
     d = [driver.kpm_degree(xintv, s) for s in (1, 2, 4, 8, 16)]
     assert d == sorted(d) and 50 <= d[0] and d[-1] <= 3000
     assert driver.kpm_degree((0.5, 0.5 + 1e-9, 0.1, 1.0e4), 4) == 3000
+
+
+def test_slice_waves_partition_and_fit():
+    """Waves are consecutive, cover every slice once, and each fits the
+    budget unless it is a single slice too large by itself."""
+    n = 8_600_000
+    rng = np.random.default_rng(4)
+    counts = rng.integers(20, 251, size=16)              # a C4-like spread of slice counts
+    per = 16.0 * n
+    for budget in (50e9, 200e9, 1e15):
+        waves = driver.plan_slice_waves(counts, n, budget)
+        assert waves[0][0] == 0 and waves[-1][1] == counts.size
+        assert all(w[1] == v[0] for w, v in zip(waves, waves[1:]))
+        for i0, i1 in waves:
+            need = sum((3.0 * c + 40) * per for c in counts[i0:i1])
+            assert need <= budget or i1 - i0 == 1
+        if budget == 1e15:
+            assert waves == [(0, 16)]
+    # at 200 GB (a 288 GB device holding the operator and work besides) the
+    # 16 slices need several waves: the whole basis would not fit at once
+    total = sum((3.0 * c + 40) * per for c in counts)
+    assert total > 288e9
+    assert len(driver.plan_slice_waves(counts, n, 200e9)) >= 2
+    # width cap
+    assert all(i1 - i0 <= 4 for i0, i1 in driver.plan_slice_waves(np.ones(10), 100, 1e12, max_width=4))

@@ -276,6 +276,32 @@ def test_assembled_eigenpairs_match_same_operator(assembled):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("block,split", [(2, 2), (4, 2), (4, 4)])
+def test_assembled_split_chains_and_auto_ap_degree(assembled, block, split):
+    """Filter chains of the fluid-solid operator on concurrent streams (one
+    or two columns each) with the Ap~ solve at the degree of its own error
+    bound (apdeg="auto", below the reference's 100): the same mid-band
+    eigenpairs as the degree-100 operator's shift-invert reference, 1e-12."""
+    from normalmodes_amd.driver import apply_pevsl, setupmatvec_fs
+    name, Q, mv, S, B, w = assembled
+    if name != "core-p2-job2":
+        pytest.skip("one body suffices for the stream split")
+    a, b = _band(w, w[-1] / 1000.0)
+    ref = orc.shift_invert_eigs(S, B, a, b)
+    mv2 = setupmatvec_fs(Q, bdeg="auto", apdeg="auto")
+    try:
+        assert mv2.fs.apdeg < mv.fs.apdeg == 100
+        r = apply_pevsl(mv2, interval=(a, b), block=block, split=split, verbose=20)
+    finally:
+        mv2.free()
+    assert r.lam.size == ref.size
+    err = np.max(np.abs(r.lam - ref) / ref)
+    print(f"{name} block {block} split {split} apdeg {mv2.fs.apdeg}: max rel err {err:.2e}", flush=True)
+    assert err < 1e-12
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+
+
+@pytest.mark.gpu
 def test_assembled_seismic_band(assembled):
     """The ~20 lowest seismic modes above the fluid's zero-energy cluster.
     There lam_max / lam is ~2e4 for these Earth-sized bodies, and every FP64

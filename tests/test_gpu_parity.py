@@ -416,6 +416,30 @@ def test_fp32_bsolve_eigenvalues_match_fp64(small_p2, dense_eigs):
 
 
 @pytest.mark.parametrize("block", [1, 8])
+def test_fp32_stiffness_filter_error_stated(small_p2, dense_eigs, block):
+    """FP32 values of A~ in the filter products as well (NMG_FP32_FILTER_A):
+    the filtered subspace then belongs to the FP32-rounded pencil, whose low
+    modes move by ~eps32 lam_max / lam relative to their gaps; Rayleigh-Ritz
+    and the residual stay FP64, so the eigenvalues keep the error quadratic
+    in that vector error.  The per-eigenvalue error against the FP64 path
+    and the residual are printed and bounded, not gated at 1e-10."""
+    from normalmodes_amd import pevsl as pv
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl
+    mv = setupmatvec(small_p2, bdeg="auto")
+    try:
+        r64 = apply_pevsl(mv, interval=(1.0, 12.0), block=block)
+        r32 = apply_pevsl(mv, interval=(1.0, 12.0), block=block, precision=pv.FP32_FILTER_A)
+    finally:
+        mv.free()
+    assert r32.lam.size == r64.lam.size
+    err = np.abs(r32.lam - r64.lam) / r64.lam
+    print(f"FP32 stiffness filter, block {block}: max rel eig err {err.max():.2e}, "
+          f"max residual {r32.res2.max():.2e}", flush=True)
+    assert err.max() < 1e-6
+    assert np.all(r32.res2 < 1e-2)
+
+
+@pytest.mark.parametrize("block", [1, 8])
 def test_empty_interval_stops_early(small_p2, dense_eigs, block):
     """An interval inside a spectral gap: no pair, and the stop comes from the
     converged largest Ritz value, not from maxit (single-vector and block)."""

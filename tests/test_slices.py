@@ -141,6 +141,25 @@ def test_slices_union_equals_single_interval(small_p2, mv_small, dense_eigs, sli
 
 
 @pytest.mark.gpu
+def test_slice_waves_union_equals_single_interval(small_p2, mv_small, dense_eigs):
+    """Eight slices under a basis budget that fits about two at a time: the
+    waves run one after another and their union is the single interval's."""
+    from normalmodes_amd.driver import apply_pevsl, apply_slices
+    mv = mv_small
+    w = dense_eigs(small_p2)
+    ref = w[(w >= 1.0) & (w <= 60.0)]
+    r = apply_slices(mv, interval=(1.0, 60.0), n_slices=8, slicer="uniform", kpm_nvec=16,
+                     basis_budget=5.0e6)
+    assert len(r.waves) >= 3, r.waves
+    assert r.lam.size == ref.size
+    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
+    assert np.all(r.res2 <= 1e-10)
+    assert int(np.sum(r.slice_counts)) == r.lam.size
+    s = apply_pevsl(mv, interval=(1.0, 60.0), vectors=False)
+    assert np.max(np.abs(s.lam - r.lam) / s.lam) < 1e-12
+
+
+@pytest.mark.gpu
 def test_slices_boundary_through_multiplet(small_p2, mv_small, dense_eigs):
     """A slice boundary placed exactly on an eigenvalue: the pair is kept once."""
     from normalmodes_amd.driver import apply_slices

@@ -148,6 +148,17 @@ int nmg_chebiter_free(nmg_handle cheb);
  * steps are separated by grid barriers; slower on MI355X at the sizes
  * measured, because of the barrier cost). */
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
+/* Mixed-precision refinement of the Chebyshev solve (no reference
+ * counterpart): with passes >= 1 every solve runs `passes` inner Chebyshev
+ * solves of degree inner_deg on FP32 values and FP32 iterates (FP64 row
+ * accumulation), pass p on the FP64 residual b - B~x of the sum of the
+ * previous passes, the sum kept in FP64.  Each pass divides the error by
+ * about max(2 rho^inner_deg, the FP32 floor ~1e-6), so two passes reach
+ * ~1e-12 with two FP32 solves and one FP64 product.  passes = 0 restores the
+ * plain FP64 solve of degree deg.  Applies to the solver's B-solve and to
+ * the Ap~ solve of the fluid-solid operator alike (nmg_chebiter_noalloc
+ * keeps the plain form). */
+int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg);
 /* persistent: 1 if the persistent solve is planned and selected, 2 if it
  * could be selected (mode 0 now), 0 if it does not apply; grid, LDS bytes
  * (no reference counterpart) */

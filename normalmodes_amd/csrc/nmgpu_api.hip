@@ -413,6 +413,29 @@ DevMat* clone_devmat(const DevMat& S, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+struct DVec {
+  double* p = nullptr;
+  int64_t n = 0;
+  void ensure(int64_t m) {
+    if (m <= n) return;
+    dfree(p);
+    p = dmalloc<double>(m);
+    n = m;
+  }
+  ~DVec() { dfree(p); }
+};
+struct FVec {
+  float* p = nullptr;
+  int64_t n = 0;
+  void ensure(int64_t m) {
+    if (m <= n) return;
+    dfree(p);
+    p = dmalloc<float>(m);
+    n = m;
+  }
+  ~FVec() { dfree(p); }
+};
+
 struct Cheb {
   DevMat* mat = nullptr;
   double lmin = 0, lmax = 0;
@@ -425,6 +448,11 @@ struct Cheb {
   // workgroups) to 62 us (256 workgroups), against ~5.6 us for a whole
   // fused step launch at 52k DoF, so the per-step launches are the default.
   int mode = 0;
+  // mixed-precision refinement (0: off): `refine` passes, each an inner
+  // Chebyshev solve of degree inner_deg with FP32 values and FP32 iterates
+  // on the FP64 residual b - B~x of the sum so far, added in FP64
+  int refine = 0, inner_deg = 0;
+  FVec own32;           // inner work (3 n K floats) when the caller brings none
   PersistPlan plan[2];  // FP64 / FP32 values
   bool planned[2] = {false, false};
   const PersistPlan* persistent(bool fp32) {
@@ -486,28 +514,6 @@ struct KernelProbe {
   }
 };
 
-struct DVec {
-  double* p = nullptr;
-  int64_t n = 0;
-  void ensure(int64_t m) {
-    if (m <= n) return;
-    dfree(p);
-    p = dmalloc<double>(m);
-    n = m;
-  }
-  ~DVec() { dfree(p); }
-};
-struct FVec {
-  float* p = nullptr;
-  int64_t n = 0;
-  void ensure(int64_t m) {
-    if (m <= n) return;
-    dfree(p);
-    p = dmalloc<float>(m);
-    n = m;
-  }
-  ~FVec() { dfree(p); }
-};
 
 // Fluid-solid operator of sparsefsAV (src/mod_matvec.f90:498-520):
 //   w = D [Ad (D v) + E (Dp C_p (Dp ET (D v)))],  C_p ~ Ap~^-1 by Chebyshev
@@ -762,8 +768,54 @@ static void cheb_steps(const Cheb& C, const double* b, double* out, double* tmp,
   }
 }
 
+// x ~ B~^-1 b on K interleaved columns by mixed-precision refinement:
+//   x_1 = S(b),  x_p = x_{p-1} + S(b - B~ x_{p-1})
+// S an inner Chebyshev solve of degree inner_deg on FP32 values and FP32
+// iterates (FP64 row accumulation), the residual and the sum in FP64.  Each
+// pass divides the error by max(2 rho^inner_deg, the FP32 floor ~1e-6).
+static void refine_solve(Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s,
+                         KernelProbe* probe, float* w32) {
+  const int64_t nk = C.mat->nrow * K;
+  if (!w32) {
+    C.own32.ensure(3 * nk);
+    w32 = C.own32.p;
+  }
+  float* b32 = w32;
+  float* fa = w32 + nk;
+  float* fb = w32 + 2 * nk;
+  const double th = 0.5 * (C.lmax + C.lmin), de = 0.5 * (C.lmax - C.lmin), sg = th / de;
+  const bool pr = probe && probe->armed();
+  for (int p = 0; p < C.refine; ++p) {
+    const double* rhs = b;
+    if (p > 0) {
+      k_block_spmv(K, *C.mat, out, tmp, s, false);  // FP64 values
+      k_sub_from(nk, b, tmp, s);
+      rhs = tmp;
+    }
+    float* fk = fa;
+    float* fm = fb;
+    k_cheb32_start(nk, 1.0 / th, rhs, b32, fk, fm, s);
+    const int mid = (C.inner_deg + 1) / 2;
+    double w = 2.0;
+    for (int k = 1; k <= C.inner_deg; ++k) {
+      w = 1.0 / (1.0 - w / (4.0 * sg * sg));
+      if (pr && p == 0 && k == mid) probe->before(s);
+      k_block_cheb_step32(K, *C.mat, b32, fk, fm, w, 1.0 / th, s, true);
+      if (pr && p == 0 && k == mid) probe->after(s);
+      std::swap(fk, fm);
+    }
+    if (p == 0) k_f32_to_f64(nk, fk, out, s);
+    else k_add_f32(nk, fk, out, s);
+  }
+  if (probe) {
+    probe->bytes = C.mat->matrix_bytes(true) + 4.0 * 4.0 * (double)nk;
+    probe->launches = 1;
+  }
+}
+
 static void cheb_solve(Cheb& C, const double* b, double* out, double* tmp, hipStream_t s,
                        bool fp32, KernelProbe* probe = nullptr) {
+  if (C.refine > 0) return refine_solve(C, 1, b, out, tmp, s, probe, nullptr);
   const int64_t n = C.mat->nrow;
   const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)n;
   if (const PersistPlan* P = C.persistent(fp32)) {
@@ -1291,6 +1343,7 @@ static void do_cheblannr(Solver& S, const double* xintv, int maxit, double tol, 
 // and the update still formed in FP64 per entry): half the vector bytes.
 static void block_cheb(Cheb& C, int K, const double* b, double* out, double* tmp, hipStream_t s, bool fp32,
                        KernelProbe* probe, float* w32 = nullptr) {
+  if (C.refine > 0) return refine_solve(C, K, b, out, tmp, s, probe, w32);
   if (K == 1 && !w32) return cheb_solve(C, b, out, tmp, s, fp32, probe);
   const int64_t nk = C.mat->nrow * K;
   const double step_bytes = C.mat->matrix_bytes(fp32) + 4.0 * 8.0 * (double)nk;
@@ -3492,6 +3545,20 @@ int nmg_chebiter_set_mode(nmg_handle cheb, int mode) {
     Cheb* C = get<Cheb>(cheb, K_CHEB);
     NMG_CHECK(mode == 0 || mode == 1, NMG_EINVAL, "mode must be 0 (one launch per step) or 1 (persistent)");
     C->mode = mode;
+  });
+}
+
+int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg) {
+  return guarded([&] {
+    Cheb* C = get<Cheb>(cheb, K_CHEB);
+    NMG_CHECK(passes == 0 || (passes >= 1 && passes <= 8 && inner_deg >= 1 && inner_deg <= 10000), NMG_EINVAL,
+              "passes must be 0 (off) or 1..8 with an inner degree >= 1");
+    if (passes > 0 && !(C->mat->val32 || C->mat->bval32)) {
+      make_fp32(*C->mat, default_stream());
+      NMG_HIP(hipStreamSynchronize(default_stream()));
+    }
+    C->refine = passes;
+    C->inner_deg = passes > 0 ? inner_deg : 0;
   });
 }
 

@@ -223,6 +223,8 @@ void k_block_cheb_step32(int K, const DevMat& M, const float* b, const float* xk
                          double ith, hipStream_t s, bool fp32);
 void k_cheb32_start(int64_t n, double ith, const double* b, float* b32, float* x1, float* x0, hipStream_t s);
 void k_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t s);
+void k_sub_from(int64_t n, const double* b, double* y, hipStream_t s);   // y = b - y
+void k_add_f32(int64_t n, const float* x, double* y, hipStream_t s);     // y += x
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32 = false);
 void k_block_filter_step(int K, const DevMat& A, const double* x, const double* vk, double* vkm, double* y,

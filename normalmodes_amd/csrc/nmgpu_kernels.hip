@@ -1472,6 +1472,24 @@ void k_cheb32_start(int64_t n, double ith, const double* b, float* b32, float* x
   hipLaunchKernelGGL(cheb32_start_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ith, b, b32, x1, x0);
   NMG_HIP(hipGetLastError());
 }
+// y = b - y (the residual of a refinement pass, after y = B~x)
+__global__ void sub_from_k(int64_t n, const double* __restrict__ b, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = b[i] - y[i];
+}
+void k_sub_from(int64_t n, const double* b, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(sub_from_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b, y);
+  NMG_HIP(hipGetLastError());
+}
+// y += x (an FP32 correction added to an FP64 solution)
+__global__ void add_f32_k(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] += (double)x[i];
+}
+void k_add_f32(int64_t n, const float* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(add_f32_k, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y);
+  NMG_HIP(hipGetLastError());
+}
 __global__ void f32_to_f64_k(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     y[i] = (double)x[i];

@@ -18,7 +18,7 @@ import numpy as np
 from . import pevsl as pv
 from .assemble import FluidProblem
 
-__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "plan_slice_waves", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
+__all__ = ["MatvecSetup", "FluidSetup", "FluidProblem", "setupmatvec_fs", "ap_chebiter_degree", "upload", "setup_bsolve", "setupmatvec", "apply_pevsl", "apply_slices", "plan_slice_waves", "refine_bsolves", "refine_inner_degree", "kpm_degree", "solve_input", "EigResult", "chebiter_degree",
            "freq_to_lambda", "lambda_to_freq"]
 
 EVINT = 800
@@ -56,6 +56,30 @@ def chebiter_degree(porder: int, blmin: float, blmax: float, policy="reference",
         rho = (math.sqrt(k) - 1.0) / (math.sqrt(k) + 1.0)
         return max(1, int(math.ceil(math.log(eps / 2.0) / math.log(rho))))
     raise ValueError(policy)
+
+
+def refine_inner_degree(lmin: float, lmax: float, per_pass: float = 1e-6) -> int:
+    """Inner degree of a refinement pass: the smallest c with 2 rho^c <= per_pass
+    (below that the FP32 iterates' own floor, ~1e-6, dominates)."""
+    k = lmax / lmin
+    rho = (math.sqrt(k) - 1.0) / (math.sqrt(k) + 1.0)
+    return max(1, int(math.ceil(math.log(per_pass / 2.0) / math.log(rho))))
+
+
+def refine_bsolves(mv: "MatvecSetup", passes: int = 2, per_pass: float = 1e-6, ap: bool = True) -> dict:
+    """Switch the B~ solve (and the Ap~ solve of a fluid-solid set-up) to
+    mixed-precision refinement: ``passes`` FP32 inner solves on FP64
+    residuals (``pevsl.chebiter_set_refine``); passes = 0 restores the plain
+    FP64 solves.  Returns the inner degrees."""
+    out = {}
+    deg = refine_inner_degree(mv.blmin, mv.blmax, per_pass) if passes else 0
+    pv.chebiter_set_refine(mv.chebB, passes, deg)
+    out["B"] = deg
+    if ap and mv.fs is not None:
+        dp = refine_inner_degree(mv.fs.aplmin, mv.fs.aplmax, per_pass) if passes else 0
+        pv.chebiter_set_refine(mv.fs.chebAp, passes, dp)
+        out["Ap"] = dp
+    return out
 
 
 @dataclass

@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_filter", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
+           "bench_bsolve", "bench_filter", "chebiter_set_refine", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -389,6 +389,12 @@ def apply(h: Handle, which: int, x):
     y = np.empty_like(x)
     _check(_lib().nmg_apply(_i64(h), ctypes.c_int(which), _p(x, _f64p), _p(y, _f64p)))
     return y
+
+
+def chebiter_set_refine(cheb: Handle, passes: int, inner_deg: int = 0):
+    """Mixed-precision refinement of a Chebyshev solve: ``passes`` inner FP32
+    solves of degree ``inner_deg`` on FP64 residuals (0 passes: plain FP64)."""
+    _check(_lib().nmg_chebiter_set_refine(_i64(cheb), ctypes.c_int(passes), ctypes.c_int(inner_deg)))
 
 
 def bench_filter(h: Handle, pol: Handle, K: int = 1, reps: int = 3) -> float:

@@ -18,6 +18,9 @@ Parity pins (tests/test_oracle_*.py):
                        (eigenvalues to 1e-12 relative, same count)
   kpm_count            quadrature of the damped density vs the closed-form
                        integral; exact moments of a known spectrum
+  FluidPencil          the dense Schur complement D (Ad - E Ap^-1 ET) D of an
+                       assembled fluid-solid body (operator) and p applied to
+                       its dense eigenvalues (filter)
   shift_invert_eigs    closed-form eigenvalues of the 1D P1 finite-element
                        pencil (stiffness, consistent mass) at the low end of a
                        wide spectrum, where a dense QR-based solver is limited
@@ -240,6 +243,52 @@ class Pencil:
         lib().or_residuals(ctypes.byref(self.s), ctypes.c_int(lam.size), _p(lam, _f64p),
                            _p(X, _f64p), _p(r1, _f64p), _p(r2, _f64p))
         return r1, r2
+
+
+class FluidPencil:
+    """The fluid-solid pencil as setupmatvec prepares it for sparsefsAV
+    (src/mod_matvec.f90:98-222, 498-520): D from diag(B), B~ = D B D with its
+    Chebyshev solve, Dp from diag(-Ap), Ap~ = Dp (-Ap) Dp with its Chebyshev
+    solve, and the operator
+        A~ x = D [Ad (D x) + E (Dp C_p (Dp ET (D x)))],  C_p ~ Ap~^-1,
+    i.e. D (Ad - E Ap^-1 ET) D.  Built from the C primitives (CSR product and
+    chebiter); the filter recurrence is or_chebav's.  Test infrastructure and
+    the bench's CPU baseline only."""
+
+    def __init__(self, Q, *, bdeg, bounds_B, apdeg, bounds_Ap):
+        self.n = Q.n
+        self.d, bval = diagscale(Q.B, 1.0)
+        self.Bt = CSRView(Q.B.rowptr, Q.B.col, bval)
+        self.dp, apval = diagscale(Q.Ap, -1.0)
+        self.Apt = CSRView(Q.Ap.rowptr, Q.Ap.col, apval)
+        self.Ad, self.E, self.ET = (CSRView.of(M) for M in (Q.Ad, Q.E, Q.ET))
+        self.bdeg, (self.blmin, self.blmax) = bdeg, bounds_B
+        self.apdeg, (self.aplmin, self.aplmax) = apdeg, bounds_Ap
+
+    def apply_a(self, x):
+        v0 = self.d * x
+        w0 = spmv(self.Ad, v0)
+        x1 = self.dp * spmv(self.ET, v0)
+        y1 = self.dp * chebiter(self.Apt, self.aplmin, self.aplmax, self.apdeg, x1)
+        return self.d * (w0 + spmv(self.E, y1))
+
+    def apply_b(self, x):
+        return spmv(self.Bt, x)
+
+    def bsolve(self, b):
+        return chebiter(self.Bt, self.blmin, self.blmax, self.bdeg, b)
+
+    def chebav(self, pol: Pol, z):
+        """y = sum_k mu_k v_k, v_{k+1} = t (A~ B~^-1 v_k - cc v_k) - v_{k-1}."""
+        vk = np.array(z, dtype=np.float64)
+        vkm = np.zeros_like(vk)
+        y = pol.mu[0] * vk
+        for k in range(1, pol.deg + 1):
+            t = (1.0 if k == 1 else 2.0) / pol.dd
+            v = t * (self.apply_a(self.bsolve(vk)) - pol.cc * vk) - (0.0 if k == 1 else vkm)
+            y += pol.mu[k] * v
+            vkm, vk = vk, v
+        return y
 
 
 def findpol(xintv, thr_int=0.8, thr_ext=0.7, *, min_deg=2, max_deg=10000, damping=2) -> Pol:

@@ -302,6 +302,28 @@ def test_assembled_split_chains_and_auto_ap_degree(assembled, block, split):
 
 
 @pytest.mark.gpu
+def test_assembled_refined_solves(assembled):
+    """The fluid-solid operator with both Chebyshev solves (B~ and Ap~) in
+    mixed-precision refinement: mid-band eigenpairs as the FP64 operator's."""
+    from normalmodes_amd.driver import apply_pevsl, refine_bsolves
+    name, Q, mv, S, B, w = assembled
+    if name != "core-p2-job2":
+        pytest.skip("one body suffices")
+    a, b = _band(w, w[-1] / 1000.0)
+    ref = orc.shift_invert_eigs(S, B, a, b)
+    try:
+        degs = refine_bsolves(mv, passes=2)
+        r = apply_pevsl(mv, interval=(a, b), block=4, split=2, verbose=20)
+    finally:
+        refine_bsolves(mv, passes=0)
+    assert r.lam.size == ref.size
+    err = np.max(np.abs(r.lam - ref) / ref)
+    print(f"{name} refined solves {degs}: max rel err {err:.2e}, max residual {r.res2.max():.2e}", flush=True)
+    assert err < 1e-12
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+
+
+@pytest.mark.gpu
 def test_assembled_seismic_band(assembled):
     """The ~20 lowest seismic modes above the fluid's zero-energy cluster.
     There lam_max / lam is ~2e4 for these Earth-sized bodies, and every FP64

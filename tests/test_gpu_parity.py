@@ -415,6 +415,65 @@ def test_fp32_bsolve_eigenvalues_match_fp64(small_p2, dense_eigs):
     assert np.all(r32.res2 < 1e-4)
 
 
+def test_refined_bsolve_accuracy(small_p2):
+    """Mixed-precision refinement of the B~ solve: one FP32 pass stops near
+    the FP32 floor, two passes reach ~1e-12, three the FP64 solve's level;
+    measured against the exact solve of the same scaled mass (sparse LU),
+    single column and 8 columns."""
+    import scipy.sparse.linalg as spl
+    pv = _pv()
+    from normalmodes_amd.driver import setupmatvec, refine_inner_degree
+    mv = setupmatvec(small_p2, bdeg="auto")
+    h = pv.pevsl_start(0)
+    try:
+        pv.pevsl_setprobsizes(h, mv.n, mv.n, -1)
+        pv.pevsl_setbmv(h, mv.matB)
+        pv.pevsl_setbsol_chebiter(h, 2, mv.chebB)
+        pv.pevsl_setamv_dad(h, mv.matA, mv.d)
+        pv.pevsl_set_geneig(h)
+        import scipy.sparse as sp
+        Bt = (sp.diags(mv.d) @ small_p2.B.to_scipy() @ sp.diags(mv.d)).tocsc()
+        lu = spl.splu(Bt)
+        X = np.random.default_rng(11).standard_normal((8, mv.n))
+        ref = np.stack([lu.solve(x) for x in X])
+        deg = refine_inner_degree(mv.blmin, mv.blmax)
+        errs = {}
+        for passes in (0, 1, 2, 3):
+            pv.chebiter_set_refine(mv.chebB, passes, deg)
+            y1 = pv.apply(h, 2, X[0])
+            Y = pv.apply_block(h, 2, X)
+            errs[passes] = (_rel(y1, ref[0]), _rel(Y, ref))
+        pv.chebiter_set_refine(mv.chebB, 0)
+    finally:
+        pv.pevsl_finish(h)
+        mv.free()
+    print("refined B-solve errors (1 column, 8 columns) by passes:", errs, flush=True)
+    for k in (0, 1):
+        assert errs[0][k] < 1e-13
+        assert 1e-9 < errs[1][k] < 1e-5          # FP32 floor after one pass
+        assert errs[2][k] < 1e-11
+        assert errs[3][k] < 3e-13
+
+
+@pytest.mark.parametrize("block", [1, 8])
+def test_refined_bsolve_solve_meets_gate(small_p2, dense_eigs, block):
+    """The whole solve with the refined B-solve: the same eigenvalues as the
+    dense solver to 1e-12 and the 1e-10 residual gate, recomputed."""
+    from normalmodes_amd.driver import setupmatvec, apply_pevsl, refine_bsolves
+    mv = setupmatvec(small_p2, bdeg="auto")
+    try:
+        refine_bsolves(mv, passes=2)
+        r = apply_pevsl(mv, interval=(1.0, 12.0), block=block)
+    finally:
+        mv.free()
+    w = dense_eigs(small_p2)
+    ref = w[(w >= 1.0) & (w <= 12.0)]
+    assert r.lam.size == ref.size
+    assert np.max(np.abs(r.lam - ref) / ref) < 1e-12
+    assert np.all(r.res2 <= 1e-10) and r.target_met
+    _check_gate(small_p2, r)
+
+
 @pytest.mark.parametrize("block", [1, 8])
 def test_fp32_stiffness_filter_error_stated(small_p2, dense_eigs, block):
     """FP32 values of A~ in the filter products as well (NMG_FP32_FILTER_A):

@@ -327,3 +327,46 @@ def test_shift_invert_eigs_closed_form():
         wd = orc.shift_invert_eigs(K.toarray(), M.toarray(), a, b)
         assert np.max(np.abs(wd - ref) / ref) < tol
         assert np.max(np.abs(dense[i0:i1 + 1] - ref) / ref) > 3 * err
+
+
+def test_fluid_pencil_operator_and_filter():
+    """The oracle's fluid-solid operator against the dense Schur complement
+    of an assembled fluid-core body, and its filter against p applied to the
+    dense eigenvalues of the scaled pencil (the Chebyshev solves at degrees
+    whose error bound is below 1e-13)."""
+    from normalmodes_amd import models
+    from normalmodes_amd.assemble import assemble
+    from normalmodes_amd.mesh import ball_mesh
+    m = models.fluid_core(0.5, radius=6371.0)
+    Q = assemble(ball_mesh(600, porder=2, interfaces=m.interfaces, scale=m.radius), m, job=2)
+    B = Q.B.to_scipy().toarray()
+    Ap = Q.Ap.to_scipy().toarray()
+    d = 1.0 / np.sqrt(np.diag(B))
+    dp = 1.0 / np.sqrt(-np.diag(Ap))
+    Bt = B * d[:, None] * d[None, :]
+    Apt = -Ap * dp[:, None] * dp[None, :]
+    wb, wa = np.linalg.eigvalsh(Bt), np.linalg.eigvalsh(Apt)
+
+    def degree(lo, hi):
+        k = hi / lo
+        rho = (np.sqrt(k) - 1) / (np.sqrt(k) + 1)
+        return int(np.ceil(np.log(5e-14) / np.log(rho)))
+    F = orc.FluidPencil(Q, bdeg=degree(wb[0], wb[-1]), bounds_B=(wb[0], wb[-1]),
+                        apdeg=degree(wa[0], wa[-1]), bounds_Ap=(wa[0], wa[-1]))
+    S = Q.Ad.to_scipy().toarray() - Q.E.to_scipy().toarray() @ np.linalg.solve(Ap, Q.ET.to_scipy().toarray())
+    St = S * d[:, None] * d[None, :]
+    x = np.random.default_rng(2).standard_normal(Q.n)
+    ref = St @ x
+    assert np.linalg.norm(F.apply_a(x) - ref) <= 1e-11 * np.linalg.norm(ref)
+    assert np.linalg.norm(Bt @ F.bsolve(x) - x) <= 1e-11 * np.linalg.norm(x)
+    # filter: p(A~ B~^-1) acts on B~-eigenvectors' images as p(lambda)
+    w, V = sl.eigh(St, Bt)
+    lmin, lmax = w[0] - 1e-9 * abs(w[-1]), w[-1] * (1 + 1e-9)
+    a, b = w[Q.n // 3], w[Q.n // 3 + 6]
+    pol = orc.findpol([a, b, lmin, lmax], 0.8, 0.7)
+    z = Bt @ V[:, :40] @ np.ones(40)                       # sum of B~ v_j (images of eigenvectors)
+    y = F.chebav(pol, z)
+    t = (w[:40] - pol.cc) / pol.dd
+    pv_ = np.polynomial.chebyshev.chebval(t, pol.mu)
+    ref_y = Bt @ V[:, :40] @ pv_
+    assert np.linalg.norm(y - ref_y) <= 1e-8 * np.linalg.norm(z)

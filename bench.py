@@ -10,10 +10,18 @@ FindPol, ChebLanNr with Rayleigh-Ritz extraction and the residual check
 Mesh generation, assembly and the host->HBM upload are outside the timed
 region and reported separately.
 
-Workload (default ``c1``): homogeneous solid unit ball, seeded ~12k-tet P2
-mesh, ~52k DoF, the interval holding the ~30 lowest non-rigid modes.
+Workload (default ``c2``): the three-layer ball with a fluid outer core,
+radius 6371 km, JOB=2 with its reference gravity, seeded ~466k-tet P2 mesh,
+~2M displacement DoF plus ~1e5 pressures; all eigenpairs of the seismic band
+``--band-mhz``.  The timed step is B~ bounds and Chebyshev set-up, LanBounds,
+FindPol, block ChebLanNr (4 columns as two 2-column filter chains on two
+streams) with extraction, and the residual check; beside it one FP32-filter
+solve, the filter's cost per column and degree by precision, every product
+rate and the oracle's CPU baseline.  ``--workload c1``: the homogeneous
+solid unit ball, seeded ~12k-tet P2 mesh, ~52k DoF, the interval holding the
+~30 lowest non-rigid modes (the earlier headline, kept for comparison).
 
-Also reported on the same JSON line:
+Also reported on the c1 line:
   roofline      the dominant kernel (fused Chebyshev B-solve step), algorithmic
                 bytes per launch over its mean duration from HIP events sampled
                 on the solver's stream during the timed solves;
@@ -133,8 +141,12 @@ def kernel_only(args):
     from normalmodes_amd.workloads import build_problem
     P = build_problem(args.workload)
     kpart = args.block // args.split if args.block > 1 and args.split > 1 else args.block
-    mv = upload_with_layouts(P, args.layouts, sorted({1, kpart}))
-    driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
+    if isinstance(P, driver.FluidProblem):
+        mv = driver.setupmatvec_fs(P, bdeg="auto" if args.bdeg == "auto" else args.bdeg, apdeg=args.apdeg,
+                                   tune=sorted({1, kpart}))
+    else:
+        mv = upload_with_layouts(P, args.layouts, sorted({1, kpart}))
+        driver.setup_bsolve(mv, bdeg="auto" if args.bdeg == "auto" else args.bdeg)
     n_copy = 1 << 24   # 16M doubles: 128 MiB read + 128 MiB written per launch
     t_copy = pv.bench_copy(n_copy, 5)
     info = pv.chebiter_info(mv.chebB)
@@ -152,12 +164,14 @@ def kernel_only(args):
     mv.free()
 
 
-def solve_once(mv, interval, bdeg, precision, probe_every, block, split=0):
+def solve_once(mv, interval, bdeg, precision, probe_every, block, split=0, refine=0, verbose=0):
     from normalmodes_amd import driver, pevsl as pv
     t0 = time.perf_counter()
     driver.setup_bsolve(mv, bdeg=bdeg)
+    driver.refine_bsolves(mv, passes=refine)
     r = driver.apply_pevsl(mv, interval=interval, precision=precision, vectors=False,
-                           probe_every=probe_every, block=block, split=split if block > 1 else 0)
+                           probe_every=probe_every, block=block, split=split if block > 1 else 0,
+                           verbose=verbose)
     return time.perf_counter() - t0, r
 
 
@@ -207,71 +221,268 @@ def spmv_rates(name, reps=50):
     return out
 
 
+WORKLOAD_TEXT = {
+    "c1": ("c1: homogeneous solid unit ball (rho=1, vp=1.73, vs=1), seeded jittered-lattice Delaunay mesh of "
+           "{tets} P2 tets, {n} DoF, all eigenpairs in [{a:.6g}, {b:.6g}]"),
+    "tiny": "tiny: homogeneous solid unit ball, {tets} P2 tets, {n} DoF, all eigenpairs in [{a:.6g}, {b:.6g}]",
+    "c2s": ("c2s: the 2M-DoF three-layer unit ball (rho 13->4, vp 11->8, vs 3.5->6.5 piecewise linear in r, "
+            "interfaces at r = 0.19 and 0.55 as tetrahedron faces) with the outer core solid, seeded "
+            "isosurface-stuffed BCC mesh of {tets} P2 tets, {n} DoF, all eigenpairs in [{a:.6g}, {b:.6g}] "
+            "(231 pairs, the 5-fold 218.4 multiplet up to 2830.2)"),
+}
+
+
 # the 2M-DoF three-layer ball with a fluid outer core (JOB=2, Earth-sized,
-# radius 6371 km) and the band of its ~200 lowest seismic modes in mHz
-C2_BAND_MHZ = (0.25, 1.3)
+# radius 6371 km): the band of its ~200 seismic modes in mHz, placed clear of
+# the fluid's cluster of ~9e4 near-zero (undertone) eigenvalues below 0.13 mHz
+C2_BAND_MHZ = (0.8, 1.3)
 
 
-def c2_components(reps=20):
-    """Measured pieces of the 2M-DoF fluid-solid solve: assembly, set-up,
-    LanBounds, FindPol's degree for the ~200-mode band, one timed filter
-    application (single vector, FP64) and the rates of every product and
-    Chebyshev step of the operator.  A whole solve of this body takes
-    Lanczos steps x filter degree x the per-degree time; it is reported as
-    such, not as a measured headline."""
+def product_rates(mv, reps=20):
+    """Rates of every product and Chebyshev step of a fluid-solid set-up."""
+    from normalmodes_amd import pevsl as pv
+    rates = {}
+    for key, mat in (("Ad", mv.fs.matAd), ("B", mv.matB), ("Ap", mv.fs.matAp), ("E", mv.fs.matE),
+                     ("ET", mv.fs.matET)):
+        for fp32 in (False, True):
+            sec, b = pv.bench_spmv(mat, reps, fp32)
+            rates[key + ("_fp32" if fp32 else "")] = dict(layout=pv.csr_info(mat)["layout"],
+                                                          us=round(sec * 1e6, 2), gbs=round(b / sec / 1e9, 1),
+                                                          frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
+    for key, cheb in (("B_chebstep", mv.chebB), ("Ap_chebstep", mv.fs.chebAp)):
+        for fp32 in (False, True):
+            sec, b = pv.bench_chebstep(cheb, reps, fp32)
+            rates[key + ("_fp32" if fp32 else "")] = dict(us=round(sec * 1e6, 2), gbs=round(b / sec / 1e9, 1),
+                                                          frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
+    return rates
+
+
+def filter_rates(mv, interval, block, split, lanbounds_tol, reps=1):
+    """Seconds of one filter application per column and degree, by precision
+    mode, at the solve's block width and stream split and at one column."""
+    from normalmodes_amd import driver, pevsl as pv
+    out = {}
+    h = driver._open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
+    try:
+        lmin, lmax = pv.pevsl_lanbounds(h, 3000, 5000, lanbounds_tol)
+        pol = pv.pevsl_findpol(np.array([interval[0], interval[1], lmin, lmax]), 0.8, 0.7)
+        deg = pv.pol_info(pol).deg
+        try:
+            for label, prec, passes in (("fp64", pv.FP64, 0), ("fp64_refined", pv.FP64, 2),
+                                        ("fp32_values", pv.FP32_FILTER, 0), ("fp32_filter_A", pv.FP32_FILTER_A, 0),
+                                        ("fp32_iterates", pv.FP32_BSOLVE, 0)):
+                pv.pevsl_set_precision(h, prec)
+                driver.refine_bsolves(mv, passes=passes)
+                row = {}
+                for K, sp in ((1, 0), (block, split)):
+                    pv.set_split_streams(h, sp)
+                    sec = pv.bench_filter(h, pol, K, reps=reps)
+                    row[f"K{K}"] = round(sec / K / deg * 1e3, 4)
+                out[label] = row
+        finally:
+            driver.refine_bsolves(mv, passes=0)
+            pv.pevsl_freepol(pol)
+    finally:
+        pv.pevsl_finish(h)
+    out["unit"] = "ms per column and filter degree"
+    out["degree"] = deg
+    return out
+
+
+def run_fluid(args):
+    """The 2M-DoF fluid-solid headline: setupmatvec_fs, then timed solves of
+    the band (B~ bounds and Chebyshev set-up, LanBounds, FindPol, ChebLanNr
+    with extraction, residual check), FP64 for ``value`` and one FP32-filter
+    solve beside it."""
+    from normalmodes_amd import driver, pevsl as pv
+    from normalmodes_amd.workloads import build_problem
+    t0 = time.perf_counter()
+    Q = build_problem(args.workload)
+    t_build = time.perf_counter() - t0
+    band = tuple(float(v) for v in args.band_mhz.split(","))
+    interval = (driver.freq_to_lambda(band[0]), driver.freq_to_lambda(band[1]))
+    block = args.block or 4
+    split = 2 if args.split is None else args.split
+    kpart = block // split if split > 1 and block > 1 else block
+    log(f"[bench] built {args.workload}: n={Q.n} pressures={Q.n_p} tets={Q.mesh.n_elem} in {t_build:.1f}s")
+    t0 = time.perf_counter()
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg=args.apdeg, tune=sorted({1, kpart, block}))
+    t_setup = time.perf_counter() - t0
+    lb = (3000, 5000, args.lanbounds_tol)
+
+    def solve(precision, probe_every, refine):
+        t1 = time.perf_counter()
+        driver.setup_bsolve(mv, bdeg="auto")
+        driver.refine_bsolves(mv, passes=refine)
+        r = driver.apply_pevsl(mv, interval=interval, precision=precision, vectors=False, block=block,
+                               split=split, probe_every=probe_every, lanbounds=lb, verbose=max(1, 24 // block))
+        return time.perf_counter() - t1, r
+
+    for i in range(args.warmup):
+        el, r = solve(pv.FP64, 0, args.refine)
+        log(f"[bench] warmup {i}: {el:.1f}s nev={r.lam.size}")
+    times, r = [], None
+    for i in range(args.steps):
+        el, r = solve(pv.FP64, args.probe_every, args.refine)
+        times.append(el)
+        log(f"[bench] step {i}: {el:.1f}s nev={r.lam.size} deg={r.deg} columns={r.lan_steps} "
+            f"max res2={np.max(r.res2) if r.res2.size else float('nan'):.2e}")
+    T = max(times)
+    gate_ok = bool(r.res2.size > 0 and np.all(r.res2 <= GATE))
+    chk = r.probe.get("chebstep", {"seconds": 0.0, "bytes": 0.0, "samples": 0})
+    achieved = chk["bytes"] / chk["seconds"] / 1e9 if chk["seconds"] > 0 else 0.0
+    kname = f"block{kpart}_chebstep" if kpart > 1 else "mass3_chebstep"
+    traffic, traffic_src = traffic_from_profiles(f"{kname}:{args.workload}")
+    line = {
+        "metric": METRIC, "value": round(T, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(T * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": (f"{args.workload}: three-layer ball of radius 6371 km (solid inner core r<0.19, fluid "
+                         f"outer core 0.19-0.55 with vs=0 and pressure DoFs, solid mantle; rho 13->4, vp 11->8, "
+                         f"vs 3.5->6.5 piecewise linear in r), JOB=2 with its own reference gravity, seeded "
+                         f"isosurface-stuffed BCC mesh of {Q.mesh.n_elem} P2 tets, {Q.n} displacement DoF + "
+                         f"{Q.n_p} pressures, all eigenpairs in [{band[0]}, {band[1]}] mHz"),
+            "n_dof": Q.n, "n_pressure": Q.n_p, "n_tets": Q.mesh.n_elem,
+            "nnz": {k: getattr(Q, k).nnz for k in ("Ad", "B", "E", "ET", "Ap")},
+            "band_mhz": list(band), "interval": list(interval), "nev": int(r.lam.size),
+            "filter_degree": r.deg, "bsolve_degree": mv.bdeg, "ap_degree": mv.fs.apdeg,
+            "lanczos_columns": r.lan_steps, "lanbounds": [r.lmin, r.lmax], "lanbounds_tol": args.lanbounds_tol,
+            "block": block, "filter_chains": split if split > 1 else 1, "refine_passes": args.refine,
+            "kernel_layouts": mv.tuned, "parallelism": "1 GPU", "global_batch": 1, "seq_len": Q.n,
+        },
+        "gate": {"resid_max": float(np.max(r.res2)) if r.res2.size else None,
+                 "resid_ref_max": float(np.max(r.res_ref)) if r.res_ref.size else None,
+                 "passed": gate_ok, "threshold": GATE, "target_met": bool(r.target_met)},
+        "phases_s": {k: round(v, 3) for k, v in r.timings.items()},
+        "setup_s": {"build": round(t_build, 2), "setupmatvec_fs": round(t_setup, 2)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": (f"fused {kpart}-column Chebyshev B-solve step (scalar mass x I3 product + update on "
+                       f"{kpart} interleaved columns) of one filter chain; {split if split > 1 else 1} chains "
+                       f"run at once on as many streams"),
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+            "bytes_per_launch": chk["bytes"], "mean_launch_us": round(chk["seconds"] * 1e6, 3),
+            "samples": chk["samples"],
+            "concurrent_parts": split if split > 1 else 1,
+            "aggregate_achieved": round(achieved * (split if split > 1 else 1), 1),
+        },
+    }
+    if args.fp32_compare:
+        try:
+            mode = {"values": pv.FP32_FILTER, "filter_A": pv.FP32_FILTER_A, "iterates": pv.FP32_BSOLVE}[args.fp32_mode]
+            el32, r32 = solve(mode, 0, 0)
+            m = min(r32.lam.size, r.lam.size)
+            line["fp32_filter"] = {
+                "seconds": round(el32, 3), "mode": args.fp32_mode, "nev": int(r32.lam.size),
+                "speedup_vs_fp64": round(T / el32, 3),
+                "max_rel_eig_diff_vs_fp64": float(np.max(np.abs(r32.lam[:m] - r.lam[:m]) / np.abs(r.lam[:m])))
+                if m and r32.lam.size == r.lam.size else None,
+                "resid_max": float(np.max(r32.res2)) if r32.res2.size else None,
+                "lanczos_columns": r32.lan_steps}
+            log(f"[bench] fp32 filter: {line['fp32_filter']}")
+        except Exception as e:
+            line["fp32_filter"] = {"error": str(e)}
+    try:
+        line["filter_cost"] = filter_rates(mv, interval, block, split, args.lanbounds_tol)
+        log(f"[bench] filter cost: {line['filter_cost']}")
+    except Exception as e:
+        line["filter_cost"] = {"error": str(e)}
+    try:
+        line["spmv_hbm"] = product_rates(mv)
+        log(f"[bench] product rates: {line['spmv_hbm']}")
+    except Exception as e:
+        line["spmv_hbm"] = {"error": str(e)}
+    try:
+        line["reorth_gemm"] = reorth_rate(Q.n, min(r.lan_steps, 512), block)
+    except Exception as e:
+        line["reorth_gemm"] = {"error": str(e)}
+    if args.cpu_seconds > 0:
+        try:
+            line["cpu_baseline"] = cpu_baseline_fluid(Q, mv, r, args.cpu_seconds)
+            log(f"[bench] cpu baseline: {line['cpu_baseline']}")
+        except Exception as e:
+            line["cpu_baseline"] = {"error": str(e)}
+    mv.free()
+    print(json.dumps(line), flush=True)
+
+
+def fluid_pieces(args):
+    """Measured pieces of the fluid-solid ball (c2): assembly and set-up,
+    LanBounds, the FindPol filter the reference's rule picks for the band,
+    the filter's cost per column and degree by precision, and every product
+    rate.  Its whole solve is not run: FindPol's end-of-spectrum rule makes
+    the filter of any band of the lowest seismic modes a low-pass one over
+    [lmin, b], which passes the fluid's ~1e5 near-zero eigenvalues."""
     from normalmodes_amd import driver, pevsl as pv
     from normalmodes_amd.workloads import build_problem
     t0 = time.perf_counter()
     Q = build_problem("c2")
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    mv = driver.setupmatvec_fs(Q, bdeg="auto", tune=(1,))
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg="auto", tune=(1, 2))
     t_setup = time.perf_counter() - t0
-    out = dict(workload="c2: three-layer ball (radius 6371 km; solid inner core r<0.19, fluid outer core "
-                        "0.19-0.55, solid mantle), JOB=2 reference gravity, P2, seeded ~450k-tet mesh",
+    band = C2_BAND_MHZ
+    interval = (driver.freq_to_lambda(band[0]), driver.freq_to_lambda(band[1]))
+    out = dict(workload=("c2: the three-layer ball of radius 6371 km with its fluid outer core (vs = 0, pressure "
+                         "DoFs), JOB=2 with its reference gravity, stuffed mesh"),
                n_dof=Q.n, n_pressure=Q.n_p, n_tets=Q.mesh.n_elem,
                nnz={k: getattr(Q, k).nnz for k in ("Ad", "B", "E", "ET", "Ap")},
-               build_s=round(t_build, 2), setup_s=round(t_setup, 2), bsolve_degree=mv.bdeg,
-               ap_degree=mv.fs.apdeg, kernel_layouts=mv.tuned)
+               build_s=round(t_build, 2), setup_s=round(t_setup, 2), bsolve_degree=mv.bdeg, ap_degree=mv.fs.apdeg,
+               band_mhz=list(band))
     try:
-        rates = {}
-        for key, mat in (("Ad", mv.fs.matAd), ("B", mv.matB), ("Ap", mv.fs.matAp), ("E", mv.fs.matE),
-                         ("ET", mv.fs.matET)):
-            sec, b = pv.bench_spmv(mat, reps, False)
-            rates[key] = dict(layout=pv.csr_info(mat)["layout"], us=round(sec * 1e6, 2),
-                              gbs=round(b / sec / 1e9, 1), frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
-        for key, cheb in (("B_chebstep", mv.chebB), ("Ap_chebstep", mv.fs.chebAp)):
-            sec, b = pv.bench_chebstep(cheb, reps, False)
-            rates[key] = dict(us=round(sec * 1e6, 2), gbs=round(b / sec / 1e9, 1),
-                              frac=round(b / sec / 1e9 / HBM_PEAK_GBS, 3))
-        out["rates"] = rates
         h = driver._open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
         try:
             t0 = time.perf_counter()
             lmin, lmax = pv.pevsl_lanbounds(h, 3000, 5000, 1e-5)
             out["lanbounds"] = dict(seconds=round(time.perf_counter() - t0, 2), lmin=lmin, lmax=lmax,
                                     steps=pv.solve_stats(h)["lan_steps"])
-            a, b_ = driver.freq_to_lambda(C2_BAND_MHZ[0]), driver.freq_to_lambda(C2_BAND_MHZ[1])
-            pol = pv.pevsl_findpol(np.array([a, b_, lmin, lmax]), 0.8, 0.7)
-            try:
-                deg = pv.pol_info(pol).deg
-                z = np.random.default_rng(7).standard_normal(mv.n)
-                pv.filter_apply(h, pol, z)
-                t0 = time.perf_counter()
-                pv.filter_apply(h, pol, z)
-                tf = time.perf_counter() - t0
-            finally:
-                pv.pevsl_freepol(pol)
-            out["filter"] = dict(band_mhz=list(C2_BAND_MHZ), interval=[a, b_], degree=deg,
-                                 seconds_per_application=round(tf, 3),
-                                 ms_per_degree=round(tf / deg * 1e3, 3),
-                                 note="one Lanczos step of the single-vector solve applies the filter once; "
-                                      "host copies of one vector included")
+            pol = pv.pevsl_findpol(np.array([interval[0], interval[1], lmin, lmax]), 0.8, 0.7)
+            info = pv.pol_info(pol)
+            ftype = 1 if info.gam <= -1.0 + 1e-12 else (2 if info.gam >= 1.0 - 1e-12 else 0)
+            out["findpol"] = dict(degree=info.deg, type=ftype, gamma=info.gam,
+                                  note="type 1: the interval touches the lower end of the spectrum (within "
+                                       "5e-4 of its width), so the filter is low-pass over [lmin, b]")
+            pv.pevsl_freepol(pol)
         finally:
             pv.pevsl_finish(h)
+        out["filter_cost"] = filter_rates(mv, interval, 4, 2, 1e-5)
+        out["rates"] = product_rates(mv, reps=10)
     finally:
         mv.free()
     return out
+
+
+def cpu_baseline_fluid(Q, mv, r, seconds_budget):
+    """The oracle's fluid-solid pencil (CPU restatement of sparsefsAV and the
+    filter, C + OpenMP primitives) timed on a bounded sample: degrees of the
+    same filter on the same operator; scaled by the filter degrees of the
+    GPU solve, counted per column (a single-vector CPU solve applies the
+    filter once per Lanczos vector)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    F = orc.FluidPencil(Q, bdeg=mv.bdeg, bounds_B=(mv.blmin, mv.blmax), apdeg=mv.fs.apdeg,
+                        bounds_Ap=(mv.fs.aplmin, mv.fs.aplmax))
+    pol = orc.findpol(r.xintv, 0.8, 0.7)
+    z = orc.start_vector(Q.n, 7)
+    small = orc.Pol(**{**pol.__dict__, "deg": 1, "mu": pol.mu[:2].copy()})
+    t0 = orc.wtime()
+    F.chebav(small, z)
+    per_deg = orc.wtime() - t0
+    m = int(max(1, min(pol.deg, seconds_budget / max(per_deg, 1e-6))))
+    samp = orc.Pol(**{**pol.__dict__, "deg": m, "mu": pol.mu[:m + 1].copy()})
+    t0 = orc.wtime()
+    F.chebav(samp, z)
+    t_s = orc.wtime() - t0
+    per_deg = t_s / m
+    degrees = r.deg * r.filter_applies + r.lan_steps_bounds
+    return dict(value=round(per_deg * degrees, 1), unit="s", cores=orc.num_threads(), kind="port",
+                cpu_model=cpu_model(), host_threads=os.cpu_count(),
+                sample=(f"oracle fluid-solid filter of degree {m} (of {pol.deg}; B-solve degree {mv.bdeg}, "
+                        f"Ap solve degree {mv.fs.apdeg}) on the same {Q.n}-DoF operator: {t_s:.1f} s; scaled by "
+                        f"the {degrees} filter degrees of the GPU solve ({r.filter_applies} filtered columns x "
+                        f"{r.deg} + {r.lan_steps_bounds} LanBounds steps)"))
 
 
 def cpu_baseline(P, mv, r, seconds_budget):
@@ -348,31 +559,48 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed full solves")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed full solves before timing")
-    ap.add_argument("--workload", default="c1", choices=["c1", "tiny"])
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed full solves before timing (default 0 for c2, 1 otherwise)")
+    ap.add_argument("--workload", default="c2s", choices=["c2s", "c2", "c2r", "c1", "tiny"],
+                    help="c2s: the 2M-DoF three-layer ball (headline whole solve); c2: the same ball with its "
+                         "fluid outer core (measured pieces; its whole solve is beyond reach, see DESIGN); "
+                         "c1: the 52k-DoF solid ball")
+    ap.add_argument("--band-mhz", default=",".join(str(v) for v in C2_BAND_MHZ),
+                    help="band of the fluid-solid workloads, 'lo,hi' in mHz")
+    ap.add_argument("--apdeg", default="auto", help="Ap~ Chebyshev degree of the fluid-solid set-up: auto | reference")
+    ap.add_argument("--refine", type=int, default=None,
+                    help="mixed-precision refinement passes of the Chebyshev solves in the timed solve (0: plain "
+                         "FP64; default 2 for c2s, 0 otherwise)")
+    ap.add_argument("--fp64-compare", type=int, default=0,
+                    help="also time one plain FP64 solve beside a refined headline (several minutes at c2s)")
+    ap.add_argument("--fluid-pieces", type=int, default=None,
+                    help="with c2s, also measure the fluid-solid ball's set-up, bounds, filter and product costs "
+                         "(default 1 for c2s)")
+    ap.add_argument("--lanbounds-tol", type=float, default=1e-5, help="LanBounds tolerance (the reference's 1e-5)")
+    ap.add_argument("--fp32-mode", default="filter_A", choices=["values", "filter_A", "iterates"],
+                    help="precision of the FP32-filter solve: B~ values; + stiffness values; B~ values + iterates")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--bdeg", default="auto", help="B-solve degree: auto | reference | <int>")
     ap.add_argument("--probe-every", type=int, default=64)
-    ap.add_argument("--block", type=int, default=8,
+    ap.add_argument("--block", type=int, default=0,
                     help="Lanczos block width (1 = the reference's single-vector ChebLanNr; the block variant "
                          "reorthogonalises with FP64 GEMMs and streams the matrices once per K columns)")
-    ap.add_argument("--split", type=int, default=2,
+    ap.add_argument("--split", type=int, default=None,
                     help="run each k-column B-solve as this many concurrent parts on as many streams (2 or 4; "
                          "0 = one stream)")
-    ap.add_argument("--single-compare", type=int, default=1,
+    ap.add_argument("--single-compare", type=int, default=None,
                     help="with --block > 1, also time one single-vector (block 1) solve (0 skips)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget; 0 skips")
     ap.add_argument("--cpu-full", type=int, default=0,
                     help="also time one complete single-vector solve of the CPU restatement (several minutes)")
-    ap.add_argument("--spmv-workload", default="c2s", help="size for the SpMV HBM rates; none skips")
-    ap.add_argument("--fp32-compare", type=int, default=1, help="also run one FP32-filter solve (0 skips)")
+    ap.add_argument("--spmv-workload", default=None,
+                    help="size for the SpMV HBM rates; none skips (default: the headline's own matrices for c2s, "
+                         "c2s for c1)")
+    ap.add_argument("--fp32-compare", type=int, default=None, help="also run one FP32-filter solve (0 skips)")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="copy kernel + the --spmv-workload product rates only, --steps launches each (counter runs)")
-    ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint of the solved pairs (0 skips)")
-    ap.add_argument("--c2", type=int, default=1,
-                    help="measure the pieces of the 2M-DoF fluid-solid (c2) solve: set-up, bounds, filter degree, "
-                         "one filter application and the product rates (0 skips)")
+    ap.add_argument("--adjoint", type=int, default=None, help="time the adjoint of the solved pairs (0 skips)")
     ap.add_argument("--layouts", default="auto",
                     help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
                          "'A:<g>,B:<g>' (g in 8,16,32,64), e.g. for profiler runs, where timing-based tuning "
@@ -384,6 +612,23 @@ def main():
             "(the path would shard by spectrum slices, one group per GPU); run with --gpus 1")
         sys.exit(2)
     bdeg = args.bdeg if args.bdeg in ("auto", "reference") else int(args.bdeg)
+    if args.warmup is None:
+        args.warmup = 0 if args.workload in ("c2", "c2r", "c2s") else 1
+    fluid = args.workload in ("c2", "c2r")
+    if fluid and not (args.kernel_only or args.spmv_only):
+        run_fluid(args)
+        return
+    big = args.workload == "c2s"
+    args.block = args.block or (4 if fluid or big else 8)
+    args.split = 2 if args.split is None else args.split
+    args.refine = (2 if big else 0) if args.refine is None else args.refine
+    for name, small_default in (("single_compare", 1), ("fp32_compare", 1), ("adjoint", 1)):
+        if getattr(args, name) is None:
+            setattr(args, name, 0 if big else small_default)
+    if args.fluid_pieces is None:
+        args.fluid_pieces = 1 if big else 0
+    if args.spmv_workload is None:
+        args.spmv_workload = "none" if big else "c2s"
     if args.kernel_only:
         args.bdeg = bdeg
         kernel_only(args)
@@ -411,13 +656,15 @@ def main():
     t_upload = time.perf_counter() - t0
     precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
 
+    verbose = max(1, 24 // args.block) if P.n > 500000 else 0   # progress lines of long solves
     for i in range(args.warmup):
-        el, r = solve_once(mv, interval, bdeg, precision, 0, args.block, args.split)
+        el, r = solve_once(mv, interval, bdeg, precision, 0, args.block, args.split, args.refine, verbose)
         log(f"[bench] warmup {i}: {el:.2f}s nev={r.lam.size} steps={r.lan_steps}")
     times = []
     r = None
     for i in range(args.steps):
-        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every, args.block, args.split)
+        el, r = solve_once(mv, interval, bdeg, precision, args.probe_every, args.block, args.split, args.refine,
+                           verbose)
         times.append(el)
         log(f"[bench] step {i}: {el:.2f}s nev={r.lam.size} deg={r.deg} steps={r.lan_steps} "
             f"max res2={np.max(r.res2) if r.res2.size else float('nan'):.2e}")
@@ -428,6 +675,8 @@ def main():
     persistent = pv.chebiter_info(mv.chebB)["persistent"]
     kname = ("cheb_persistent_k" if persistent else
              ("mass3_chebstep" if args.block == 1 else f"block{kpart}_chebstep"))
+    if args.refine:
+        kname = f"block{kpart}_chebstep32" if kpart > 1 else "chebstep32"
     ktag = f"{kname}:{args.workload}"
     traffic, traffic_src = traffic_from_profiles(ktag)
     line = {
@@ -441,12 +690,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if precision == pv.FP64 else "f32-filter/f64",
+        "dtype": ("f64" if precision == pv.FP64 and not args.refine else
+                  "f64 with f32 inner Chebyshev passes (f64 residuals)" if precision == pv.FP64 else
+                  "f32-filter/f64"),
         "data": "synthetic",
         "config": {
-            "workload": (f"{args.workload}: homogeneous solid unit ball (rho=1, vp=1.73, vs=1), "
-                         f"seeded jittered-lattice Delaunay mesh of {P.mesh.n_elem} P2 tets, "
-                         f"{P.n} DoF, all eigenpairs in [{interval[0]:.6g}, {interval[1]:.6g}]"),
+            "workload": WORKLOAD_TEXT.get(args.workload, args.workload).format(
+                tets=P.mesh.n_elem, n=P.n, a=interval[0], b=interval[1]),
+            "refine_passes": args.refine,
             "n_dof": P.n, "nnz_A": P.A.nnz, "nnz_B": P.B.nnz, "n_tets": P.mesh.n_elem,
             "interval": list(interval), "nev": int(r.lam.size), "filter_degree": r.deg,
             "bsolve_degree": mv.bdeg, "lanczos_steps": r.lan_steps,
@@ -480,9 +731,27 @@ def main():
             "filter_step": r.probe["filterstep"],
         },
     }
+    if args.fp64_compare and args.refine:
+        try:
+            el64, r64 = solve_once(mv, interval, bdeg, pv.FP64, 0, args.block, args.split, 0, verbose)
+            line["fp64_plain"] = {
+                "seconds": round(el64, 3), "nev": int(r64.lam.size), "lanczos_columns": r64.lan_steps,
+                "resid_max": float(np.max(r64.res2)) if r64.res2.size else None,
+                "speedup_of_value": round(el64 / T, 3),
+                "max_rel_eig_diff_vs_value": float(np.max(np.abs(r64.lam - r.lam) / np.abs(r.lam)))
+                if r64.lam.size == r.lam.size else None}
+            log(f"[bench] plain fp64: {line['fp64_plain']}")
+        except Exception as e:
+            line["fp64_plain"] = {"error": str(e)}
+    if P.n > 500000:
+        try:
+            line["filter_cost"] = filter_rates(mv, interval, args.block, args.split, 1e-5)
+            log(f"[bench] filter cost: {line['filter_cost']}")
+        except Exception as e:
+            line["filter_cost"] = {"error": str(e)}
     if args.block > 1:
         try:
-            line["reorth_gemm"] = reorth_rate(P.n, r.lan_steps, args.block)
+            line["reorth_gemm"] = reorth_rate(P.n, min(r.lan_steps, 512), args.block)
         except Exception as e:
             line["reorth_gemm"] = {"error": str(e)}
     if args.single_compare and args.block > 1:
@@ -533,12 +802,12 @@ def main():
             log(f"[bench] spmv rates: {line['spmv_hbm']}")
         except Exception as e:  # report, but do not lose the headline line
             line["spmv_hbm"] = {"error": str(e)}
-    if args.c2:
+    if args.fluid_pieces:
         try:
-            line["c2"] = c2_components()
-            log(f"[bench] c2 pieces: {line['c2']}")
+            line["c2_fluid"] = fluid_pieces(args)
+            log(f"[bench] c2 fluid pieces: {line['c2_fluid']}")
         except Exception as e:
-            line["c2"] = {"error": str(e)}
+            line["c2_fluid"] = {"error": str(e)}
     if args.cpu_seconds > 0:
         try:
             # the CPU restatement runs the reference's single-vector ChebLanNr:

@@ -522,6 +522,7 @@ struct KernelProbe {
 struct FsWork {
   DVec t[3], p[3];  // n*K and np*K
   DVec aw;          // operator output of a split filter chain (n*K)
+  FVec f32;         // inner FP32 work of a refined Ap~ solve (3 np*K)
 };
 struct FsOp {
   DevMat* Ad = nullptr;  // n x n
@@ -1417,7 +1418,9 @@ static void block_bsolve(Solver& S, int K, const double* b, double* out, double*
     }
     for (int i = 0; i < 3 * P; ++i) S.hb[i].ensure(S.n * H);
   }
-  const bool v32 = S.fp32v() && !C.persistent(S.fp32());
+  // FP32 work per part: FP32 iterates, or the inner passes of a refined
+  // solve (each concurrent part needs its own)
+  const bool v32 = (S.fp32v() && !C.persistent(S.fp32())) || C.refine > 0;
   if (v32)
     for (int q = 0; q < P; ++q) S.hf[q].ensure(3 * S.n * H);
   const bool graphed = S.graphs && !C.persistent(S.fp32());
@@ -1476,7 +1479,12 @@ static void apply_fs(FsOp& F, int K, const double* x, double* y, hipStream_t s, 
   k_block_spmv(K, *F.Ad, v0, w0, s, f32);   // w0 = Ad v0
   k_block_spmv(K, *F.ET, v0, x0, s, f32);   // x0 = ET v0
   k_rowmul(F.np, K, F.dp.p, x0, tp, s);     // x1 = Dp x0
-  block_cheb(*F.cp, K, tp, y0, x0, s, f32, nullptr);  // y0 ~ Ap~^-1 x1
+  float* w32 = nullptr;
+  if (F.cp->refine > 0) {
+    W.f32.ensure(3 * F.np * K);
+    w32 = W.f32.p;
+  }
+  block_cheb(*F.cp, K, tp, y0, x0, s, f32, nullptr, w32);  // y0 ~ Ap~^-1 x1
   k_rowmul(F.np, K, F.dp.p, y0, tp, s);     // y1 = Dp y0
   k_block_spmv(K, *F.E, tp, w1, s, f32);    // w1 = E y1
   k_add_rowmul(F.n, K, F.d.p, w0, w1, y, s);  // w = D (w0 + w1)
@@ -1533,7 +1541,7 @@ static void block_filter_split(Solver& S, int K, int P, const PolyFilter& Pf, co
     for (int q = 0; q < 3; ++q) NMG_HIP(hipStreamCreateWithFlags(&S.sx[q], hipStreamNonBlocking));
     for (int q = 0; q < 4; ++q) NMG_HIP(hipEventCreateWithFlags(&S.ev[q], hipEventDisableTiming));
   }
-  const bool v32 = S.fp32v() && !C.persistent(S.fp32());
+  const bool v32 = (S.fp32v() && !C.persistent(S.fp32())) || C.refine > 0;  // per-chain FP32 work
   double* zq[4];
   double* yq[4];
   double* mq[4];

@@ -52,7 +52,12 @@ def build_problem(name: str, porder: int = 2, seed: int = MESH_SEED):
 #   c1: a = lambda_7 / 2 (above the six rigid-body zero modes) and b at the
 #   midpoint of the first gap after the 36th eigenvalue, from the shift-invert
 #   eigenvalues of the c1 pencil (tests/golden/c1_eigs.json): 33 eigenpairs.
+#   c2s: placed by the KPM density of states of the c2s pencil (8 Rademacher
+#   vectors, degree 3000) to hold ~200 modes above the six rigid-body modes;
+#   the device's FP64 solve finds 231 pairs in it, from the 5-fold 218.42
+#   multiplet to 2830.19, with 15.6% and 0.5% margins to the ends.
 INTERVALS = {
     "tiny": (1.0, 12.0),
     "c1": (3.1423366861823694, 24.499667699964398),
+    "c2s": (189.0, 2845.0),
 }

@@ -367,8 +367,11 @@ int nmg_dense_product(int op, int kernel, int64_t n, int m, int K, double alpha,
 int nmg_bench_dense(int op, int kernel, int64_t n, int m, int K, int reps, double *seconds, double *bytes,
                     double *flops);
 /* Which kernels a solver's dense products run on (no reference
- * counterpart): 1 (default) the MFMA kernels, 0 rocBLAS DGEMM and the FMA
- * split reduction. */
+ * counterpart): 2 (default) the MFMA kernels, except A^T B with fewer than 16
+ * columns (the block recurrence's V^T W at block widths 1-8), which runs on
+ * the FMA split reduction; 1 the MFMA kernels throughout; 0 rocBLAS DGEMM
+ * and the FMA split reduction.  nmg_dense_product / nmg_bench_dense take
+ * the same kernel codes. */
 int nmg_set_dense_kernels(nmg_handle h, int mfma);
 /* A plain copy kernel of n doubles, `reps` times (calibration of HBM byte
  * counters: it moves 16 n bytes per launch); mean seconds per launch. */

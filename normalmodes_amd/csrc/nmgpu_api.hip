@@ -572,7 +572,10 @@ struct Solver {
   int target_met = 1;       // last cheblannr: every pair met resid_target (1), or it stopped on stagnation (0)
   double resid_best = 0.0;  // max ||A~x - lam B~x||/|lam| of the returned pairs at the residual stop (0: not checked)
   DVec mvx, mvy;            // staging of the MVFunc-shaped callbacks (nmg_amv / nmg_bmv / nmg_bsol)
-  int mfma = 1;             // dense products: 1 hand-written FP64 MFMA kernels, 0 rocBLAS DGEMM + FMA split reduction
+  // dense products: 2 (default) MFMA for A B and for A^T B with K >= 16 (the
+  // FMA split reduction for narrower A^T B, where a 16x16 MFMA tile would be
+  // mostly empty); 1 MFMA for all; 0 rocBLAS DGEMM + the FMA split reduction
+  int mfma = 2;
 
   std::shared_ptr<FsOp> fs;  // fluid-solid Schur operator in place of A (nmg_set_amv_fs)
   DVec aw;                   // product A~ in of the unfused filter step (n*K)
@@ -1066,7 +1069,7 @@ static void dense_nn(Solver& S, int64_t n, int K, int m, double alpha, const dou
 }
 static void dense_tn(Solver& S, int64_t n, int m, int K, const double* A, int64_t lda, const double* B,
                      int64_t ldb, double* C, int64_t ldc) {
-  if (!S.mfma) return gemm_tn_any(S, n, m, K, A, lda, B, ldb, C, ldc);
+  if (S.mfma == 0 || (S.mfma == 2 && K < 16)) return gemm_tn_any(S, n, m, K, A, lda, B, ldb, C, ldc);
   for (int c0 = 0; c0 < K; c0 += 64) {
     const int w = std::min(64, K - c0);
     S.gw.ensure((int64_t)gemm_tn_mfma_work_size(n, m, w));
@@ -3454,7 +3457,7 @@ static void with_dense_solver(int kernel, F&& f) {
   Solver S;
   S.s = default_stream();
   S.borrowed = true;
-  S.mfma = kernel ? 1 : 0;
+  S.mfma = kernel;
   struct Blas {
     rocblas_handle h = nullptr;
     ~Blas() {
@@ -3530,7 +3533,7 @@ int nmg_bench_dense(int op, int kernel, int64_t n, int m, int K, int reps, doubl
 int nmg_set_dense_kernels(nmg_handle h, int mfma) {
   return guarded([&] {
     Solver* S = get<Solver>(h, K_SOLVER);
-    NMG_CHECK(mfma == 0 || mfma == 1, NMG_EINVAL, "mode must be 0 or 1");
+    NMG_CHECK(mfma >= 0 && mfma <= 2, NMG_EINVAL, "mode must be 0, 1 or 2");
     S->mfma = mfma;
   });
 }

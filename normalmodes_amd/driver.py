@@ -284,7 +284,7 @@ def split_parts(block: int, split="auto") -> int:
 
 
 def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block=1, resid_target=None,
-                 split="auto", graphs=False, verbose=0, split_filter=True, dense_kernels=1):
+                 split="auto", graphs=False, verbose=0, split_filter=True, dense_kernels=2):
     """pEVSL_Start and the operator settings of src/mod_pevsl.f90:54-82."""
     h = pv.pevsl_start(0)                                     # :54
     try:
@@ -311,8 +311,8 @@ def _open_solver(mv: MatvecSetup, *, precision, seed, rtol, probe_every=0, block
                 pv.set_split_filter(h, False)
         if graphs:
             pv.set_graphs(h, True)
-        if dense_kernels != 1:
-            pv.set_dense_kernels(h, bool(dense_kernels))
+        if dense_kernels != 2:
+            pv.set_dense_kernels(h, dense_kernels)
         if verbose:
             pv.set_verbose(h, verbose)
         if resid_target is None and precision != pv.FP64:
@@ -333,10 +333,11 @@ def apply_pevsl(mv: MatvecSetup, *, lowfreq: float | None = None, upfreq: float 
                 rtol: float = 1e-9, vectors: bool = True, residual_check: bool = True,
                 probe_every: int = 0, block: int = 1, resid_target: float | None = None,
                 keep: bool = False, split="auto", graphs: bool = False, verbose: int = 0,
-                split_filter: bool = True, dense_kernels: int = 1) -> EigResult:
+                split_filter: bool = True, dense_kernels: int = 2) -> EigResult:
     """pnm_apply_pevsl (src/mod_pevsl.f90:16-222) on the device-resident operator.
-    ``dense_kernels``: 1 the FP64 MFMA kernels for the block recurrence and the
-    extraction's dense products, 0 rocBLAS / the FMA split reduction."""
+    ``dense_kernels``: 2 the FP64 MFMA kernels for the block recurrence and the
+    extraction's dense products except A^T B narrower than 16 columns (FMA
+    split reduction), 1 MFMA throughout, 0 rocBLAS / the FMA split reduction."""
     if interval is None:
         a, b = freq_to_lambda(lowfreq), freq_to_lambda(upfreq)
     else:

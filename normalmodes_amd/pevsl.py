@@ -470,8 +470,9 @@ def bench_dense(op: int, kernel: int, n: int, m: int, K: int, reps: int = 10):
     return sec.value, b.value, f.value
 
 
-def set_dense_kernels(h: Handle, mfma: bool = True):
-    _check(_lib().nmg_set_dense_kernels(_i64(h), ctypes.c_int(int(mfma))))
+def set_dense_kernels(h: Handle, mode: int = 2):
+    """2: MFMA except narrow A^T B (FMA split reduction); 1: MFMA throughout; 0: rocBLAS + FMA."""
+    _check(_lib().nmg_set_dense_kernels(_i64(h), ctypes.c_int(int(mode))))
 
 
 def bench_copy(n: int, reps: int = 10) -> float:

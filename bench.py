@@ -331,7 +331,7 @@ def run_fluid(args):
     kpart = block // split if split > 1 and block > 1 else block
     log(f"[bench] built {args.workload}: n={Q.n} pressures={Q.n_p} tets={Q.mesh.n_elem} in {t_build:.1f}s")
     t0 = time.perf_counter()
-    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg=args.apdeg, tune=sorted({1, kpart, block}))
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg=args.apdeg, tune=sorted({1, kpart, block}), verbose=True)
     t_setup = time.perf_counter() - t0
     lb = (3000, 5000, args.lanbounds_tol)
 
@@ -448,7 +448,7 @@ def fluid_pieces(args):
     Q = build_problem("c2")
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg="auto", tune=(1, 2))
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg="auto", tune=(1, 2), verbose=True)
     t_setup = time.perf_counter() - t0
     band = C2_BAND_MHZ
     interval = (driver.freq_to_lambda(band[0]), driver.freq_to_lambda(band[1]))
@@ -493,7 +493,7 @@ def fluid_rates_only(args):
     t_build = time.perf_counter() - t0
     log(f"[bench] built {args.workload}: n={Q.n} pressures={Q.n_p} in {t_build:.1f}s")
     t0 = time.perf_counter()
-    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg="auto", tune=(1, 4, 16))
+    mv = driver.setupmatvec_fs(Q, bdeg="auto", apdeg="auto", tune=(1, 4, 16), verbose=True)
     t_setup = time.perf_counter() - t0
     log(f"[bench] setupmatvec_fs {t_setup:.1f}s")
     mats = {k: getattr(Q, k) for k in ("Ad", "B", "E", "ET", "Ap")}

@@ -148,14 +148,20 @@ def ap_chebiter_degree(porder: int, lmin: float, lmax: float, policy="reference"
 
 
 def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bounds=(1000, 2000, 1e-12),
-                   ap_bounds=(2000, 3000, 1e-12), tune=(), apdeg="reference") -> MatvecSetup:
+                   ap_bounds=(2000, 3000, 1e-12), tune=(), apdeg="reference", verbose: bool = False) -> MatvecSetup:
     """setupmatvec for a body with fluid regions (src/mod_matvec.f90:24-222):
     B scaled and its Chebyshev solve as in the solid case; Ad, E, ET uploaded;
     Ap negated and scaled (Apdiagscaling), its bounds by LanBounds and its
     Chebyshev solve (degree by ``apdeg``, see ``ap_chebiter_degree``).
     ``tune``: block widths for which the layouts of B and Ad are autotuned on
-    the device."""
+    the device.  ``verbose``: a progress line on stderr after each phase
+    (long set-ups of large bodies)."""
+    import sys as _sys
     t0 = time.perf_counter()
+
+    def say(what):
+        if verbose:
+            print(f"[setupmatvec_fs] {what} at {time.perf_counter() - t0:.1f}s", file=_sys.stderr, flush=True)
     n = problem.n
     matB = pv.csr_create(problem.B.rowptr, problem.B.col, problem.B.val)
     d = pv.csr_diagscale(matB, 1.0)                                   # :47-71
@@ -164,25 +170,32 @@ def setupmatvec_fs(problem: FluidProblem, *, bdeg="reference", seed: int = 7, bo
                     matET=pv.csr_create(problem.ET.rowptr, problem.ET.col, problem.ET.val, n_cols=n),
                     matAp=pv.csr_create(problem.Ap.rowptr, problem.Ap.col, problem.Ap.val), dp=None)
     fs.dp = pv.csr_diagscale(fs.matAp, -1.0)                          # :137-140, 345-441
+    say("uploaded")
     tuned = {}
     for K in tune:
         for key, mat in (("B", matB), ("Ad", fs.matAd)):
             if pv.csr_info(mat)["layout"] != "csr":
                 tuned[f"{key}{K}"] = pv.csr_autotune(mat, K, 10)
+        say(f"autotuned {K} columns")
     hp = pv.pevsl_start(0)                                            # :152
     try:
         pv.pevsl_setprobsizes(hp, problem.n_p, problem.n_p, -1)
         pv.pevsl_set_seed(hp, seed)
         pv.pevsl_setamv(hp, fs.matAp)                                 # :157
+        if verbose:
+            pv.set_verbose(hp, 100)
         fs.aplmin, fs.aplmax = pv.pevsl_lanbounds(hp, *ap_bounds)     # :160-162
     finally:
         pv.pevsl_finish(hp)
+    say("Ap bounds")
     fs.apdeg = ap_chebiter_degree(problem.porder, fs.aplmin, fs.aplmax, apdeg)
     fs.chebAp = pv.pevsl_setup_chebiter(fs.aplmin, fs.aplmax, fs.apdeg, fs.matAp)   # :174
     mv = MatvecSetup(n=n, matA=0, matB=matB, d=d, blmin=0.0, blmax=0.0, bdeg=0, chebB=0,
                      porder=problem.porder, timings=dict(upload=time.perf_counter() - t0), fs=fs)
     mv.tuned = tuned
-    return setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds)
+    mv = setup_bsolve(mv, bdeg=bdeg, seed=seed, bounds=bounds, verbose=verbose)
+    say("B bounds and Chebyshev solve")
+    return mv
 
 
 def upload(problem, *, porder: int | None = None, tune=(1,)) -> MatvecSetup:
@@ -214,7 +227,7 @@ def upload(problem, *, porder: int | None = None, tune=(1,)) -> MatvecSetup:
 
 
 def setup_bsolve(mv: MatvecSetup, *, bdeg="reference", seed: int = 7,
-                 bounds=(1000, 2000, 1e-12)) -> MatvecSetup:
+                 bounds=(1000, 2000, 1e-12), verbose: bool = False) -> MatvecSetup:
     """Lanczos bounds of B~ and its Chebyshev solve (src/mod_matvec.f90:75-95)."""
     t0 = time.perf_counter()
     if mv.chebB:
@@ -225,6 +238,8 @@ def setup_bsolve(mv: MatvecSetup, *, bdeg="reference", seed: int = 7,
         pv.pevsl_setprobsizes(hB, mv.n, mv.n, -1)
         pv.pevsl_set_seed(hB, seed)
         pv.pevsl_setamv(hB, mv.matB)                     # :80
+        if verbose:
+            pv.set_verbose(hB, 100)
         mv.blmin, mv.blmax = pv.pevsl_lanbounds(hB, *bounds)   # :83-85
     finally:
         pv.pevsl_finish(hB)                              # :95

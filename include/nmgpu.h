@@ -148,6 +148,23 @@ int nmg_chebiter_free(nmg_handle cheb);
  * steps are separated by grid barriers; slower on MI355X at the sizes
  * measured, because of the barrier cost). */
 int nmg_chebiter_set_mode(nmg_handle cheb, int mode);
+/* --- device FE producer (solid regions) --------------------------------- */
+/* The element stiffness and mass of CGE3D_ISO (src/mod_cg_create_matrix.f90
+ * :980-1266) formed on the GPU, one wavefront per element, and gathered into
+ * the CSR values of matrixstruct's pattern (:1269-1455) in ascending element
+ * order per row (deterministic; A exactly symmetric).  Host arrays in and
+ * out: nodes [n_nodes*3], elems [n_elem*np] (0-based, reference local
+ * order), vslot [4] vertex slots, lam/mu/rho [n_elem*np], g [n_elem*np*3]
+ * reference gravity (km/s^2) or NULL (JOB=1), refM [np*np], refD [3*np*np],
+ * node adjacency adj_ptr [n_nodes+1] / adj (sorted), the row offsets of A
+ * (3x3 blocks over the adjacency) and B (scalar mass per component); fills
+ * a_val and b_val.  np = 4 or 10.  Fails on a flat element. */
+int nmg_assemble_elastic(int64_t n_nodes, int64_t n_elem, int np, const double *nodes, const int32_t *elems,
+                         const int32_t *vslot, const double *lam, const double *mu, const double *rho,
+                         const double *g, const double *refM, const double *refD, const int64_t *adj_ptr,
+                         const int32_t *adj, const int64_t *a_rowptr, double *a_val, const int64_t *b_rowptr,
+                         double *b_val);
+
 /* Mixed-precision refinement of the Chebyshev solve (no reference
  * counterpart): with passes >= 1 every solve runs `passes` inner Chebyshev
  * solves of degree inner_deg on FP32 values and FP32 iterates (FP64 row

@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={HIP_ARCH}", "-fPIC", "-share
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-fopenmp", f"-I{INCLUDE}"]
 
-GPU_SOURCES = ["nmgpu_kernels.hip", "nmgpu_mfma.hip", "nmgpu_api.hip", "nmgpu_host.cpp"]
+GPU_SOURCES = ["nmgpu_kernels.hip", "nmgpu_mfma.hip", "nmgpu_assemble.hip", "nmgpu_api.hip", "nmgpu_host.cpp"]
 GPU_LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 

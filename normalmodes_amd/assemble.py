@@ -113,11 +113,40 @@ def node_adjacency(mesh: TetMesh):
     return ptr, adj
 
 
+def csr_pattern(ptr, adj):
+    """Row offsets and columns of A (3x3 blocks: row 3v+p holds 3w+q for
+    every neighbour w of v, ascending) and B (row 3v+p holds 3w+p) from the
+    sorted node adjacency: the layout of matrixstruct
+    (src/mod_cg_create_matrix.f90:1269-1455)."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    adj = np.asarray(adj, dtype=np.int64)
+    deg = np.diff(ptr)
+    n = deg.size
+
+    def ranges(starts, lens):
+        tot = int(lens.sum())
+        off = np.concatenate([[0], np.cumsum(lens)[:-1]])
+        return np.arange(tot, dtype=np.int64) - np.repeat(off - starts, lens)
+
+    a_len = np.repeat(3 * deg, 3)
+    a_rowptr = np.concatenate([[0], np.cumsum(a_len)]).astype(np.int64)
+    c3 = (3 * adj[:, None] + np.arange(3)[None, :]).ravel()          # per node, 3 per neighbour
+    a_col = c3[ranges(np.repeat(3 * ptr[:-1], 3), a_len)].astype(np.int32)
+    b_len = np.repeat(deg, 3)
+    b_rowptr = np.concatenate([[0], np.cumsum(b_len)]).astype(np.int64)
+    w = adj[ranges(np.repeat(ptr[:-1], 3), b_len)]
+    p = np.repeat(np.tile(np.arange(3), n), b_len)
+    b_col = (3 * w + p).astype(np.int32)
+    return a_rowptr, a_col, b_rowptr, b_col
+
+
 def assemble(mesh: TetMesh, model: RadialModel | None = None, *, rho=None, vp=None, vs=None,
-             g0=None, job: int = 1, name: str = "") -> Problem:
+             g0=None, job: int = 1, name: str = "", device: bool = False) -> Problem:
     """Assemble A and B.  Give either a model or per-element-node rho, vp, vs.
 
     ``g0`` (n_elem, pNp, 3) is the reference gravity in km/s^2, used when job == 2.
+    ``device``: form the element matrices of a solid body on the GPU
+    (``nmg_assemble_elastic``; the pattern is built here) instead of the CPU.
     """
     if model is not None:
         rho, vp, vs = element_node_values(mesh, model)
@@ -156,6 +185,14 @@ def assemble(mesh: TetMesh, model: RadialModel | None = None, *, rho=None, vp=No
     rho_c = np.ascontiguousarray(rho, dtype=np.float64)
     nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
     elems = np.ascontiguousarray(mesh.elems, dtype=np.int32)
+    if device:
+        from . import pevsl
+        a_rowptr, a_col, b_rowptr, b_col = csr_pattern(ptr, adj)
+        pevsl.assemble_elastic(nodes, elems, vslot, lam_c, mu_c, rho_c, g, M, D, ptr, adj, a_rowptr, a_val,
+                               b_rowptr, b_val)
+        n = 3 * nn
+        return Problem(A=CSR(n, n, a_rowptr, a_col, a_val), B=CSR(n, n, b_rowptr, b_col, b_val),
+                       mesh=mesh, lam=lam, mu=mu, rho=rho, job=job, name=name)
     rc = lib.nm_assemble_elastic(
         ctypes.c_int64(nn), ctypes.c_int64(ne), ctypes.c_int(npl), _p(nodes, _f64p), _p(elems, _i32p),
         _p(vslot, _i32p), _p(lam_c, _f64p), _p(mu_c, _f64p), _p(rho_c, _f64p), _p(g, _f64p),

@@ -3559,6 +3559,21 @@ int nmg_chebiter_set_mode(nmg_handle cheb, int mode) {
   });
 }
 
+int nmg_assemble_elastic(int64_t n_nodes, int64_t n_elem, int np, const double* nodes, const int32_t* elems,
+                         const int32_t* vslot, const double* lam, const double* mu, const double* rho,
+                         const double* g, const double* refM, const double* refD, const int64_t* adj_ptr,
+                         const int32_t* adj, const int64_t* a_rowptr, double* a_val, const int64_t* b_rowptr,
+                         double* b_val) {
+  return guarded([&] {
+    NMG_CHECK(nodes && elems && vslot && lam && mu && rho && refM && refD && adj_ptr && adj && a_rowptr && a_val &&
+                  b_rowptr && b_val,
+              NMG_EINVAL, "null argument");
+    const int rc = assemble_elastic_device(n_nodes, n_elem, np, nodes, elems, vslot, lam, mu, rho, g, refM, refD,
+                                           adj_ptr, adj, a_rowptr, a_val, b_rowptr, b_val, default_stream());
+    NMG_CHECK(rc == 0, NMG_EINVAL, "flat element " + std::to_string(-rc - 1));
+  });
+}
+
 int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);

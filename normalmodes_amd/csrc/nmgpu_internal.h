@@ -224,6 +224,13 @@ void k_block_cheb_step32(int K, const DevMat& M, const float* b, const float* xk
 void k_cheb32_start(int64_t n, double ith, const double* b, float* b32, float* x1, float* x0, hipStream_t s);
 void k_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t s);
 void k_sub_from(int64_t n, const double* b, double* y, hipStream_t s);   // y = b - y
+// device producer of the solid-region A and B values on a given CSR pattern
+// (nmgpu_assemble.hip); 0, or -(1 + e) for a flat element e
+int assemble_elastic_device(int64_t n_nodes, int64_t n_elem, int np, const double* nodes, const int32_t* elems,
+                            const int32_t* vslot, const double* lam, const double* mu, const double* rho,
+                            const double* g, const double* refM, const double* refD, const int64_t* adj_ptr,
+                            const int32_t* adj, const int64_t* a_rowptr, double* a_val, const int64_t* b_rowptr,
+                            double* b_val, hipStream_t s);
 void k_add_f32(int64_t n, const float* x, double* y, hipStream_t s);     // y += x
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32 = false);

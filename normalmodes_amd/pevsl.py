@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_filter", "chebiter_set_refine", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
+           "bench_bsolve", "bench_filter", "chebiter_set_refine", "assemble_elastic", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -389,6 +389,22 @@ def apply(h: Handle, which: int, x):
     y = np.empty_like(x)
     _check(_lib().nmg_apply(_i64(h), ctypes.c_int(which), _p(x, _f64p), _p(y, _f64p)))
     return y
+
+
+def assemble_elastic(nodes, elems, vslot, lam, mu, rho, g, refM, refD, adj_ptr, adj, a_rowptr, a_val,
+                     b_rowptr, b_val):
+    """Solid-region A and B values on the GPU (nmg_assemble_elastic); a_val
+    and b_val (preallocated, float64) are filled in place."""
+    n_nodes = adj_ptr.size - 1
+    n_elem, np_ = elems.shape
+    for a, t in ((a_val, np.float64), (b_val, np.float64)):
+        if a.dtype != t or not a.flags.c_contiguous:
+            raise ValueError("a_val / b_val must be contiguous float64")
+    _check(_lib().nmg_assemble_elastic(
+        _i64(n_nodes), _i64(n_elem), ctypes.c_int(np_), _p(nodes, _f64p), _p(elems, _i32p), _p(vslot, _i32p),
+        _p(lam, _f64p), _p(mu, _f64p), _p(rho, _f64p), None if g is None else _p(g, _f64p), _p(refM, _f64p),
+        _p(refD, _f64p), _p(adj_ptr, _i64p), _p(adj, _i32p), _p(a_rowptr, _i64p), _p(a_val, _f64p),
+        _p(b_rowptr, _i64p), _p(b_val, _f64p)))
 
 
 def chebiter_set_refine(cheb: Handle, passes: int, inner_deg: int = 0):

@@ -12,6 +12,19 @@ sys.path.insert(0, str(ROOT))
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libnmgpu.so")
     config.addinivalue_line("markers", "slow: longer CPU test")
+    config.addinivalue_line("markers", "gpu_long: GPU parity test of several minutes (run on its own with "
+                                       "-m gpu_long -s, so that its progress lines show)")
+
+
+def pytest_collection_modifyitems(config, items):
+    """gpu_long tests run only when the marker expression names them."""
+    expr = config.getoption("-m") or ""
+    if "gpu_long" in expr:
+        return
+    skip = pytest.mark.skip(reason="several-minute GPU test: run with -m gpu_long -s")
+    for item in items:
+        if item.get_closest_marker("gpu_long"):
+            item.add_marker(skip)
 
 
 def _gpu_available():

@@ -323,7 +323,7 @@ def test_assembled_refined_solves(assembled):
     assert np.all(r.res2 <= 1e-10) and r.target_met
 
 
-@pytest.mark.gpu
+@pytest.mark.gpu_long
 def test_assembled_seismic_band(assembled):
     """The ~20 lowest seismic modes above the fluid's zero-energy cluster.
     There lam_max / lam is ~2e4 for these Earth-sized bodies, and every FP64

@@ -632,7 +632,30 @@ def cpu_full_solve(P, interval, bdeg):
                       if k in ("nsteps", "deg", "lmin", "lmax", "lanbounds_steps", "ncand")})
 
 
-def main():
+def resolve_defaults(args):
+    """Per-workload defaults of the flags left unset; returns whether the
+    workload is a fluid-solid body.  c2s (the headline) runs block 4 as two
+    2-column chains, plain FP64, no warm-up solve, and measures the fluid
+    ball's pieces beside it; c1 keeps its block 8 and its comparisons."""
+    fluid = args.workload in ("c2", "c2r", "c4")
+    big = args.workload == "c2s"
+    if args.warmup is None:
+        args.warmup = 0 if fluid or big else 1
+    if not fluid:
+        args.block = args.block or (4 if big else 8)
+        args.split = 2 if args.split is None else args.split
+    args.refine = 0 if args.refine is None else args.refine
+    for name, small_default in (("single_compare", 1), ("fp32_compare", 1), ("adjoint", 1)):
+        if getattr(args, name) is None:
+            setattr(args, name, 0 if (big or fluid) else small_default)
+    if args.fluid_pieces is None:
+        args.fluid_pieces = 1 if big else 0
+    if args.spmv_workload is None:
+        args.spmv_workload = "none" if big or fluid else "c2s"
+    return fluid
+
+
+def build_parser():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed full solves")
@@ -682,30 +705,21 @@ def main():
                     help="'auto': autotune the product layouts on the device; or fixed lanes per node row as "
                          "'A:<g>,B:<g>' (g in 8,16,32,64), e.g. for profiler runs, where timing-based tuning "
                          "is perturbed")
-    args = ap.parse_args()
+    return ap
+
+
+def main(argv=None):
+    args = build_parser().parse_args(argv)
 
     if args.gpus != 1:
         log("nmgpu runs on one GPU: collectives and sharding are not part of it "
             "(the path would shard by spectrum slices, one group per GPU); run with --gpus 1")
         sys.exit(2)
     bdeg = args.bdeg if args.bdeg in ("auto", "reference") else int(args.bdeg)
-    if args.warmup is None:
-        args.warmup = 0 if args.workload in ("c2", "c2r", "c2s") else 1
-    fluid = args.workload in ("c2", "c2r", "c4")
+    fluid = resolve_defaults(args)
     if fluid and not (args.kernel_only or args.spmv_only):
         run_fluid(args)
         return
-    big = args.workload == "c2s"
-    args.block = args.block or (4 if fluid or big else 8)
-    args.split = 2 if args.split is None else args.split
-    args.refine = 0 if args.refine is None else args.refine
-    for name, small_default in (("single_compare", 1), ("fp32_compare", 1), ("adjoint", 1)):
-        if getattr(args, name) is None:
-            setattr(args, name, 0 if big else small_default)
-    if args.fluid_pieces is None:
-        args.fluid_pieces = 1 if big else 0
-    if args.spmv_workload is None:
-        args.spmv_workload = "none" if big else "c2s"
     if args.kernel_only:
         args.bdeg = bdeg
         kernel_only(args)

@@ -53,3 +53,16 @@ def test_profile_commands_use_valid_layout_specs():
 def test_bad_layout_specs_are_refused(spec):
     with pytest.raises(ValueError):
         _bench().parse_layouts(spec)
+
+
+@pytest.mark.parametrize("workload,block,split,warmup,pieces", [("c2s", 4, 2, 0, 1), ("c1", 8, 2, 1, 0)])
+def test_workload_defaults(workload, block, split, warmup, pieces):
+    """The headline (c2s) runs block 4 as two chains, plain FP64, without a
+    warm-up solve and with the fluid ball's pieces; c1 keeps block 8."""
+    bench = _bench()
+    args = bench.build_parser().parse_args(["--workload", workload])
+    assert bench.resolve_defaults(args) is False
+    assert (args.block, args.split, args.warmup, args.fluid_pieces, args.refine) == (block, split, warmup, pieces, 0)
+    assert bench.build_parser().parse_args([]).workload == "c2s"
+    a2 = bench.build_parser().parse_args(["--workload", "c2"])
+    assert bench.resolve_defaults(a2) is True and a2.warmup == 0

@@ -38,9 +38,10 @@ def _check_gate(P, r, gate=1e-10):
     recomputed 2-norm residual meets the gate."""
     r1, r2 = _recomputed_residuals(P, r.lam, r.X)
     assert np.all(r2 <= gate), r2
-    # a residual near 1e-13 is the difference of terms of order |lam|: its
-    # own rounding (~1e-15 relative to |lam|) is the absolute floor
-    floor = 1e-14
+    # two evaluations of the same residual differ by its own rounding: the
+    # product A~x carries eps ||A~|| ||x||, i.e. ~eps lam_max / |lam| relative
+    # to |lam| (and never less than ~1e-14)
+    floor = np.maximum(1e-14, 0.1 * np.finfo(float).eps * abs(r.lmax) / np.abs(r.lam))
     assert np.all(np.abs(r.res2 - r2) <= 1e-3 * r2 + floor), (r.res2, r2)
     assert np.all(np.abs(r.res_ref - r1) <= 1e-3 * r1 + floor / np.sqrt(P.n)), (r.res_ref, r1)
 

@@ -27,14 +27,6 @@ def pytest_collection_modifyitems(config, items):
             item.add_marker(skip)
 
 
-def _gpu_available():
-    try:
-        import torch  # noqa: F401
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def small_p2():
     """P2 homogeneous ball, ~300 tets (1.7k DoF): the dense oracle runs in seconds."""

@@ -679,7 +679,9 @@ def build_parser():
     ap.add_argument("--lanbounds-tol", type=float, default=1e-5, help="LanBounds tolerance (the reference's 1e-5)")
     ap.add_argument("--fp32-mode", default="filter_A", choices=["values", "filter_A", "iterates"],
                     help="precision of the FP32-filter solve: B~ values; + stiffness values; B~ values + iterates")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32a", "fp32i"],
+                    help="timed solve: fp64; fp32 = FP32 B~ values in the B-solve; fp32a = and FP32 stiffness "
+                         "values in the filter; fp32i = FP32 B~ values and iterates in the k-column B-solves")
     ap.add_argument("--bdeg", default="auto", help="B-solve degree: auto | reference | <int>")
     ap.add_argument("--probe-every", type=int, default=64)
     ap.add_argument("--block", type=int, default=0,
@@ -748,7 +750,8 @@ def main(argv=None):
     kpart = args.block // args.split if args.block > 1 and args.split > 1 else args.block
     mv = upload_with_layouts(P, args.layouts, sorted({1, args.block, kpart}))
     t_upload = time.perf_counter() - t0
-    precision = pv.FP64 if args.precision == "fp64" else pv.FP32_FILTER
+    precision = {"fp64": pv.FP64, "fp32": pv.FP32_FILTER, "fp32a": pv.FP32_FILTER_A,
+                 "fp32i": pv.FP32_BSOLVE}[args.precision]
 
     verbose = max(1, 24 // args.block) if P.n > 500000 else 0   # progress lines of long solves
     for i in range(args.warmup):
@@ -787,7 +790,9 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": ("f64" if precision == pv.FP64 and not args.refine else
                   "f64 with f32 inner Chebyshev passes (f64 residuals)" if precision == pv.FP64 else
-                  "f32 B-solve values, f64 stiffness and recurrence"),
+                  {pv.FP32_FILTER: "f32 B-solve values, f64 stiffness and recurrence",
+                   pv.FP32_FILTER_A: "f32 B-solve and stiffness values in the filter, f64 recurrence",
+                   pv.FP32_BSOLVE: "f32 B-solve values and iterates, f64 stiffness and recurrence"}[precision]),
         "data": "synthetic",
         "config": {
             "workload": WORKLOAD_TEXT.get(args.workload, args.workload).format(

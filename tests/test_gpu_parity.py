@@ -495,8 +495,9 @@ def test_fp32_stiffness_filter_error_stated(small_p2, dense_eigs, block):
     err = np.abs(r32.lam - r64.lam) / r64.lam
     print(f"FP32 stiffness filter, block {block}: max rel eig err {err.max():.2e}, "
           f"max residual {r32.res2.max():.2e}", flush=True)
-    assert err.max() < 1e-6
-    assert np.all(r32.res2 < 1e-2)
+    # measured on MI355X: 1.8e-13 and 2.5e-6 (block 1 and 8)
+    assert err.max() < 1e-11
+    assert np.all(r32.res2 < 1e-4)
 
 
 @pytest.mark.parametrize("block", [1, 8])

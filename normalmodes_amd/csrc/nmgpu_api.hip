@@ -3585,6 +3585,9 @@ int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg) {
     }
     C->refine = passes;
     C->inner_deg = passes > 0 ? inner_deg : 0;
+    // the single-column solve's FP32 work, allocated here so that a solve
+    // replayed from a captured graph never allocates inside the capture
+    if (passes > 0) C->own32.ensure(3 * C->mat->nrow);
   });
 }
 

@@ -624,9 +624,11 @@ def cpu_full_solve(P, interval, bdeg):
     is checked against."""
     sys.path.insert(0, str(ROOT))
     from oracle import oracle as orc
+    orc.set_verbose(20)
     t0 = orc.wtime()
     lam, _, info = orc.solve_interval(P.A, P.B, interval, bdeg=bdeg, return_info=True)
     sec = orc.wtime() - t0
+    orc.set_verbose(0)
     return dict(seconds=round(sec, 2), nev=int(lam.size), cores=orc.num_threads(), cpu_model=cpu_model(),
                 info={k: (v if isinstance(v, (int, float)) else str(v)) for k, v in info.items()
                       if k in ("nsteps", "deg", "lmin", "lmax", "lanbounds_steps", "ncand")})

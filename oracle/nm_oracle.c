@@ -47,6 +47,18 @@ typedef struct {
   int amode;         /* 0: operator D A D ; 1: operator B~ */
 } or_prob;
 
+/* progress lines on stderr every `or_verbose_every` Lanczos steps (0: silent),
+ * for long CPU solves */
+#include <stdio.h>
+static int or_verbose_every = 0;
+void or_set_verbose(int every) { or_verbose_every = every; }
+static void or_progress(const char *what, int k) {
+  if (or_verbose_every > 0 && k % or_verbose_every == 0) {
+    fprintf(stderr, "[oracle] %s: step %d\n", what, k);
+    fflush(stderr);
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* vectors                                                             */
 static double dot(int64_t n, const double *x, const double *y) {
@@ -435,6 +447,7 @@ int or_lanbounds(const or_prob *p, int geneig, int mlan, int maxit, double tol, 
   for (k = 0; k < m; ++k) {
     double *v = L.V + (size_t)k * n, *z = L.Z + (size_t)k * n;
     double *vn = L.V + (size_t)(k + 1) * n, *zn = L.Z + (size_t)(k + 1) * n;
+    or_progress("lanbounds", k);
     or_apply_a(p, v, zn, work);
     if (k > 0) axpy(n, -beta, L.Z + (size_t)(k - 1) * n, zn);
     double alpha = dot(n, v, zn);
@@ -688,6 +701,7 @@ int or_cheblannr(const or_prob *p, int geneig, const double *xintv, int maxit, d
   for (k = 0; k < maxit; ++k) {
     double *v = L.V + (size_t)k * n, *z = L.Z + (size_t)k * n;
     double *vn = L.V + (size_t)(k + 1) * n, *zn = L.Z + (size_t)(k + 1) * n;
+    or_progress("cheblannr", k);
     or_chebav(p, geneig, deg, mu, pol->cc, pol->dd, z, zn, work);
     if (k > 0) axpy(n, -beta, L.Z + (size_t)(k - 1) * n, zn);
     double alpha = dot(n, v, zn);

@@ -383,6 +383,12 @@ def wtime():
     return lib().or_wtime()
 
 
+def set_verbose(every: int):
+    """Progress lines on stderr every ``every`` Lanczos steps of the C
+    restatement (0: silent), for long CPU solves."""
+    lib().or_set_verbose(ctypes.c_int(int(every)))
+
+
 def solve_interval(A, B, xintv_ab, *, bdeg=45, thr_int=0.8, thr_ext=0.7, lb=(3000, 5000, 1e-5),
                    seed=7, tol=1e-5, rtol=1e-9, maxit=9624, return_info=False):
     """Full restated path of pnm_apply_pevsl (src/mod_pevsl.f90:54-162) on CPU.

@@ -102,3 +102,14 @@ def test_slice_waves_partition_and_fit():
     assert len(driver.plan_slice_waves(counts, n, 200e9)) >= 2
     # width cap
     assert all(i1 - i0 <= 4 for i0, i1 in driver.plan_slice_waves(np.ones(10), 100, 1e12, max_width=4))
+
+
+def test_refine_inner_degree_meets_its_per_pass_bound():
+    """The inner degree of a refinement pass: the smallest c with
+    2 rho^c <= the per-pass target (the FP32 iterates' own floor is ~1e-6)."""
+    for lo, hi, target in ((0.25, 4.35, 1e-6), (0.3, 3.0, 1e-5), (1.0, 2.0, 1e-7)):
+        c = driver.refine_inner_degree(lo, hi, target)
+        rho = (math.sqrt(hi / lo) - 1) / (math.sqrt(hi / lo) + 1)
+        assert 2 * rho ** c <= target < 2 * rho ** (c - 1)
+    # two passes of the default reach about the square of the per-pass error
+    assert driver.refine_inner_degree(0.25, 4.35) < driver.chebiter_degree(2, 0.25, 4.35, "auto")

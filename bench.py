@@ -10,16 +10,19 @@ FindPol, ChebLanNr with Rayleigh-Ritz extraction and the residual check
 Mesh generation, assembly and the host->HBM upload are outside the timed
 region and reported separately.
 
-Workload (default ``c2``): the three-layer ball with a fluid outer core,
-radius 6371 km, JOB=2 with its reference gravity, seeded ~466k-tet P2 mesh,
-~2M displacement DoF plus ~1e5 pressures; all eigenpairs of the seismic band
-``--band-mhz``.  The timed step is B~ bounds and Chebyshev set-up, LanBounds,
-FindPol, block ChebLanNr (4 columns as two 2-column filter chains on two
-streams) with extraction, and the residual check; beside it one FP32-filter
-solve, the filter's cost per column and degree by precision, every product
-rate and the oracle's CPU baseline.  ``--workload c1``: the homogeneous
-solid unit ball, seeded ~12k-tet P2 mesh, ~52k DoF, the interval holding the
-~30 lowest non-rigid modes (the earlier headline, kept for comparison).
+Workload (default ``c2s``): the three-layer unit ball (rho 13->4, vp 11->8,
+vs 3.5->6.5) with its outer core solid, seeded ~459k-tet isosurface-stuffed
+P2 mesh, ~1.9M DoF; all eigenpairs in [189, 2845] (231 pairs).  The timed
+step is B~ bounds and Chebyshev set-up, LanBounds, FindPol, block ChebLanNr
+(4 columns as two 2-column filter chains on two streams) with extraction,
+and the residual check; beside it the filter's cost per column and degree
+by precision, the pieces of the same ball with its fluid outer core
+(``c2``: set-up, LanBounds, FindPol, filter cost, product rates) and the
+oracle's CPU baseline.  One solve takes minutes here, so the 2M-DoF
+workloads time exactly one solve with no warm-up solve unless
+``--long-solves 1``; the line records the steps asked for.  ``--workload
+c1``: the homogeneous solid unit ball, seeded ~12k-tet P2 mesh, ~52k DoF,
+the interval holding the ~30 lowest non-rigid modes.
 
 Also reported on the c1 line:
   roofline      the dominant kernel (fused Chebyshev B-solve step), algorithmic
@@ -363,6 +366,7 @@ def run_fluid(args):
     line = {
         "metric": METRIC, "value": round(T, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(T * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "steps_requested": args.requested,
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {
             "workload": (f"{args.workload}: three-layer ball of radius 6371 km (solid inner core r<0.19, fluid "
@@ -643,6 +647,17 @@ def resolve_defaults(args):
     big = args.workload == "c2s"
     if args.warmup is None:
         args.warmup = 0 if fluid or big else 1
+    args.requested = {"steps": args.steps, "warmup": args.warmup}
+    args.capped = False
+    if (big or fluid) and not (args.kernel_only or args.spmv_only or args.long_solves):
+        # one whole solve at 2M DoF takes about nine minutes: time exactly one,
+        # with no warm-up solve (the set-up's autotuner has already launched
+        # every kernel of the solve), whatever --steps / --warmup ask for
+        if args.steps != 1 or args.warmup != 0:
+            log(f"[bench] {args.workload}: a whole solve takes minutes, so this run times 1 solve with no "
+                f"warm-up solve (asked: --steps {args.steps} --warmup {args.warmup}; --long-solves 1 keeps them)")
+            args.capped = True
+        args.steps, args.warmup = 1, 0
     if not fluid:
         args.block = args.block or (4 if big else 8)
         args.split = 2 if args.split is None else args.split
@@ -701,6 +716,9 @@ def build_parser():
                     help="size for the SpMV HBM rates; none skips (default: the headline's own matrices for c2s, "
                          "c2s for c1)")
     ap.add_argument("--fp32-compare", type=int, default=None, help="also run one FP32-filter solve (0 skips)")
+    ap.add_argument("--long-solves", type=int, default=0,
+                    help="1: run --steps / --warmup whole solves as asked on the 2M-DoF workloads (each takes "
+                         "minutes); 0 (default): time exactly one solve there, with no warm-up solve")
     ap.add_argument("--kernel-only", action="store_true", help="copy kernel + dominant kernel only (counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="copy kernel + the --spmv-workload product rates only, --steps launches each (counter runs)")
@@ -786,6 +804,7 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "steps_requested": args.requested,
         "ms_per_step": round(T * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",

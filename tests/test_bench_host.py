@@ -66,3 +66,23 @@ def test_workload_defaults(workload, block, split, warmup, pieces):
     assert bench.build_parser().parse_args([]).workload == "c2s"
     a2 = bench.build_parser().parse_args(["--workload", "c2"])
     assert bench.resolve_defaults(a2) is True and a2.warmup == 0
+
+
+@pytest.mark.parametrize("workload,capped", [("c2s", True), ("c2", True), ("c1", False)])
+def test_long_solves_time_one_solve(workload, capped):
+    """A 2M-DoF whole solve takes minutes: asked for 20 timed and 5 warm-up
+    solves, the bench times one with no warm-up and records what was asked;
+    c1 keeps the steps as asked, and so does --long-solves 1."""
+    bench = _bench()
+    args = bench.build_parser().parse_args(["--workload", workload, "--steps", "20", "--warmup", "5"])
+    bench.resolve_defaults(args)
+    assert args.requested == {"steps": 20, "warmup": 5}
+    assert args.capped is capped
+    assert (args.steps, args.warmup) == ((1, 0) if capped else (20, 5))
+    keep = bench.build_parser().parse_args(["--workload", workload, "--steps", "3", "--warmup", "1",
+                                            "--long-solves", "1"])
+    bench.resolve_defaults(keep)
+    assert (keep.steps, keep.warmup, keep.capped) == (3, 1, False)
+    counters = bench.build_parser().parse_args(["--workload", workload, "--steps", "20", "--kernel-only"])
+    bench.resolve_defaults(counters)
+    assert counters.steps == 20

@@ -652,7 +652,7 @@ def resolve_defaults(args):
     if (big or fluid) and not (args.kernel_only or args.spmv_only or args.long_solves):
         # one whole solve at 2M DoF takes about nine minutes: time exactly one,
         # with no warm-up solve (the set-up's autotuner has already launched
-        # every kernel of the solve), whatever --steps / --warmup ask for
+        # the product and B-solve kernels), whatever --steps / --warmup ask for
         if args.steps != 1 or args.warmup != 0:
             log(f"[bench] {args.workload}: a whole solve takes minutes, so this run times 1 solve with no "
                 f"warm-up solve (asked: --steps {args.steps} --warmup {args.warmup}; --long-solves 1 keeps them)")

@@ -165,6 +165,31 @@ int nmg_assemble_elastic(int64_t n_nodes, int64_t n_elem, int np, const double *
                          const int32_t *adj, const int64_t *a_rowptr, double *a_val, const int64_t *b_rowptr,
                          double *b_val);
 
+/* --- device FE producer (fluid-solid bodies) ------------------------------ */
+/* Ad, B, E, ET and Ap of CGFSE3D_ISO (src/mod_cg_create_matrix.f90:103-977)
+ * on the layout of matrixstruct_general (:1458-2033), formed on the GPU:
+ * per-element geometry and face data and the fluid element matrices one
+ * thread per element, the solid elements' stiffness with the solid
+ * producer's kernel, and every row gathered one thread per node in
+ * ascending element order (deterministic; E = ET^T exactly).  The arguments
+ * and outputs are those of the CPU producer nm_assemble_fs (host arrays):
+ * nfp face nodes (3 or 6), massF [4*nfp*nfp] and fmask [4*nfp] of the
+ * reference faces, efluid [n_elem], neigh [n_elem*4] (-1 on the boundary),
+ * node statuses vstat and the DoF layout vstt / pstt, the five row-offset
+ * arrays (checked against the statuses and the adjacency); fills the
+ * columns and values, and stats[4] (interface elements, elements with four
+ * fluid-solid vertices, fluid elements, solid elements) when not NULL.
+ * Fails on a flat element, and on a fluid element on the free surface when
+ * g is NULL. */
+int nmg_assemble_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double *nodes, const int32_t *elems,
+                    const int32_t *vslot, const double *lam, const double *mu, const double *rho, const double *g,
+                    const int8_t *efluid, const int32_t *neigh, int purefluid, const double *refM, const double *refD,
+                    const double *massF, const int32_t *fmask, const int64_t *adj_ptr, const int32_t *adj,
+                    const int8_t *vstat, const int64_t *vstt, const int64_t *pstt, const int64_t *ad_ptr,
+                    int32_t *ad_col, double *ad_val, const int64_t *b_ptr, int32_t *b_col, double *b_val,
+                    const int64_t *e_ptr, int32_t *e_col, double *e_val, const int64_t *et_ptr, int32_t *et_col,
+                    double *et_val, const int64_t *ap_ptr, int32_t *ap_col, double *ap_val, int64_t *stats);
+
 /* Mixed-precision refinement of the Chebyshev solve (no reference
  * counterpart): with passes >= 1 every solve runs `passes` inner Chebyshev
  * solves of degree inner_deg on FP32 values and FP32 iterates (FP64 row

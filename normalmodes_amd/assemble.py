@@ -145,8 +145,9 @@ def assemble(mesh: TetMesh, model: RadialModel | None = None, *, rho=None, vp=No
     """Assemble A and B.  Give either a model or per-element-node rho, vp, vs.
 
     ``g0`` (n_elem, pNp, 3) is the reference gravity in km/s^2, used when job == 2.
-    ``device``: form the element matrices of a solid body on the GPU
-    (``nmg_assemble_elastic``; the pattern is built here) instead of the CPU.
+    ``device``: form the matrices on the GPU (``nmg_assemble_elastic`` for a
+    solid body, whose pattern is built here; ``nmg_assemble_fs`` for a body
+    with fluid elements) instead of the CPU.
     """
     if model is not None:
         rho, vp, vs = element_node_values(mesh, model)
@@ -157,7 +158,8 @@ def assemble(mesh: TetMesh, model: RadialModel | None = None, *, rho=None, vp=No
     if fluid.any():
         if job == 2 and g0 is None and model is not None:
             g0 = reference_gravity(mesh, model)
-        return assemble_fs(mesh, rho=rho, vp=vp, vs=vs, g0=g0 if job == 2 else None, job=job, name=name)
+        return assemble_fs(mesh, rho=rho, vp=vp, vs=vs, g0=g0 if job == 2 else None, job=job, name=name,
+                           device=device)
     if job == 2 and g0 is None and model is not None and tuple(model.axes) == (1.0, 1.0, 1.0):
         g0 = reference_gravity(mesh, model)
     lib = load_asm()
@@ -275,13 +277,15 @@ def node_status(mesh: TetMesh, fluid):
 
 
 def assemble_fs(mesh: TetMesh, *, rho, vp, vs, g0=None, job: int = 1, name: str = "",
-                purefluid: bool = False) -> FluidProblem:
+                purefluid: bool = False, device: bool = False) -> FluidProblem:
     """Fluid-solid assembly (CGFSE3D_ISO, src/mod_cg_create_matrix.f90:103-977,
     on the layout of matrixstruct_general, :1458-2033): Ad, B over the
     displacement DoFs (six at fluid-solid nodes), E, ET and Ap over the
     pressure DoFs.  ``g0`` (n_elem, pNp, 3, km/s^2) switches on the gravity
     terms (JOB=2); without it every gravity term is zero (JOB=1, where the
-    reference leaves its local gravity unset)."""
+    reference leaves its local gravity unset).  ``device``: the same
+    matrices formed on the GPU (``nmg_assemble_fs``); the layout and row
+    offsets are built here either way."""
     lam, mu = lame(rho, vp, vs)
     fluid = _fluid_elements(vs, mu)
     lib = load_asm()
@@ -319,15 +323,20 @@ def assemble_fs(mesh: TetMesh, *, rho, vp, vs, g0=None, job: int = 1, name: str 
     nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
     elems = np.ascontiguousarray(mesh.elems, dtype=np.int32)
     vslot = np.asarray(vertex_slots(mesh.porder), dtype=np.int32)
-    rc = lib.nm_assemble_fs(
-        ctypes.c_int64(nn), ctypes.c_int64(ne), ctypes.c_int(np_), ctypes.c_int(nfp), _p(nodes, _f64p),
-        _p(elems, _i32p), _p(vslot, _i32p), _p(np.ascontiguousarray(lam), _f64p),
-        _p(np.ascontiguousarray(mu), _f64p), _p(np.ascontiguousarray(rho, dtype=np.float64), _f64p),
-        _p(g, _f64p), _p(ef, i8p), _p(neigh, _i32p), ctypes.c_int(int(purefluid)),
-        _p(np.ascontiguousarray(M), _f64p), _p(np.ascontiguousarray(D), _f64p),
-        _p(np.ascontiguousarray(massF), _f64p), _p(np.ascontiguousarray(fmask), _i32p),
-        _p(ptr, _i64p), _p(adj, _i32p), _p(vst, i8p), _p(vstt, _i64p), _p(pstt, _i64p), *args,
-        _p(stats, _i64p))
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (lam, mu, rho, M, D, massF)]
+    fmask_c = np.ascontiguousarray(fmask, dtype=np.int32)
+    cargs = [ctypes.c_int64(nn), ctypes.c_int64(ne), ctypes.c_int(np_), ctypes.c_int(nfp), _p(nodes, _f64p),
+             _p(elems, _i32p), _p(vslot, _i32p), _p(keep[0], _f64p), _p(keep[1], _f64p), _p(keep[2], _f64p),
+             _p(g, _f64p), _p(ef, i8p), _p(neigh, _i32p), ctypes.c_int(int(purefluid)),
+             _p(keep[3], _f64p), _p(keep[4], _f64p), _p(keep[5], _f64p), _p(fmask_c, _i32p),
+             _p(ptr, _i64p), _p(adj, _i32p), _p(vst, i8p), _p(vstt, _i64p), _p(pstt, _i64p), *args,
+             _p(stats, _i64p)]
+    if device:
+        from . import pevsl
+        pevsl.assemble_fs(cargs)
+        rc = 0
+    else:
+        rc = lib.nm_assemble_fs(*cargs)
     if rc == -2:
         raise RuntimeError("fluid-solid assembly: an element entry has no slot in the pattern")
     if rc < 0:

@@ -3574,6 +3574,26 @@ int nmg_assemble_elastic(int64_t n_nodes, int64_t n_elem, int np, const double* 
   });
 }
 
+int nmg_assemble_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                    const int32_t* vslot, const double* lam, const double* mu, const double* rho, const double* g,
+                    const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM, const double* refD,
+                    const double* massF, const int32_t* fmask, const int64_t* adj_ptr, const int32_t* adj,
+                    const int8_t* vstat, const int64_t* vstt, const int64_t* pstt, const int64_t* ad_ptr,
+                    int32_t* ad_col, double* ad_val, const int64_t* b_ptr, int32_t* b_col, double* b_val,
+                    const int64_t* e_ptr, int32_t* e_col, double* e_val, const int64_t* et_ptr, int32_t* et_col,
+                    double* et_val, const int64_t* ap_ptr, int32_t* ap_col, double* ap_val, int64_t* stats) {
+  return guarded([&] {
+    NMG_CHECK(nodes && elems && vslot && lam && mu && rho && efluid && neigh && refM && refD && massF && fmask &&
+                  adj_ptr && adj && vstat && vstt && pstt && ad_ptr && ad_col && ad_val && b_ptr && b_col && b_val &&
+                  e_ptr && e_col && e_val && et_ptr && et_col && et_val && ap_ptr && ap_col && ap_val,
+              NMG_EINVAL, "null argument");
+    assemble_fs_device(n_nodes, n_elem, np, nfp, nodes, elems, vslot, lam, mu, rho, g, efluid, neigh, purefluid,
+                       refM, refD, massF, fmask, adj_ptr, adj, vstat, vstt, pstt, ad_ptr, ad_col, ad_val, b_ptr,
+                       b_col, b_val, e_ptr, e_col, e_val, et_ptr, et_col, et_val, ap_ptr, ap_col, ap_val, stats,
+                       default_stream());
+  });
+}
+
 int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <vector>
 
+#include "nmgpu_fs_elem.h"
 #include "nmgpu_internal.h"
 
 namespace nmg {
@@ -258,6 +259,34 @@ __global__ void gather_k(int64_t n_nodes, int np, const int32_t* __restrict__ el
   }
 }
 
+// fluid-solid body: per-element data (one thread per element); codes of
+// the first flat element and the first fluid element on a free surface
+// without gravity
+__global__ void fs_info_k(fs::Body M, fs::Info* __restrict__ info, int* __restrict__ bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M.n_elem) return;
+  const int rc = fs::element_info(M, e, &info[e]);
+  if (rc) atomicMin(&bad[rc - 1], (int)e);
+}
+
+// fluid element matrices, one thread per fluid element of the list
+__global__ void fs_fluid_k(fs::Body M, int64_t nf, const int64_t* __restrict__ flist,
+                           const int64_t* __restrict__ koff, const int64_t* __restrict__ scm, double* W,
+                           double* SC) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int64_t e = flist[i];
+  fs::fluid_element(M, e, W + koff[e], scm[e] >= 0 ? SC + scm[e] : nullptr);
+}
+
+// rows of the fluid-solid pattern, one thread per node
+__global__ void fs_gather_k(fs::Body M, fs::Gather G, unsigned long long* __restrict__ miss) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.n_nodes) return;
+  const int64_t m = fs::gather_node(M, G, v);
+  if (m) atomicAdd(miss, (unsigned long long)m);
+}
+
 template <typename T>
 struct Dev {
   T* p = nullptr;
@@ -349,6 +378,297 @@ int assemble_elastic_device(int64_t n_nodes, int64_t n_elem, int np, const doubl
   NMG_HIP(hipMemcpyAsync(b_val, dbv.p, sizeof(double) * 3 * nadj, hipMemcpyDeviceToHost, s));
   NMG_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+void assemble_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                        const int32_t* vslot, const double* lam, const double* mu, const double* rho, const double* g,
+                        const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM,
+                        const double* refD, const double* massF, const int32_t* fmask, const int64_t* adj_ptr,
+                        const int32_t* adj, const int8_t* vstat, const int64_t* vstt, const int64_t* pstt,
+                        const int64_t* ad_ptr, int32_t* ad_col, double* ad_val, const int64_t* b_ptr, int32_t* b_col,
+                        double* b_val, const int64_t* e_ptr, int32_t* e_col, double* e_val, const int64_t* et_ptr,
+                        int32_t* et_col, double* et_val, const int64_t* ap_ptr, int32_t* ap_col, double* ap_val,
+                        int64_t* stats, hipStream_t s) {
+  using fs::hasp;
+  using fs::vnum;
+  NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "pNp must be 4 (P1) or 10 (P2)");
+  NMG_CHECK(nfp == (np == 4 ? 3 : 6), NMG_EINVAL, "face nodes must be 3 (P1) or 6 (P2)");
+  NMG_CHECK(n_nodes > 0 && n_elem > 0 && n_elem < 0x3fffffff, NMG_EINVAL, "empty or oversized mesh");
+  for (int k = 0; k < 4; ++k) NMG_CHECK(vslot[k] >= 0 && vslot[k] < np, NMG_EINVAL, "bad vertex slot");
+  for (int k = 0; k < 4 * nfp; ++k) NMG_CHECK(fmask[k] >= 0 && fmask[k] < np, NMG_EINVAL, "bad face mask");
+  for (int64_t i = 0; i < n_elem * np; ++i)
+    NMG_CHECK(elems[i] >= 0 && elems[i] < n_nodes, NMG_EINVAL, "element node id out of range");
+  for (int64_t i = 0; i < n_elem * 4; ++i)
+    NMG_CHECK(neigh[i] >= -1 && neigh[i] < n_elem, NMG_EINVAL, "face neighbour out of range");
+  for (int64_t e = 0; e < n_elem; ++e) NMG_CHECK(efluid[e] == 0 || efluid[e] == 1, NMG_EINVAL, "bad fluid flag");
+  NMG_CHECK(adj_ptr[0] == 0, NMG_EINVAL, "adjacency offsets must start at 0");
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    NMG_CHECK(adj_ptr[v + 1] >= adj_ptr[v], NMG_EINVAL, "adjacency offsets decrease");
+    NMG_CHECK(vstat[v] >= 0 && vstat[v] <= 5, NMG_EINVAL, "node status out of range");
+  }
+  const int64_t nadj = adj_ptr[n_nodes];
+  for (int64_t k = 0; k < nadj; ++k) NMG_CHECK(adj[k] >= 0 && adj[k] < n_nodes, NMG_EINVAL, "adjacency out of range");
+  // the DoF layout and the row offsets must be those of the statuses and the
+  // adjacency: the gather writes each row's columns from them
+  int64_t n_dof = 0, n_p = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    NMG_CHECK(vstt[v] == n_dof && pstt[v] == (hasp(vstat[v]) ? n_p : -1), NMG_EINVAL,
+              "vstt / pstt do not match the node statuses");
+    n_dof += vnum(vstat[v]);
+    if (hasp(vstat[v])) n_p++;
+  }
+  {
+    std::vector<int64_t> ad(n_dof + 1, 0), bb(n_dof + 1, 0), ee(n_dof + 1, 0), et(n_p + 1, 0), ap(n_p + 1, 0);
+    for (int64_t v = 0; v < n_nodes; ++v) {
+      const int64_t a0 = adj_ptr[v], deg = adj_ptr[v + 1] - a0;
+      const int8_t sv = vstat[v];
+      int64_t nS = 0, nF = 0, nFS = 0;
+      for (int64_t k = 0; k < deg; ++k) {
+        const int8_t sw = vstat[adj[a0 + k]];
+        if (!hasp(sw) || vnum(sw) == 6) nS++;
+        if (hasp(sw)) nF++;
+        if (vnum(sw) == 6) nFS++;
+      }
+      auto set3 = [&](int64_t row, int64_t na, int64_t nb, int64_t ne) {
+        for (int j = 0; j < 3; ++j) {
+          ad[row + j + 1] = ad[row + j] + na;
+          bb[row + j + 1] = bb[row + j] + nb;
+          ee[row + j + 1] = ee[row + j] + ne;
+        }
+      };
+      const int64_t r = vstt[v];
+      if (hasp(sv)) {
+        const int64_t p = pstt[v];
+        if (vnum(sv) == 3) {
+          NMG_CHECK(nF == deg, NMG_EINVAL, "a pure fluid node has a neighbour without a pressure");
+          ap[p + 1] = ap[p] + deg;
+          et[p + 1] = et[p] + 3 * deg;
+          set3(r, 3 * deg, deg, deg);
+        } else {
+          set3(r, 3 * nS, nS, nFS);
+          ap[p + 1] = ap[p] + nF;
+          et[p + 1] = et[p] + 3 * (nF + nFS);
+          set3(r + 3, 3 * nF, nF, nF);
+        }
+      } else {
+        NMG_CHECK(vnum(sv) == 3, NMG_EINVAL, "a six-DoF node without a pressure");
+        set3(r, 3 * deg, deg, 0);
+      }
+    }
+    NMG_CHECK(std::equal(ad.begin(), ad.end(), ad_ptr) && std::equal(bb.begin(), bb.end(), b_ptr) &&
+                  std::equal(ee.begin(), ee.end(), e_ptr) && std::equal(et.begin(), et.end(), et_ptr) &&
+                  std::equal(ap.begin(), ap.end(), ap_ptr),
+              NMG_EINVAL, "row offsets do not match the fluid-solid pattern of the adjacency");
+  }
+  const int L = 3 * np, LF = L + np, N2 = np * np, Ls = 3 * nfp;
+  // element lists, pool offsets, interface blocks, statistics
+  std::vector<int64_t> koff(n_elem), scm(n_elem, -1), flist, slist;
+  int64_t wpool = 0, spool = 0;
+  int64_t st[4] = {0, 0, 0, 0};
+  for (int64_t e = 0; e < n_elem; ++e) {
+    if (!efluid[e]) {
+      koff[e] = (int64_t)slist.size() * L * L;
+      slist.push_back(e);
+      st[3]++;
+      continue;
+    }
+    koff[e] = wpool;
+    wpool += (int64_t)LF * LF;
+    flist.push_back(e);
+    st[2]++;
+    int c = 0;
+    for (int k = 0; k < 4; ++k) c += vstat[elems[e * np + vslot[k]]] == 2;
+    if (c == 3) {
+      st[0]++;
+      if (g) {
+        scm[e] = spool;
+        spool += (int64_t)Ls * Ls;
+      }
+    }
+    if (c == 4) st[1]++;
+  }
+  if (stats)
+    for (int k = 0; k < 4; ++k) stats[k] = st[k];
+  const int64_t ns = (int64_t)slist.size(), nf = (int64_t)flist.size();
+  // node -> (element, slot) in ascending element order
+  std::vector<int64_t> n2e_ptr(n_nodes + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e)
+    for (int a = 0; a < np; ++a) n2e_ptr[elems[e * np + a] + 1]++;
+  for (int64_t v = 0; v < n_nodes; ++v) n2e_ptr[v + 1] += n2e_ptr[v];
+  std::vector<int64_t> n2e(n2e_ptr[n_nodes]);
+  std::vector<int32_t> n2s(n2e_ptr[n_nodes]);
+  {
+    std::vector<int64_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
+    for (int64_t e = 0; e < n_elem; ++e)
+      for (int a = 0; a < np; ++a) {
+        const int64_t v = elems[e * np + a];
+        n2e[fill[v]] = e;
+        n2s[fill[v]] = a;
+        fill[v]++;
+      }
+  }
+  // solid elements' inputs, compacted for the solid element kernel
+  std::vector<int32_t> se((size_t)ns * np);
+  std::vector<double> sl((size_t)ns * np), sm((size_t)ns * np), sr((size_t)ns * np), sg(g ? (size_t)ns * np * 3 : 0);
+  for (int64_t i = 0; i < ns; ++i) {
+    const int64_t e = slist[i];
+    for (int a = 0; a < np; ++a) {
+      se[i * np + a] = elems[e * np + a];
+      sl[i * np + a] = lam[e * np + a];
+      sm[i * np + a] = mu[e * np + a];
+      sr[i * np + a] = rho[e * np + a];
+      if (g)
+        for (int c = 0; c < 3; ++c) sg[(i * np + a) * 3 + c] = g[(e * np + a) * 3 + c];
+    }
+  }
+  const int64_t nnz_ad = ad_ptr[n_dof], nnz_b = b_ptr[n_dof], nnz_e = e_ptr[n_dof], nnz_et = et_ptr[n_p],
+                nnz_ap = ap_ptr[n_p];
+  Dev<double> dn(n_nodes * 3), dl(n_elem * np), dr(n_elem * np), dM(N2), dD(3 * N2), dMF(4 * nfp * nfp);
+  Dev<double> dg(g ? n_elem * np * 3 : 0);
+  Dev<int32_t> de(n_elem * np), dvs(4), dnb(n_elem * 4), dfm(4 * nfp), dadj(nadj), dn2s(n2s.size());
+  Dev<int8_t> def(n_elem), dst(n_nodes);
+  Dev<int64_t> dap(n_nodes + 1), dvt(n_nodes), dpt(n_nodes), dn2ep(n_nodes + 1), dn2e(n2e.size());
+  Dev<int64_t> dko(n_elem), dsc(n_elem), dfl(nf);
+  Dev<int64_t> rp_ad(n_dof + 1), rp_b(n_dof + 1), rp_e(n_dof + 1), rp_et(n_p + 1), rp_ap(n_p + 1);
+  Dev<int32_t> c_ad(nnz_ad), c_b(nnz_b), c_e(nnz_e), c_et(nnz_et), c_ap(nnz_ap);
+  Dev<double> v_ad(nnz_ad), v_b(nnz_b), v_e(nnz_e), v_et(nnz_et), v_ap(nnz_ap);
+  Dev<fs::Info> dinfo(n_elem);
+  Dev<double> dW(wpool), dSC(spool), dKs((size_t)ns * L * L), dMs((size_t)ns * N2);
+  Dev<int32_t> dse(ns * np);
+  Dev<double> dsl(ns * np), dsm(ns * np), dsr(ns * np), dsg(g ? ns * np * 3 : 0);
+  Dev<int> dbad(3);
+  Dev<unsigned long long> dmiss(1);
+  dn.up(nodes, n_nodes * 3, s);
+  dl.up(lam, n_elem * np, s);
+  dr.up(rho, n_elem * np, s);
+  dM.up(refM, N2, s);
+  dD.up(refD, 3 * N2, s);
+  dMF.up(massF, 4 * nfp * nfp, s);
+  if (g) dg.up(g, n_elem * np * 3, s);
+  de.up(elems, n_elem * np, s);
+  dvs.up(vslot, 4, s);
+  dnb.up(neigh, n_elem * 4, s);
+  dfm.up(fmask, 4 * nfp, s);
+  dadj.up(adj, nadj, s);
+  dn2s.up(n2s.data(), n2s.size(), s);
+  def.up(efluid, n_elem, s);
+  dst.up(vstat, n_nodes, s);
+  dap.up(adj_ptr, n_nodes + 1, s);
+  dvt.up(vstt, n_nodes, s);
+  dpt.up(pstt, n_nodes, s);
+  dn2ep.up(n2e_ptr.data(), n_nodes + 1, s);
+  dn2e.up(n2e.data(), n2e.size(), s);
+  dko.up(koff.data(), n_elem, s);
+  dsc.up(scm.data(), n_elem, s);
+  dfl.up(flist.data(), nf, s);
+  rp_ad.up(ad_ptr, n_dof + 1, s);
+  rp_b.up(b_ptr, n_dof + 1, s);
+  rp_e.up(e_ptr, n_dof + 1, s);
+  rp_et.up(et_ptr, n_p + 1, s);
+  rp_ap.up(ap_ptr, n_p + 1, s);
+  dse.up(se.data(), se.size(), s);
+  dsl.up(sl.data(), sl.size(), s);
+  dsm.up(sm.data(), sm.size(), s);
+  dsr.up(sr.data(), sr.size(), s);
+  if (g) dsg.up(sg.data(), sg.size(), s);
+  const int big[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  dbad.up(big, 3, s);
+  const unsigned long long zero = 0;
+  dmiss.up(&zero, 1, s);
+
+  fs::Body B;
+  B.n_nodes = n_nodes;
+  B.n_elem = n_elem;
+  B.np = np;
+  B.nfp = nfp;
+  B.purefluid = purefluid ? 1 : 0;
+  B.nodes = dn.p;
+  B.elems = de.p;
+  B.vslot = dvs.p;
+  B.lam = dl.p;
+  B.rho = dr.p;
+  B.g = g ? dg.p : nullptr;
+  B.efluid = def.p;
+  B.neigh = dnb.p;
+  B.vstat = dst.p;
+  B.refM = dM.p;
+  B.refD = dD.p;
+  B.massF = dMF.p;
+  B.fmask = dfm.p;
+  hipLaunchKernelGGL(fs_info_k, dim3((unsigned)((n_elem + 255) / 256)), dim3(256), 0, s, B, dinfo.p, dbad.p);
+  NMG_HIP(hipGetLastError());
+  if (ns > 0) {
+    // the solid elements' stiffness (CGE3D_ISO with gravity), the kernel of
+    // the solid producer on the compacted element list
+    if (np == 10)
+      hipLaunchKernelGGL(elem_k<10>, dim3((unsigned)ns), dim3(64), 0, s, ns, dn.p, dse.p, dvs.p, dsl.p, dsm.p, dsr.p,
+                         g ? dsg.p : nullptr, dM.p, dD.p, dKs.p, dMs.p, dbad.p + 2);
+    else
+      hipLaunchKernelGGL(elem_k<4>, dim3((unsigned)ns), dim3(64), 0, s, ns, dn.p, dse.p, dvs.p, dsl.p, dsm.p, dsr.p,
+                         g ? dsg.p : nullptr, dM.p, dD.p, dKs.p, dMs.p, dbad.p + 2);
+    NMG_HIP(hipGetLastError());
+  }
+  int badh[3];
+  NMG_HIP(hipMemcpyAsync(badh, dbad.p, sizeof(badh), hipMemcpyDeviceToHost, s));
+  NMG_HIP(hipStreamSynchronize(s));
+  NMG_CHECK(badh[0] == big[0], NMG_EINVAL, "flat element " + std::to_string(badh[0]));
+  NMG_CHECK(badh[2] == big[0], NMG_EINVAL, "flat element " + std::to_string(slist[badh[2] < ns ? badh[2] : 0]));
+  NMG_CHECK(badh[1] == big[0], NMG_EINVAL,
+            "fluid element " + std::to_string(badh[1]) +
+                " lies on the free surface: its surface term divides by the mean |g.n|, which needs gravity (JOB=2)");
+  if (nf > 0) {
+    hipLaunchKernelGGL(fs_fluid_k, dim3((unsigned)((nf + 63) / 64)), dim3(64), 0, s, B, nf, dfl.p, dko.p, dsc.p,
+                       dW.p, dSC.p);
+    NMG_HIP(hipGetLastError());
+  }
+  fs::Gather G;
+  G.adj_ptr = dap.p;
+  G.adj = dadj.p;
+  G.vstt = dvt.p;
+  G.pstt = dpt.p;
+  G.n2e_ptr = dn2ep.p;
+  G.n2e = dn2e.p;
+  G.n2s = dn2s.p;
+  G.info = dinfo.p;
+  G.koff = dko.p;
+  G.scm = dsc.p;
+  G.Ks = dKs.p;
+  G.W = dW.p;
+  G.SC = dSC.p;
+  G.ad_ptr = rp_ad.p;
+  G.b_ptr = rp_b.p;
+  G.e_ptr = rp_e.p;
+  G.et_ptr = rp_et.p;
+  G.ap_ptr = rp_ap.p;
+  G.ad_col = c_ad.p;
+  G.b_col = c_b.p;
+  G.e_col = c_e.p;
+  G.et_col = c_et.p;
+  G.ap_col = c_ap.p;
+  G.ad_val = v_ad.p;
+  G.b_val = v_b.p;
+  G.e_val = v_e.p;
+  G.et_val = v_et.p;
+  G.ap_val = v_ap.p;
+  hipLaunchKernelGGL(fs_gather_k, dim3((unsigned)((n_nodes + 127) / 128)), dim3(128), 0, s, B, G, dmiss.p);
+  NMG_HIP(hipGetLastError());
+  unsigned long long missh = 0;
+  NMG_HIP(hipMemcpyAsync(&missh, dmiss.p, sizeof(missh), hipMemcpyDeviceToHost, s));
+  auto down = [&](auto* h, auto* d, int64_t n) {
+    if (n) NMG_HIP(hipMemcpyAsync(h, d, n * sizeof(*h), hipMemcpyDeviceToHost, s));
+  };
+  down(ad_col, c_ad.p, nnz_ad);
+  down(ad_val, v_ad.p, nnz_ad);
+  down(b_col, c_b.p, nnz_b);
+  down(b_val, v_b.p, nnz_b);
+  down(e_col, c_e.p, nnz_e);
+  down(e_val, v_e.p, nnz_e);
+  down(et_col, c_et.p, nnz_et);
+  down(et_val, v_et.p, nnz_et);
+  down(ap_col, c_ap.p, nnz_ap);
+  down(ap_val, v_ap.p, nnz_ap);
+  NMG_HIP(hipStreamSynchronize(s));
+  NMG_CHECK(missh == 0, NMG_EINVAL, "fluid-solid assembly: an element entry has no slot in the pattern");
 }
 
 }  // namespace nmg

@@ -231,6 +231,15 @@ int assemble_elastic_device(int64_t n_nodes, int64_t n_elem, int np, const doubl
                             const double* g, const double* refM, const double* refD, const int64_t* adj_ptr,
                             const int32_t* adj, const int64_t* a_rowptr, double* a_val, const int64_t* b_rowptr,
                             double* b_val, hipStream_t s);
+void assemble_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                        const int32_t* vslot, const double* lam, const double* mu, const double* rho, const double* g,
+                        const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM,
+                        const double* refD, const double* massF, const int32_t* fmask, const int64_t* adj_ptr,
+                        const int32_t* adj, const int8_t* vstat, const int64_t* vstt, const int64_t* pstt,
+                        const int64_t* ad_ptr, int32_t* ad_col, double* ad_val, const int64_t* b_ptr, int32_t* b_col,
+                        double* b_val, const int64_t* e_ptr, int32_t* e_col, double* e_val, const int64_t* et_ptr,
+                        int32_t* et_col, double* et_val, const int64_t* ap_ptr, int32_t* ap_col, double* ap_val,
+                        int64_t* stats, hipStream_t s);
 void k_add_f32(int64_t n, const float* x, double* y, hipStream_t s);     // y += x
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32 = false);

@@ -23,7 +23,7 @@ __all__ = ["PevslError", "Handle", "pevsl_start", "pevsl_finish", "pevsl_setprob
            "residuals", "solve_stats", "set_lanczos_controls", "filter_apply", "apply",
            "bench_spmv", "bench_chebstep", "set_kernel_probe", "kernel_probe", "set_block_size", "apply_block", "set_residual_target", "bench_block_chebstep", "set_kernel_map", "set_tiling", "set_xcd_order", "set_stream_loads", "set_value_order", "csr_autotune", "kpmdos", "dos_count", "spslicer",
            "solve_slices", "slice_counts", "set_split_streams", "chebiter_set_mode", "chebiter_info",
-           "bench_bsolve", "bench_filter", "chebiter_set_refine", "assemble_elastic", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
+           "bench_bsolve", "bench_filter", "chebiter_set_refine", "assemble_elastic", "assemble_fs", "bench_copy", "gemm_tn", "FP64", "FP32_FILTER", "FP32_BSOLVE", "FP32_FILTER_A", "LAYOUTS"]
 
 FP64 = 0
 FP32_FILTER = 1
@@ -405,6 +405,13 @@ def assemble_elastic(nodes, elems, vslot, lam, mu, rho, g, refM, refD, adj_ptr, 
         _p(lam, _f64p), _p(mu, _f64p), _p(rho, _f64p), None if g is None else _p(g, _f64p), _p(refM, _f64p),
         _p(refD, _f64p), _p(adj_ptr, _i64p), _p(adj, _i32p), _p(a_rowptr, _i64p), _p(a_val, _f64p),
         _p(b_rowptr, _i64p), _p(b_val, _f64p)))
+
+
+def assemble_fs(args):
+    """Fluid-solid Ad, B, E, ET, Ap on the GPU (nmg_assemble_fs): ``args`` is
+    the ctypes argument list of the CPU producer nm_assemble_fs, whose
+    column and value arrays are filled in place."""
+    _check(_lib().nmg_assemble_fs(*args))
 
 
 def chebiter_set_refine(cheb: Handle, passes: int, inner_deg: int = 0):

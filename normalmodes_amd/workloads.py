@@ -35,18 +35,20 @@ SPECS = {
 }
 
 
-def build_problem(name: str, porder: int = 2, seed: int = MESH_SEED):
+def build_problem(name: str, porder: int = 2, seed: int = MESH_SEED, device: bool = False):
+    """Mesh, model and assembled matrices of a workload; ``device`` forms
+    the matrices with the GPU producer instead of the CPU one."""
     spec = SPECS[name]
     if spec["model"] == "three_layer_fluid":
         model = three_layer(6371.0)
         mesh = stuffed_ball_mesh(spec["n_tets"], porder=porder, seed=seed, interfaces=model.interfaces,
                                  scale=model.radius)
-        return assemble(mesh, model, job=spec.get("job", 1), name=name)
+        return assemble(mesh, model, job=spec.get("job", 1), name=name, device=device)
     if spec["model"] == "three_layer_solid":
         mesh = stuffed_ball_mesh(spec["n_tets"], porder=porder, seed=seed)
-        return assemble(mesh, three_layer_solid(), name=name)
+        return assemble(mesh, three_layer_solid(), name=name, device=device)
     mesh = ball_mesh(spec["n_tets"], porder=porder, seed=seed)
-    return assemble(mesh, homogeneous(), name=name)
+    return assemble(mesh, homogeneous(), name=name, device=device)
 
 # Eigenvalue intervals (lambda = omega^2 in the model's units).
 #   c1: a = lambda_7 / 2 (above the six rigid-body zero modes) and b at the

@@ -190,6 +190,26 @@ int nmg_assemble_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const doub
                     const int64_t *e_ptr, int32_t *e_col, double *e_val, const int64_t *et_ptr, int32_t *et_col,
                     double *et_val, const int64_t *ap_ptr, int32_t *ap_col, double *ap_val, int64_t *stats);
 
+/* Adjoint of a fluid-solid body (the fluid-solid counterpart of
+ * nmg_backward; the element terms are CGFSE3D_ISO's,
+ * src/mod_cg_create_matrix.f90:103-977): for k eigenpairs (eig[j],
+ * U[j*n_dof ...]) of (Ad - E Ap^-1 ET) u = lambda B u, u physical and
+ * B-normalised, and P[j*n_p ...] = -Ap^-1 ET u, writes
+ * d lambda_j / d m_e = w^T (dK_e/dm - lambda_j dM_e/dm) w, w = (u, p), for m
+ * in {rho, lam (Lame), mu} changed uniformly at the nodes of element e, into
+ * d_rho, d_lam, d_mu [j*n_elem + e].  Fluid elements: the derivatives of the
+ * fluid element matrix and the interface gravity block by forward-mode dual
+ * numbers (one thread per element and parameter), d_mu = 0; solid elements:
+ * the element stiffness at unit parameters (gravity linear in rho).  The
+ * body is described as for nmg_assemble_fs (mu is not needed); n_dof and n_p
+ * follow from vstat. */
+int nmg_backward_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double *nodes, const int32_t *elems,
+                    const int32_t *vslot, const double *lam, const double *rho, const double *g,
+                    const int8_t *efluid, const int32_t *neigh, int purefluid, const double *refM, const double *refD,
+                    const double *massF, const int32_t *fmask, const int8_t *vstat, const int64_t *vstt,
+                    const int64_t *pstt, int k, const double *eig, const double *U, const double *P, double *d_rho,
+                    double *d_lam, double *d_mu);
+
 /* Mixed-precision refinement of the Chebyshev solve (no reference
  * counterpart): with passes >= 1 every solve runs `passes` inner Chebyshev
  * solves of degree inner_deg on FP32 values and FP32 iterates (FP64 row

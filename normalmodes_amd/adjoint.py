@@ -23,7 +23,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
-__all__ = ["element_geometry", "element_gravity", "backward", "element_matrices"]
+__all__ = ["element_geometry", "element_gravity", "backward", "backward_fs", "element_matrices"]
 
 
 def element_geometry(mesh):
@@ -119,3 +119,54 @@ def element_matrices(mesh, sel, lam, mu, rho, g=None):
     if rc != 0:
         raise RuntimeError(f"flat element {-rc - 1}")
     return K, Me
+
+
+def backward_fs(Q, lam, X, purefluid: bool = False):
+    """Sensitivities of eigenpairs of a fluid-solid body (nmg_backward_fs).
+
+    ``Q``: the FluidProblem the pairs belong to; ``lam`` (k,) eigenvalues of
+    (Ad - E Ap^-1 ET) u = lambda B u; ``X`` (n, k) the physical displacement
+    vectors (B-normalised here).  The pressures p = -Ap^-1 ET u come from a
+    sparse factorisation of Ap on the host.  Returns (d_rho, d_lam, d_mu),
+    each (k, n_elem), for rho, lam (Lame) and mu changed uniformly at the
+    nodes of each element; d_mu is 0 on fluid elements."""
+    import scipy.sparse.linalg as spl
+    from .assemble import face_neighbours
+    from .refelem import reference_faces
+    L = load_gpu()
+    me = Q.mesh
+    lam = np.ascontiguousarray(np.atleast_1d(lam), dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64).reshape(Q.n, -1)
+    k = X.shape[1]
+    if lam.size != k:
+        raise ValueError("one eigenvalue per vector")
+    B = Q.B.to_scipy()
+    X = X / np.sqrt(np.einsum("ij,ij->j", X, B @ X))
+    Pm = -spl.splu(Q.Ap.to_scipy().tocsc()).solve(np.asarray(Q.ET.to_scipy() @ X))
+    U = np.ascontiguousarray(X.T)
+    P = np.ascontiguousarray(Pm.T)
+    npl = me.elems.shape[1]
+    _, M, D = reference_element(me.porder)
+    fmask, massF = reference_faces(me.porder)
+    i8p = ctypes.POINTER(ctypes.c_int8)
+    arr = dict(nodes=np.ascontiguousarray(me.nodes, dtype=np.float64),
+               elems=np.ascontiguousarray(me.elems, dtype=np.int32),
+               vslot=np.asarray(vertex_slots(me.porder), dtype=np.int32),
+               lamL=np.ascontiguousarray(Q.lam, dtype=np.float64), rho=np.ascontiguousarray(Q.rho, dtype=np.float64),
+               ef=np.ascontiguousarray(Q.fluid.astype(np.int8)), neigh=face_neighbours(me),
+               M=np.ascontiguousarray(M), D=np.ascontiguousarray(D), massF=np.ascontiguousarray(massF),
+               fmask=np.ascontiguousarray(fmask, dtype=np.int32), vst=np.ascontiguousarray(Q.vstat, dtype=np.int8),
+               vstt=np.ascontiguousarray(Q.vstt, dtype=np.int64), pstt=np.ascontiguousarray(Q.pstt, dtype=np.int64))
+    g = None if Q.g0 is None else np.ascontiguousarray(Q.g0, dtype=np.float64)
+    out = [np.empty((k, me.n_elem)) for _ in range(3)]
+    pv._check(L.nmg_backward_fs(
+        ctypes.c_int64(me.n_nodes), ctypes.c_int64(me.n_elem), ctypes.c_int(npl), ctypes.c_int(fmask.shape[1]),
+        arr["nodes"].ctypes.data_as(_f64p), arr["elems"].ctypes.data_as(_i32p), arr["vslot"].ctypes.data_as(_i32p),
+        arr["lamL"].ctypes.data_as(_f64p), arr["rho"].ctypes.data_as(_f64p),
+        None if g is None else g.ctypes.data_as(_f64p), arr["ef"].ctypes.data_as(i8p),
+        arr["neigh"].ctypes.data_as(_i32p), ctypes.c_int(int(purefluid)), arr["M"].ctypes.data_as(_f64p),
+        arr["D"].ctypes.data_as(_f64p), arr["massF"].ctypes.data_as(_f64p), arr["fmask"].ctypes.data_as(_i32p),
+        arr["vst"].ctypes.data_as(i8p), arr["vstt"].ctypes.data_as(_i64p), arr["pstt"].ctypes.data_as(_i64p),
+        ctypes.c_int(k), lam.ctypes.data_as(_f64p), U.ctypes.data_as(_f64p), P.ctypes.data_as(_f64p),
+        *(o.ctypes.data_as(_f64p) for o in out)))
+    return tuple(out)

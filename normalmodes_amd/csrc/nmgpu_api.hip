@@ -3594,6 +3594,21 @@ int nmg_assemble_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const doub
   });
 }
 
+int nmg_backward_fs(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                    const int32_t* vslot, const double* lam, const double* rho, const double* g,
+                    const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM, const double* refD,
+                    const double* massF, const int32_t* fmask, const int8_t* vstat, const int64_t* vstt,
+                    const int64_t* pstt, int k, const double* eig, const double* U, const double* P, double* d_rho,
+                    double* d_lam, double* d_mu) {
+  return guarded([&] {
+    NMG_CHECK(nodes && elems && vslot && lam && rho && efluid && neigh && refM && refD && massF && fmask && vstat &&
+                  vstt && pstt && eig && U && P && d_rho && d_lam && d_mu,
+              NMG_EINVAL, "null argument");
+    backward_fs_device(n_nodes, n_elem, np, nfp, nodes, elems, vslot, lam, rho, g, efluid, neigh, purefluid, refM,
+                       refD, massF, fmask, vstat, vstt, pstt, k, eig, U, P, d_rho, d_lam, d_mu, default_stream());
+  });
+}
+
 int nmg_chebiter_set_refine(nmg_handle cheb, int passes, int inner_deg) {
   return guarded([&] {
     Cheb* C = get<Cheb>(cheb, K_CHEB);

@@ -287,6 +287,31 @@ __global__ void fs_gather_k(fs::Body M, fs::Gather G, unsigned long long* __rest
   if (m) atomicAdd(miss, (unsigned long long)m);
 }
 
+// derivative fluid element matrices, one thread per (fluid element, parameter):
+// parameter 0 seeds drho = 1 at the element's nodes, parameter 1 dlam = 1
+__global__ void fs_dfluid_k(fs::Body M, int64_t nf, const int64_t* __restrict__ flist,
+                            const int64_t* __restrict__ koff, const int64_t* __restrict__ scm, fs::Dual* Fr,
+                            fs::Dual* Fl, fs::Dual* Sr, fs::Dual* Sl) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nf) return;
+  const int64_t e = flist[t >> 1];
+  const int q = (int)(t & 1);
+  fs::Dual lD[fs::kMaxNp], rD[fs::kMaxNp];
+  for (int a = 0; a < M.np; ++a) {
+    lD[a] = {M.lam[e * M.np + a], q == 1 ? 1.0 : 0.0};
+    rD[a] = {M.rho[e * M.np + a], q == 0 ? 1.0 : 0.0};
+  }
+  fs::fluid_element_t<fs::Dual>(M, e, lD, rD, (q == 0 ? Fr : Fl) + koff[e],
+                                scm[e] >= 0 ? (q == 0 ? Sr : Sl) + scm[e] : nullptr);
+}
+
+// eigenvalue sensitivities, one thread per (element, pair)
+__global__ void fs_adjoint_k(fs::Body M, fs::Adj A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.n_elem * A.k) return;
+  fs::element_adjoint(M, A, t % M.n_elem, (int)(t / M.n_elem));
+}
+
 template <typename T>
 struct Dev {
   T* p = nullptr;
@@ -669,6 +694,186 @@ void assemble_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const 
   down(ap_val, v_ap.p, nnz_ap);
   NMG_HIP(hipStreamSynchronize(s));
   NMG_CHECK(missh == 0, NMG_EINVAL, "fluid-solid assembly: an element entry has no slot in the pattern");
+}
+
+void backward_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                        const int32_t* vslot, const double* lam, const double* rho, const double* g,
+                        const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM,
+                        const double* refD, const double* massF, const int32_t* fmask, const int8_t* vstat,
+                        const int64_t* vstt, const int64_t* pstt, int k, const double* eig, const double* U,
+                        const double* P, double* d_rho, double* d_lam, double* d_mu, hipStream_t s) {
+  using fs::hasp;
+  using fs::vnum;
+  NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "pNp must be 4 (P1) or 10 (P2)");
+  NMG_CHECK(nfp == (np == 4 ? 3 : 6), NMG_EINVAL, "face nodes must be 3 (P1) or 6 (P2)");
+  NMG_CHECK(n_nodes > 0 && n_elem > 0 && n_elem < 0x3fffffff, NMG_EINVAL, "empty or oversized mesh");
+  NMG_CHECK(k >= 1, NMG_EINVAL, "no eigenpairs");
+  for (int q = 0; q < 4; ++q) NMG_CHECK(vslot[q] >= 0 && vslot[q] < np, NMG_EINVAL, "bad vertex slot");
+  for (int q = 0; q < 4 * nfp; ++q) NMG_CHECK(fmask[q] >= 0 && fmask[q] < np, NMG_EINVAL, "bad face mask");
+  for (int64_t i = 0; i < n_elem * np; ++i)
+    NMG_CHECK(elems[i] >= 0 && elems[i] < n_nodes, NMG_EINVAL, "element node id out of range");
+  for (int64_t i = 0; i < n_elem * 4; ++i)
+    NMG_CHECK(neigh[i] >= -1 && neigh[i] < n_elem, NMG_EINVAL, "face neighbour out of range");
+  for (int64_t e = 0; e < n_elem; ++e) NMG_CHECK(efluid[e] == 0 || efluid[e] == 1, NMG_EINVAL, "bad fluid flag");
+  int64_t n_dof = 0, n_p = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    NMG_CHECK(vstat[v] >= 0 && vstat[v] <= 5, NMG_EINVAL, "node status out of range");
+    NMG_CHECK(vstt[v] == n_dof && pstt[v] == (hasp(vstat[v]) ? n_p : -1), NMG_EINVAL,
+              "vstt / pstt do not match the node statuses");
+    n_dof += vnum(vstat[v]);
+    if (hasp(vstat[v])) n_p++;
+  }
+  // a fluid element's nodes all carry a pressure (read by the adjoint)
+  for (int64_t e = 0; e < n_elem; ++e)
+    if (efluid[e])
+      for (int a = 0; a < np; ++a)
+        NMG_CHECK(hasp(vstat[elems[e * np + a]]), NMG_EINVAL, "a fluid element has a node without a pressure");
+  const int L = 3 * np, LF = L + np, N2 = np * np, Ls = 3 * nfp;
+  std::vector<int64_t> koff(n_elem), scm(n_elem, -1), flist, slist;
+  int64_t wpool = 0, spool = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    if (!efluid[e]) {
+      koff[e] = (int64_t)slist.size() * L * L;
+      slist.push_back(e);
+      continue;
+    }
+    koff[e] = wpool;
+    wpool += (int64_t)LF * LF;
+    flist.push_back(e);
+    int c = 0;
+    for (int q = 0; q < 4; ++q) c += vstat[elems[e * np + vslot[q]]] == 2;
+    if (c == 3 && g) {
+      scm[e] = spool;
+      spool += (int64_t)Ls * Ls;
+    }
+  }
+  const int64_t ns = (int64_t)slist.size(), nf = (int64_t)flist.size();
+  std::vector<int32_t> se((size_t)ns * np);
+  std::vector<double> sg(g ? (size_t)ns * np * 3 : 0), one((size_t)ns * np, 1.0), zero((size_t)ns * np, 0.0);
+  for (int64_t i = 0; i < ns; ++i) {
+    const int64_t e = slist[i];
+    for (int a = 0; a < np; ++a) {
+      se[i * np + a] = elems[e * np + a];
+      if (g)
+        for (int c = 0; c < 3; ++c) sg[(i * np + a) * 3 + c] = g[(e * np + a) * 3 + c];
+    }
+  }
+  Dev<double> dn(n_nodes * 3), dl(n_elem * np), dr(n_elem * np), dM(N2), dD(3 * N2), dMF(4 * nfp * nfp);
+  Dev<double> dg(g ? n_elem * np * 3 : 0);
+  Dev<int32_t> de(n_elem * np), dvs(4), dnb(n_elem * 4), dfm(4 * nfp), dse(ns * np);
+  Dev<int8_t> def(n_elem), dst(n_nodes);
+  Dev<int64_t> dvt(n_nodes), dpt(n_nodes), dko(n_elem), dsc(n_elem), dfl(nf);
+  Dev<fs::Info> dinfo(n_elem);
+  Dev<fs::Dual> dFr(wpool), dFl(wpool), dSr(spool), dSl(spool);
+  Dev<double> dKl((size_t)ns * L * L), dKm((size_t)ns * L * L), dKr((size_t)ns * L * L), dMs((size_t)ns * N2);
+  Dev<double> done(ns * np), dzero(ns * np), dsg(g ? ns * np * 3 : 0);
+  Dev<double> deig(k), dU((size_t)k * n_dof), dP((size_t)k * n_p), o_r((size_t)k * n_elem), o_l((size_t)k * n_elem),
+      o_m((size_t)k * n_elem);
+  Dev<int> dbad(3);
+  dn.up(nodes, n_nodes * 3, s);
+  dl.up(lam, n_elem * np, s);
+  dr.up(rho, n_elem * np, s);
+  dM.up(refM, N2, s);
+  dD.up(refD, 3 * N2, s);
+  dMF.up(massF, 4 * nfp * nfp, s);
+  if (g) dg.up(g, n_elem * np * 3, s);
+  de.up(elems, n_elem * np, s);
+  dvs.up(vslot, 4, s);
+  dnb.up(neigh, n_elem * 4, s);
+  dfm.up(fmask, 4 * nfp, s);
+  dse.up(se.data(), se.size(), s);
+  def.up(efluid, n_elem, s);
+  dst.up(vstat, n_nodes, s);
+  dvt.up(vstt, n_nodes, s);
+  dpt.up(pstt, n_nodes, s);
+  dko.up(koff.data(), n_elem, s);
+  dsc.up(scm.data(), n_elem, s);
+  dfl.up(flist.data(), nf, s);
+  done.up(one.data(), one.size(), s);
+  dzero.up(zero.data(), zero.size(), s);
+  if (g) dsg.up(sg.data(), sg.size(), s);
+  deig.up(eig, k, s);
+  dU.up(U, (size_t)k * n_dof, s);
+  dP.up(P, (size_t)k * n_p, s);
+  const int big[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  dbad.up(big, 3, s);
+
+  fs::Body B;
+  B.n_nodes = n_nodes;
+  B.n_elem = n_elem;
+  B.np = np;
+  B.nfp = nfp;
+  B.purefluid = purefluid ? 1 : 0;
+  B.nodes = dn.p;
+  B.elems = de.p;
+  B.vslot = dvs.p;
+  B.lam = dl.p;
+  B.rho = dr.p;
+  B.g = g ? dg.p : nullptr;
+  B.efluid = def.p;
+  B.neigh = dnb.p;
+  B.vstat = dst.p;
+  B.refM = dM.p;
+  B.refD = dD.p;
+  B.massF = dMF.p;
+  B.fmask = dfm.p;
+  hipLaunchKernelGGL(fs_info_k, dim3((unsigned)((n_elem + 255) / 256)), dim3(256), 0, s, B, dinfo.p, dbad.p);
+  NMG_HIP(hipGetLastError());
+  if (ns > 0) {
+    // the solid elements' stiffness at unit lam, mu and rho: its derivatives
+    const double* unit[3][3] = {{done.p, dzero.p, dzero.p}, {dzero.p, done.p, dzero.p}, {dzero.p, dzero.p, done.p}};
+    double* outK[3] = {dKl.p, dKm.p, dKr.p};
+    for (int q = 0; q < 3; ++q) {
+      if (np == 10)
+        hipLaunchKernelGGL(elem_k<10>, dim3((unsigned)ns), dim3(64), 0, s, ns, dn.p, dse.p, dvs.p, unit[q][0],
+                           unit[q][1], unit[q][2], g ? dsg.p : nullptr, dM.p, dD.p, outK[q], dMs.p, dbad.p + 2);
+      else
+        hipLaunchKernelGGL(elem_k<4>, dim3((unsigned)ns), dim3(64), 0, s, ns, dn.p, dse.p, dvs.p, unit[q][0],
+                           unit[q][1], unit[q][2], g ? dsg.p : nullptr, dM.p, dD.p, outK[q], dMs.p, dbad.p + 2);
+      NMG_HIP(hipGetLastError());
+    }
+  }
+  int badh[3];
+  NMG_HIP(hipMemcpyAsync(badh, dbad.p, sizeof(badh), hipMemcpyDeviceToHost, s));
+  NMG_HIP(hipStreamSynchronize(s));
+  NMG_CHECK(badh[0] == big[0], NMG_EINVAL, "flat element " + std::to_string(badh[0]));
+  NMG_CHECK(badh[2] == big[0], NMG_EINVAL, "flat element " + std::to_string(slist[badh[2] < ns ? badh[2] : 0]));
+  NMG_CHECK(badh[1] == big[0], NMG_EINVAL,
+            "fluid element " + std::to_string(badh[1]) +
+                " lies on the free surface: its surface term divides by the mean |g.n|, which needs gravity (JOB=2)");
+  if (nf > 0) {
+    hipLaunchKernelGGL(fs_dfluid_k, dim3((unsigned)((2 * nf + 63) / 64)), dim3(64), 0, s, B, nf, dfl.p, dko.p,
+                       dsc.p, dFr.p, dFl.p, dSr.p, dSl.p);
+    NMG_HIP(hipGetLastError());
+  }
+  fs::Adj A;
+  A.vstt = dvt.p;
+  A.pstt = dpt.p;
+  A.info = dinfo.p;
+  A.koff = dko.p;
+  A.scm = dsc.p;
+  A.dFr = dFr.p;
+  A.dFl = dFl.p;
+  A.dSr = dSr.p;
+  A.dKl = dKl.p;
+  A.dKm = dKm.p;
+  A.dKr = dKr.p;
+  A.n_dof = n_dof;
+  A.n_p = n_p;
+  A.k = k;
+  A.eig = deig.p;
+  A.U = dU.p;
+  A.P = dP.p;
+  A.d_rho = o_r.p;
+  A.d_lam = o_l.p;
+  A.d_mu = o_m.p;
+  const int64_t nt = n_elem * (int64_t)k;
+  hipLaunchKernelGGL(fs_adjoint_k, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, s, B, A);
+  NMG_HIP(hipGetLastError());
+  NMG_HIP(hipMemcpyAsync(d_rho, o_r.p, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+  NMG_HIP(hipMemcpyAsync(d_lam, o_l.p, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+  NMG_HIP(hipMemcpyAsync(d_mu, o_m.p, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+  NMG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace nmg

@@ -154,19 +154,50 @@ NMG_HD inline int element_info(const Body& M, int64_t e, Info* out) {
   return 0;
 }
 
-// least-squares gradient of the nodal density over element e
-NMG_HD inline void rho_gradient(const Body& M, int64_t e, double grad[3]) {
+// Forward-mode dual number: value and derivative along one parameter.  The
+// fluid element below is written once over its scalar type; with Dual it
+// yields the exact derivative of every entry (the adjoint's dFT/dm).
+struct Dual {
+  double v, d;
+};
+NMG_HD inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+NMG_HD inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+NMG_HD inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+NMG_HD inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+NMG_HD inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+NMG_HD inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+NMG_HD inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+NMG_HD inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+NMG_HD inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+NMG_HD inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+NMG_HD inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+NMG_HD inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+NMG_HD inline Dual operator/(double a, Dual b) { return {a / b.v, -a * b.d / (b.v * b.v)}; }
+NMG_HD inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+NMG_HD inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+NMG_HD inline double root(double a) { return ::sqrt(a); }
+NMG_HD inline Dual root(Dual a) {
+  const double s = ::sqrt(a.v);
+  return {s, a.d / (2.0 * s)};
+}
+NMG_HD inline double value(double a) { return a; }
+NMG_HD inline double value(Dual a) { return a.v; }
+
+// least-squares gradient of the nodal density f over element e
+template <class T>
+NMG_HD inline void rho_gradient(const Body& M, int64_t e, const T* f, T grad[3]) {
   const int n = M.np;
   const int32_t* el = M.elems + e * n;
-  const double* f = M.rho + e * n;
-  double xm[3] = {0, 0, 0}, fm = 0.0;
+  double xm[3] = {0, 0, 0};
+  T fm = T{0.0};
   for (int a = 0; a < n; ++a) {
     for (int c = 0; c < 3; ++c) xm[c] += M.nodes[(int64_t)el[a] * 3 + c];
     fm += f[a];
   }
   for (int c = 0; c < 3; ++c) xm[c] /= n;
-  fm /= n;
-  double N[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, R[3] = {0, 0, 0};
+  fm = fm / (double)n;
+  double N[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  T R[3] = {T{0.0}, T{0.0}, T{0.0}};
   for (int a = 0; a < n; ++a)
     for (int r = 0; r < 3; ++r) {
       const double xr = M.nodes[(int64_t)el[a] * 3 + r] - xm[r];
@@ -179,26 +210,27 @@ NMG_HD inline void rho_gradient(const Body& M, int64_t e, double grad[3]) {
 }
 
 // Fluid element matrix FT ((4np)^2, row-major) and, for an interface
-// element with gravity, the SCM block ((3nfp)^2) into SCM (may be null).
-// D_i = sum_r invJ(i,r) Dr_r is formed entry by entry where it is used.
-NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* SCM) {
+// element with gravity, the SCM block ((3nfp)^2) into SCM (may be null), for
+// the element's nodal lam and rho (of scalar type T).  D_i = sum_r invJ(i,r)
+// Dr_r is formed entry by entry where it is used.
+template <class T>
+NMG_HD inline void fluid_element_t(const Body& M, int64_t e, const T* lam, const T* rho, T* FT, T* SCM) {
   const int n = M.np, N2 = n * n, L = 3 * n, LF = L + n, nfp = M.nfp;
   double X[12], invJ[9], detJ;
   geometry(M, e, X, invJ, &detJ);
-  const double* lam = M.lam + e * n;
-  const double* rho = M.rho + e * n;
   const double* g = M.g ? M.g + e * n * 3 : nullptr;
   const double* Mr = M.refM;
-  for (int t = 0; t < LF * LF; ++t) FT[t] = 0.0;
+  for (int t = 0; t < LF * LF; ++t) FT[t] = T{0.0};
   if (g) {
-    double gr[3];
-    rho_gradient(M, e, gr);
-    double gh[3][kMaxNp], gm[3][kMaxNp];
+    T gr[3];
+    rho_gradient(M, e, rho, gr);
+    double gh[3][kMaxNp];
+    T gm[3][kMaxNp];
     for (int a = 0; a < n; ++a) {
       const double* ga = g + a * 3;
-      const double ng = sqrt(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
-      double n2 = (gr[0] * ga[0] + gr[1] * ga[1] + gr[2] * ga[2]) / rho[a] - ng * ng / lam[a] * rho[a];
-      if (ng < 1e-15 || M.purefluid) n2 = 0.0;
+      const double ng = ::sqrt(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+      T n2 = (gr[0] * ga[0] + gr[1] * ga[1] + gr[2] * ga[2]) / rho[a] - ng * ng / lam[a] * rho[a];
+      if (ng < 1e-15 || M.purefluid) n2 = T{0.0};
       for (int i = 0; i < 3; ++i) {
         gh[i][a] = ga[i] / (ng > 1e-15 ? ng : 1e-15);
         gm[i][a] = gh[i][a] * n2 * rho[a];
@@ -216,21 +248,22 @@ NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* S
   for (int i = 0; i < 3; ++i)
     for (int a = 0; a < n; ++a)
       for (int b = 0; b < n; ++b) {
-        double s = 0.0;
+        double s0 = 0.0;
         for (int c = 0; c < n; ++c) {
           const int cb = c * n + b;
           const double d = invJ[i * 3 + 0] * M.refD[0 * N2 + cb] + invJ[i * 3 + 1] * M.refD[1 * N2 + cb] +
                            invJ[i * 3 + 2] * M.refD[2 * N2 + cb];
-          s += Mr[a * n + c] * d;
+          s0 += Mr[a * n + c] * d;
         }
-        if (g) s -= Mr[a * n + b] * (rho[b] / lam[b] * g[b * 3 + i]);
+        T s = T{s0};
+        if (g) s = s - Mr[a * n + b] * (rho[b] / lam[b] * g[b * 3 + i]);
         FT[(i * n + a) * LF + L + b] = s * detJ;
         FT[(L + b) * LF + i * n + a] = s * detJ;
       }
   // pressure block
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b) {
-      const double la = 1.0 / sqrt(lam[a]), lb = 1.0 / sqrt(lam[b]);
+      const T la = 1.0 / root(lam[a]), lb = 1.0 / root(lam[b]);
       FT[(L + a) * LF + L + b] = -detJ * (la * Mr[a * n + b] * lb);
     }
   // vertex statuses
@@ -242,16 +275,16 @@ NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* S
     const double* MF = M.massF + f * nfp * nfp;
     double nf[3], sjf;
     face_normal(X, f, nf, &sjf);
-    double srho = 0.0;
+    T srho = T{0.0};
     for (int a = 0; a < nfp; ++a) srho += rho[fm[a]];
-    srho /= nfp;
+    srho = srho / (double)nfp;
     if (M.neigh[e * 4 + f] < 0) {
       if (!g) continue;   // rejected by element_info before this runs
       double sg = 0.0;
       for (int a = 0; a < nfp; ++a) {
         double gn = 0.0;
         for (int c = 0; c < 3; ++c) gn -= g[fm[a] * 3 + c] * nf[c];
-        sg += sqrt(gn * gn);
+        sg += ::sqrt(gn * gn);
       }
       sg /= nfp;
       for (int a = 0; a < nfp; ++a)
@@ -286,9 +319,9 @@ NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* S
     const double* MF = M.massF + f * nfp * nfp;
     double nf[3], sjf;
     face_normal(X, f, nf, &sjf);
-    double rf = 0.0;
+    T rf = T{0.0};
     for (int a = 0; a < nfp; ++a) rf += rho[fm[a]];
-    rf /= nfp;
+    rf = rf / (double)nfp;
     const int Ls = 3 * nfp;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
@@ -299,6 +332,122 @@ NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* S
             SCM[(i * nfp + a) * Ls + j * nfp + b] = t * rf;
           }
   }
+}
+
+NMG_HD inline void fluid_element(const Body& M, int64_t e, double* FT, double* SCM) {
+  fluid_element_t<double>(M, e, M.lam + e * M.np, M.rho + e * M.np, FT, SCM);
+}
+
+// Adjoint of a fluid-solid body: d lambda_j / d m_e = w^T (dK_e/dm - lambda_j
+// dM_e/dm) w for m in {rho, lam, mu} changed uniformly at the nodes of
+// element e, with w = (u, p) the element's displacement and pressure values
+// of pair j, u B-normalised and p = -Ap^-1 ET u (the pressures the Schur
+// complement eliminates).  Fluid elements: dFT/dm by dual numbers through
+// fluid_element_t (dFr, dFl: value and derivative pairs), minus the
+// interface block's dSCM/drho on the solid triples, and dB/drho = M detJ
+// (element-average density) on the fluid triples; d/dmu is 0 there.  Solid
+// elements: dK/dm is the element stiffness at unit lam, mu or rho (the
+// gravity term is linear in rho) on the solid triples, dB/drho = M detJ.
+struct Adj {
+  const int64_t* vstt;
+  const int64_t* pstt;
+  const Info* info;
+  const int64_t* koff;   // solid: into the unit-parameter stiffness pools; fluid: into the dual pools
+  const int64_t* scm;    // interface block offset into dSr, or -1
+  const Dual* dFr;       // fluid dFT/drho and dFT/dlam, (4np)^2 each at koff
+  const Dual* dFl;
+  const Dual* dSr;       // interface dSCM/drho, (3nfp)^2 at scm
+  const double* dKl;     // solid stiffness at lam = 1, mu = 1, rho = 1 (the others 0), (3np)^2 at koff
+  const double* dKm;
+  const double* dKr;
+  int64_t n_dof, n_p;
+  int k;
+  const double* eig;     // [k]
+  const double* U;       // [k * n_dof], pair-major
+  const double* P;       // [k * n_p]
+  double *d_rho, *d_lam, *d_mu;   // [k * n_elem]
+};
+
+NMG_HD inline void element_adjoint(const Body& M, const Adj& A, int64_t e, int j) {
+  const int n = M.np, L = 3 * n, LF = L + n, nfp = M.nfp, Ls = 3 * nfp;
+  const int32_t* el = M.elems + e * n;
+  const double* u = A.U + (int64_t)j * A.n_dof;
+  const double* pr = A.P + (int64_t)j * A.n_p;
+  const double lam_j = A.eig[j], detJ = A.info[e].detJ;
+  const int64_t out = (int64_t)j * M.n_elem + e;
+  if (!M.efluid[e]) {
+    // local displacement i*n + a: the solid triple of node a
+    double ul[3 * kMaxNp];
+    for (int r = 0; r < L; ++r) ul[r] = u[A.vstt[el[r % n]] + r / n];
+    double ql = 0.0, qm = 0.0, qr = 0.0, mb = 0.0;
+    const double* Kl = A.dKl + A.koff[e];
+    const double* Km = A.dKm + A.koff[e];
+    const double* Kr = A.dKr + A.koff[e];
+    for (int r = 0; r < L; ++r) {
+      double sl = 0.0, sm = 0.0, sr = 0.0;
+      for (int c = 0; c < L; ++c) {
+        sl += Kl[r * L + c] * ul[c];
+        sm += Km[r * L + c] * ul[c];
+        sr += Kr[r * L + c] * ul[c];
+      }
+      ql += ul[r] * sl;
+      qm += ul[r] * sm;
+      qr += ul[r] * sr;
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < n; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < n; ++b) s += M.refM[a * n + b] * ul[i * n + b];
+        mb += ul[i * n + a] * s;
+      }
+    A.d_lam[out] = ql;
+    A.d_mu[out] = qm;
+    A.d_rho[out] = qr - lam_j * detJ * mb;
+    return;
+  }
+  // local row r < 3n: fluid triple component r / n of node r % n; r >= 3n: pressure of node r - 3n
+  double w[4 * kMaxNp];
+  for (int r = 0; r < L; ++r) {
+    const int64_t v = el[r % n];
+    w[r] = u[A.vstt[v] + vnum(M.vstat[v]) - 3 + r / n];
+  }
+  for (int a = 0; a < n; ++a) w[L + a] = pr[A.pstt[el[a]]];
+  const Dual* Fr = A.dFr + A.koff[e];
+  const Dual* Fl = A.dFl + A.koff[e];
+  double qr = 0.0, ql = 0.0, mb = 0.0;
+  for (int r = 0; r < LF; ++r) {
+    double sr = 0.0, sl = 0.0;
+    for (int c = 0; c < LF; ++c) {
+      sr += Fr[r * LF + c].d * w[c];
+      sl += Fl[r * LF + c].d * w[c];
+    }
+    qr += w[r] * sr;
+    ql += w[r] * sl;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < n; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < n; ++b) s += M.refM[a * n + b] * w[i * n + b];
+      mb += w[i * n + a] * s;
+    }
+  if (A.scm[e] >= 0) {
+    // the interface block enters the solid triples of the face's nodes with a minus sign
+    const int32_t* fm = M.fmask + A.info[e].fcid * nfp;
+    const Dual* S = A.dSr + A.scm[e];
+    double q = 0.0;
+    for (int p = 0; p < 3; ++p)
+      for (int mi = 0; mi < nfp; ++mi) {
+        const double up = u[A.vstt[el[fm[mi]]] + p];
+        double s = 0.0;
+        for (int qq = 0; qq < 3; ++qq)
+          for (int ni = 0; ni < nfp; ++ni) s += S[(p * nfp + mi) * Ls + qq * nfp + ni].d * u[A.vstt[el[fm[ni]]] + qq];
+        q += up * s;
+      }
+    qr -= q;
+  }
+  A.d_rho[out] = qr - lam_j * detJ * mb;
+  A.d_lam[out] = ql;
+  A.d_mu[out] = 0.0;
 }
 
 // the fluid-solid pattern and the element pools the gather reads

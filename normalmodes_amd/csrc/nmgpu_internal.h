@@ -240,6 +240,12 @@ void assemble_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const 
                         double* b_val, const int64_t* e_ptr, int32_t* e_col, double* e_val, const int64_t* et_ptr,
                         int32_t* et_col, double* et_val, const int64_t* ap_ptr, int32_t* ap_col, double* ap_val,
                         int64_t* stats, hipStream_t s);
+void backward_fs_device(int64_t n_nodes, int64_t n_elem, int np, int nfp, const double* nodes, const int32_t* elems,
+                        const int32_t* vslot, const double* lam, const double* rho, const double* g,
+                        const int8_t* efluid, const int32_t* neigh, int purefluid, const double* refM,
+                        const double* refD, const double* massF, const int32_t* fmask, const int8_t* vstat,
+                        const int64_t* vstt, const int64_t* pstt, int k, const double* eig, const double* U,
+                        const double* P, double* d_rho, double* d_lam, double* d_mu, hipStream_t s);
 void k_add_f32(int64_t n, const float* x, double* y, hipStream_t s);     // y += x
 void k_block_cheb_step(int K, const DevMat& M, const double* b, const double* xk, double* xm, double w,
                        double ith, hipStream_t s, bool fp32 = false);

@@ -2660,7 +2660,8 @@ static int backward_impl(nmg_handle h, int64_t n_elem, int np, const int32_t* el
     Solver* S = get<Solver>(h, K_SOLVER);
     NMG_CHECK(S->nev > 0 && S->X, NMG_ESTATE, "no eigenpairs: run nmg_cheblannr first");
     NMG_CHECK(S->dscale.n >= S->n, NMG_ESTATE, "the operator was not set with nmg_set_amv_dad");
-    NMG_CHECK(!S->fs, NMG_ENOTIMPL, "sensitivities of fluid-solid bodies are not available");
+    NMG_CHECK(!S->fs, NMG_ENOTIMPL,
+              "a fluid-solid body's sensitivities come from nmg_backward_fs (its element layout and pressures)");
     NMG_CHECK(np == 4 || np == 10, NMG_EINVAL, "np must be 4 (P1) or 10 (P2)");
     NMG_CHECK(n_elem > 0 && elems && detJ && invJ && refM && refD && d_rho && d_lam && d_mu, NMG_EINVAL,
               "null argument");

@@ -439,6 +439,30 @@ def run_fluid(args):
     print(json.dumps(line), flush=True)
 
 
+def producer_compare(Q, device=True):
+    """The fluid-solid producer on the GPU (nmg_assemble_fs) against the CPU
+    one on the same body and materials: seconds of each (host pattern work
+    included in both) and the largest difference relative to each block's
+    largest entry.  ``device=False`` runs the CPU producer on both sides."""
+    from normalmodes_amd.assemble import assemble_fs
+    vp = np.sqrt((Q.lam + 2 * Q.mu) / Q.rho)
+    vs = np.sqrt(Q.mu / Q.rho)
+    kw = dict(rho=Q.rho, vp=vp, vs=vs, g0=Q.g0, job=Q.job)
+    t0 = time.perf_counter()
+    D = assemble_fs(Q.mesh, device=device, **kw)
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    C = assemble_fs(Q.mesh, **kw)
+    t_cpu = time.perf_counter() - t0
+    same = all(np.array_equal(getattr(D, k).col, getattr(C, k).col) and
+               np.array_equal(getattr(D, k).rowptr, getattr(C, k).rowptr) for k in ("Ad", "B", "E", "ET", "Ap"))
+    err = max(float(np.abs(getattr(D, k).val - getattr(C, k).val).max() / np.abs(getattr(C, k).val).max())
+              for k in ("Ad", "B", "E", "ET", "Ap"))
+    return dict(gpu_s=round(t_dev, 3), cpu_s=round(t_cpu, 3),
+                cpu_threads=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), pattern_same=same,
+                max_rel_diff=err)
+
+
 def fluid_pieces(args):
     """Measured pieces of the fluid-solid ball (c2): assembly and set-up,
     LanBounds, the FindPol filter the reference's rule picks for the band,
@@ -462,6 +486,11 @@ def fluid_pieces(args):
                nnz={k: getattr(Q, k).nnz for k in ("Ad", "B", "E", "ET", "Ap")},
                build_s=round(t_build, 2), setup_s=round(t_setup, 2), bsolve_degree=mv.bdeg, ap_degree=mv.fs.apdeg,
                band_mhz=list(band))
+    try:
+        out["producer"] = producer_compare(Q)
+        log(f"[bench] c2 producer: {out['producer']}")
+    except Exception as e:
+        out["producer"] = {"error": str(e)}
     try:
         h = driver._open_solver(mv, precision=pv.FP64, seed=7, rtol=1e-9)
         try:

@@ -23,7 +23,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
-__all__ = ["element_geometry", "element_gravity", "backward", "backward_fs", "element_matrices"]
+__all__ = ["element_geometry", "element_gravity", "backward", "backward_fs", "pressure_solve", "element_matrices"]
 
 
 def element_geometry(mesh):
@@ -121,16 +121,51 @@ def element_matrices(mesh, sel, lam, mu, rho, g=None):
     return K, Me
 
 
+def pressure_solve(Ap, R, tol: float = 1e-13, maxit: int = 5000):
+    """Ap^-1 R for the negative definite pressure block Ap (CSR) and every
+    column of R: conjugate gradients on -Ap with its diagonal as the
+    preconditioner, one recurrence per column advanced together.  Raises
+    when a column stays above 1e-10 relative residual."""
+    A = -Ap
+    B = -np.asarray(R, dtype=np.float64).reshape(Ap.shape[0], -1)
+    d = A.diagonal()[:, None]
+    bn = np.linalg.norm(B, axis=0)
+    X = np.zeros_like(B)
+    Rr = B.copy()
+    Z = Rr / d
+    P = Z.copy()
+    rz = np.einsum("ij,ij->j", Rr, Z)
+    active = bn > 0
+    for _ in range(maxit):
+        if not active.any():
+            break
+        AP = A @ P
+        pap = np.einsum("ij,ij->j", P, AP)
+        alpha = np.where(active, rz / np.where(active, pap, 1.0), 0.0)
+        X += P * alpha
+        Rr -= AP * alpha
+        active &= np.linalg.norm(Rr, axis=0) > tol * bn
+        Z = Rr / d
+        rz_new = np.einsum("ij,ij->j", Rr, Z)
+        beta = np.where(active, rz_new / np.where(rz > 0, rz, 1.0), 0.0)
+        P = Z + P * beta
+        rz = rz_new
+    res = np.linalg.norm(A @ X - B, axis=0) / np.where(bn > 0, bn, 1.0)
+    if np.any(res > 1e-10):
+        raise RuntimeError(f"pressure solve: relative residual {res.max():.1e} after {maxit} iterations")
+    return X
+
+
 def backward_fs(Q, lam, X, purefluid: bool = False):
     """Sensitivities of eigenpairs of a fluid-solid body (nmg_backward_fs).
 
     ``Q``: the FluidProblem the pairs belong to; ``lam`` (k,) eigenvalues of
     (Ad - E Ap^-1 ET) u = lambda B u; ``X`` (n, k) the physical displacement
-    vectors (B-normalised here).  The pressures p = -Ap^-1 ET u come from a
-    sparse factorisation of Ap on the host.  Returns (d_rho, d_lam, d_mu),
+    vectors (B-normalised here).  The pressures p = -Ap^-1 ET u come from
+    Jacobi-preconditioned conjugate gradients on -Ap (negative definite) on
+    the host, all columns at once.  Returns (d_rho, d_lam, d_mu),
     each (k, n_elem), for rho, lam (Lame) and mu changed uniformly at the
     nodes of each element; d_mu is 0 on fluid elements."""
-    import scipy.sparse.linalg as spl
     from .assemble import face_neighbours
     from .refelem import reference_faces
     L = load_gpu()
@@ -142,7 +177,7 @@ def backward_fs(Q, lam, X, purefluid: bool = False):
         raise ValueError("one eigenvalue per vector")
     B = Q.B.to_scipy()
     X = X / np.sqrt(np.einsum("ij,ij->j", X, B @ X))
-    Pm = -spl.splu(Q.Ap.to_scipy().tocsc()).solve(np.asarray(Q.ET.to_scipy() @ X))
+    Pm = -pressure_solve(Q.Ap.to_scipy(), np.asarray(Q.ET.to_scipy() @ X))
     U = np.ascontiguousarray(X.T)
     P = np.ascontiguousarray(Pm.T)
     npl = me.elems.shape[1]

@@ -111,3 +111,21 @@ def test_fluid_adjoint_scaling_identities_p2():
     assert e1 < 1e-9 and e2 < 1e-9
     # the fluid elements carry part of both sums
     assert np.abs(d_lam[:, Q.fluid]).max() > 1e-6 * np.abs(d_lam).max()
+
+
+def test_pressure_solve_matches_direct_solve():
+    """The host pressure solve p = Ap^-1 r used by the fluid-solid adjoint
+    (Jacobi-preconditioned CG, all columns at once) against a sparse direct
+    solve, with a zero column among the right-hand sides."""
+    import scipy.sparse.linalg as spl
+    from normalmodes_amd.adjoint import pressure_solve
+    from normalmodes_amd.assemble import assemble_fs
+    me, rho, vp, vs, g0 = _body("core-p1-job2")
+    Q = assemble_fs(me, rho=rho, vp=vp, vs=vs, g0=g0, job=2)
+    Ap = Q.Ap.to_scipy()
+    R = np.random.default_rng(4).standard_normal((Q.n_p, 3))
+    R[:, 1] = 0.0
+    X = pressure_solve(Ap, R)
+    ref = spl.spsolve(Ap.tocsc(), R)
+    assert np.all(X[:, 1] == 0.0)
+    assert np.max(np.abs(X - ref)) <= 1e-11 * np.max(np.abs(ref))

@@ -222,6 +222,41 @@ module nmgpu_f90
       real(c_double), intent(in) :: detJ(*), invJ(*), refM(*), refD(*), gnode(*), gsym(*)
       real(c_double), intent(out) :: d_rho(*), d_lam(*), d_mu(*)
     end function
+    ! device producer of a fluid-solid body (CGFSE3D_ISO,
+    ! src/mod_cg_create_matrix.f90:103-977): Ad, B, E, ET, Ap on the layout of
+    ! matrixstruct_general (:1458-2033), 0-based ids and row offsets; pass
+    ! c_null_ptr for g without gravity and for stats when not wanted
+    integer(c_int) function nmg_assemble_fs(n_nodes, n_elem, np, nfp, nodes, elems, vslot, lam, mu, rho, g, &
+        efluid, neigh, purefluid, refM, refD, massF, fmask, adj_ptr, adj, vstat, vstt, pstt, &
+        ad_ptr, ad_col, ad_val, b_ptr, b_col, b_val, e_ptr, e_col, e_val, et_ptr, et_col, et_val, &
+        ap_ptr, ap_col, ap_val, stats) bind(C, name="nmg_assemble_fs")
+      import :: c_int, c_int8_t, c_int32_t, c_int64_t, c_double, c_ptr
+      integer(c_int64_t), value :: n_nodes, n_elem
+      integer(c_int), value :: np, nfp, purefluid
+      real(c_double), intent(in) :: nodes(*), lam(*), mu(*), rho(*), refM(*), refD(*), massF(*)
+      type(c_ptr), value :: g, stats
+      integer(c_int32_t), intent(in) :: elems(*), vslot(*), neigh(*), fmask(*), adj(*)
+      integer(c_int8_t), intent(in) :: efluid(*), vstat(*)
+      integer(c_int64_t), intent(in) :: adj_ptr(*), vstt(*), pstt(*), ad_ptr(*), b_ptr(*), e_ptr(*), et_ptr(*), &
+          ap_ptr(*)
+      integer(c_int32_t), intent(out) :: ad_col(*), b_col(*), e_col(*), et_col(*), ap_col(*)
+      real(c_double), intent(out) :: ad_val(*), b_val(*), e_val(*), et_val(*), ap_val(*)
+    end function
+    ! adjoint of a fluid-solid body for k pairs: eig(k), U(n_dof, k) physical
+    ! and B-normalised, P(n_p, k) = -Ap^-1 ET U; outputs (n_elem, k)
+    integer(c_int) function nmg_backward_fs(n_nodes, n_elem, np, nfp, nodes, elems, vslot, lam, rho, g, &
+        efluid, neigh, purefluid, refM, refD, massF, fmask, vstat, vstt, pstt, k, eig, U, P, &
+        d_rho, d_lam, d_mu) bind(C, name="nmg_backward_fs")
+      import :: c_int, c_int8_t, c_int32_t, c_int64_t, c_double, c_ptr
+      integer(c_int64_t), value :: n_nodes, n_elem
+      integer(c_int), value :: np, nfp, purefluid, k
+      real(c_double), intent(in) :: nodes(*), lam(*), rho(*), refM(*), refD(*), massF(*), eig(*), U(*), P(*)
+      type(c_ptr), value :: g
+      integer(c_int32_t), intent(in) :: elems(*), vslot(*), neigh(*), fmask(*)
+      integer(c_int8_t), intent(in) :: efluid(*), vstat(*)
+      integer(c_int64_t), intent(in) :: vstt(*), pstt(*)
+      real(c_double), intent(out) :: d_rho(*), d_lam(*), d_mu(*)
+    end function
   end interface
 
 contains
